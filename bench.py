@@ -1,0 +1,162 @@
+#!/usr/bin/env python
+"""bench.py - pixel-iterations/s of newman's per-pixel deep-zoom iteration on one MI355X.
+
+Workload (default): the deep-zoom FP64-delta configuration - 1920x1080 output with
+4x4 multisampling (7680x4320 samples), view width 1e-30, N = 20000, bailout 1024,
+series tolerance 1e-10, centre from the seed-1 zoom walk (tests/golden/views.json).
+One step = one pass of the hot path over the frame: the series-start kernel plus
+the persistent FP64 perturbation kernel, with the reference orbit and series
+already resident in HBM (the host HP precompute runs once, before timing).
+
+value = credited pixel-iterations per second: the sum of reported escape counts
+over samples not resolved by the cardioid/bulb test, series-skipped iterations
+included, divided by wall time per step.  roofline = algorithmic FP64 FLOPs of the
+iteration kernel (15 per executed perturbation step) over its HIP-event duration,
+against the FP64 vector peak.  cpu_baseline = the CPU oracle (the reference method:
+series scan + HP continuation) on a row subsample of the same frame.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pixel-iterations/s on 1 MI355X: plain FP64 vs FP32-floatexp+learned-skip"
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+FLOP_PER_PERT_STEP = 15  # z = X+d (2), |z|^2 (3), w = X+z (2), d*w + e (8)
+FLOP_PER_PLAIN_STEP = 8  # 3 mul + 5 add, |z|^2 reusing re^2, im^2
+
+
+def load_views():
+    with open(os.path.join(ROOT, "tests", "golden", "views.json")) as f:
+        return json.load(f)
+
+
+def make_workload(name):
+    import newman_amd as nm
+    views = load_views()
+    if name == "c2":
+        v = views["c2_seed1"]
+        m = nm.Mandelbrot(1080 * 4, 1920 * 4)
+        m.set_view(v["N"], v["re"], v["im"], v["width"])
+        desc = {"workload": "deep zoom, FP64 deltas: 1920x1080 output x 4x4 multisampling (7680x4320 samples), "
+                            "width 1e-30, N 20000, bailout 1024, tol 1e-10, centre = zoom walk seed 1",
+                "samples": 7680 * 4320, "N": 20000, "width": "1e-30", "precision_bits": m.precision_bits()}
+        flop = FLOP_PER_PERT_STEP
+    elif name == "c1":
+        # config 1 is ~2 us of work: B repeats of the frame in one launch set
+        m = nm.Mandelbrot(480, 640)
+        m.set_view(1024, "-0.75", "0", "3.0")
+        desc = {"workload": "plain double path: 640x480, centre -0.75+0i, width 3.0, N 1024, bailout 1024",
+                "samples": 640 * 480, "N": 1024}
+        flop = FLOP_PER_PLAIN_STEP
+    else:
+        raise SystemExit("unknown config " + name)
+    return m, desc, flop
+
+
+def cpu_baseline(desc, name, nthreads, rows_every):
+    """The oracle (reference method, HP continuation) on a row subsample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as O
+    views = load_views()
+    if name != "c2":
+        return None
+    v = views["c2_seed1"]
+    nr, nc = 1080 * 4, 1920 * 4
+    o = O.OView(nr, nc)
+    o.set_view(v["N"], v["re"], v["im"], v["width"])
+    t0 = time.perf_counter()
+    o.precompute(nthreads)
+    t_pre = time.perf_counter() - t0
+    rows = np.arange(rows_every // 2, nr, rows_every, dtype=np.int32)
+    t0 = time.perf_counter()
+    out = o.render_rows(rows, nthreads)
+    t_run = time.perf_counter() - t0
+    it = out["iterations"].astype(np.int64)
+    credited = int(it.sum())  # deep frame: no cardioid samples
+    return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
+            "sample": "oracle reference method (series scan + HP continuation, %d-bit) on rows %d::%d of the same "
+                      "7680x4320 frame (%d samples, %.3g credited px-it, %.1f s); precompute (findProbe over "
+                      "13680 probes + series) %.1f s, not included" % (
+                          o.bits(), rows_every // 2, rows_every, out.size, credited, t_run, t_pre)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--block-k", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows-every", type=int, default=1080)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the frame would shard by "
+              "rows/tiles with the reference orbit broadcast once; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import newman_amd as nm
+    m, desc, flop = make_workload(args.config)
+    t0 = time.perf_counter()
+    if not m.useHardware():
+        m.precompute()
+    t_pre = time.perf_counter() - t0
+    print("# precompute %.2f s (host HP: findProbe + series), len(X) = %s" % (t_pre, m.reference_length()),
+          file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        m.render(args.block_k, args.waves, 1, copy_back=False)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(m.render(args.block_k, args.waves, 1, copy_back=False))
+    wall = (time.perf_counter() - t0) / args.steps
+    s = stats[-1]
+    credited = s["credited"]
+    value = credited / wall
+    ms_iter = sum(x["ms_iterate"] for x in stats) / len(stats)
+    achieved = flop * s["executed"] / (ms_iter * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pixel-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded zoom-walk view)",
+        "config": dict(desc, parallelism="single GPU", block_k=s["block_k"], waves_per_simd=s["waves_per_simd"]),
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_series_iterate" if flop == FLOP_PER_PERT_STEP else "k_plain_iterate",
+                     "flop_per_step": flop, "executed_steps_per_launch": s["executed"],
+                     "ms_per_launch": ms_iter},
+        "detail": {"credited_px_it": credited, "executed_px_it": s["executed"], "skipped": s["skipped"],
+                   "n_iterated": s["n_iterated"], "n_sa_resolved": s["n_sa_resolved"],
+                   "n_cardioid": s["n_cardioid"], "n_rebased": s["n_rebased"], "ms_start": s["ms_start"],
+                   "ms_kernels": s["ms_total"], "precompute_s": t_pre,
+                   "kernel_px_it_per_s": credited / (s["ms_total"] * 1e-3)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(desc, args.config, args.cpu_threads, args.cpu_rows_every)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

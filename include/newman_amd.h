@@ -1,0 +1,147 @@
+/* newman_amd.h - C ABI of the MI355X deep-zoom render core.
+ *
+ * Drop-in boundary for newman's render core, class Mandelbrot (mandelbrot.h:7-53):
+ * every entry point below replaces one member or public field of that class and
+ * cites it.  Entry points marked "extension" have no counterpart in the
+ * reference; they expose what the GPU engine adds (batched frames, the bailout
+ * parameter, the delta arithmetic, render statistics).
+ *
+ * Conventions: plain C types only; every function returns an int status (0 = ok,
+ * < 0 = error, see NM_ERR_*) and never throws; all calls are synchronous; output
+ * buffers are owned by the caller.  A handle is not thread-safe: use one handle per
+ * thread.  HP values cross the boundary as decimal strings ("-0.75",
+ * "0.123e-30"), as the reference's view files carry them (viewer.cpp:12-22).
+ */
+#ifndef NEWMAN_AMD_H
+#define NEWMAN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NM_OK 0
+#define NM_ERR_ARG (-1)      /* bad argument (sizes, N <= 0, malformed string) */
+#define NM_ERR_HIP (-2)      /* HIP runtime error (nm_last_error has the text) */
+#define NM_ERR_STATE (-3)    /* call order: e.g. deep view rendered before precompute */
+#define NM_ERR_NODEVICE (-4) /* no HIP device: there is no CPU fallback */
+#define NM_ERR_RANGE (-5)    /* frame set larger than the engine handles */
+#define NM_ERR_IO (-6)       /* view file missing or malformed */
+#define NM_ERR_NOMEM (-7)
+
+#define NM_DELTA_FP64 0  /* perturbation deltas in FP64 */
+#define NM_DELTA_FP32X 1 /* extension: FP32 mantissa + int32 exponent deltas */
+
+/* grid.h:8-15 RenderGrid::EscapeValue, 8 bytes, same layout */
+typedef struct nm_escape_value {
+  int32_t iterations;
+  float smoothing;
+} nm_escape_value;
+
+/* extension: counters of the last render */
+typedef struct nm_render_stats {
+  uint64_t samples;      /* samples in the frame set */
+  uint64_t n_cardioid;   /* resolved by the cardioid/bulb test (mandelbrot.cpp:63-71) */
+  uint64_t n_sa_resolved; /* resolved inside the series skip: escaped (mandelbrot.cpp:186-206) or skip reached N */
+  uint64_t n_iterated;   /* handed to the iteration kernel */
+  uint64_t executed;     /* iteration steps executed by the iteration kernel */
+  uint64_t skipped;      /* sum of series-skip start indices of iterated samples */
+  uint64_t n_rebased;    /* perturbation rebases */
+  uint64_t credited;     /* sum of reported iterations over non-cardioid samples */
+  double ms_start;       /* start kernel, ms (HIP events) */
+  double ms_iterate;     /* iteration kernel, ms, mean over repeats */
+  double ms_total;       /* both, ms, mean over repeats */
+  int32_t block_k;       /* iterations per block used */
+  int32_t waves_per_simd;
+} nm_render_stats;
+
+typedef struct nm_mandelbrot nm_mandelbrot; /* class Mandelbrot (mandelbrot.h:7) */
+
+/* Mandelbrot(int nr, int nc), mandelbrot.cpp:8-17: tol 1e-10, N 256, centre
+ * -0.5+0i, sz 4/nc x 3/nr, then setPrecision. */
+int nm_create(nm_mandelbrot** out, int nr, int nc);
+/* copy construction / assignment (viewer.cpp:193, 251 copy a Mandelbrot by value) */
+int nm_clone(nm_mandelbrot** out, const nm_mandelbrot* src);
+void nm_destroy(nm_mandelbrot* h);
+
+int nm_rows(const nm_mandelbrot* h);                 /* rows(), mandelbrot.h:33 */
+int nm_cols(const nm_mandelbrot* h);                 /* cols(), mandelbrot.h:34 */
+int nm_set_N(nm_mandelbrot* h, int N);               /* public field N, mandelbrot.h:24 */
+int nm_get_N(const nm_mandelbrot* h);
+int nm_set_error_tolerance(nm_mandelbrot* h, double tol); /* error_tolerance, mandelbrot.h:23 */
+double nm_get_error_tolerance(const nm_mandelbrot* h);
+/* public field center (mandelbrot.h:25): assignment keeps the field's precision,
+ * as mpf_class assignment does */
+int nm_set_center(nm_mandelbrot* h, const char* re, const char* im);
+int nm_get_center(const nm_mandelbrot* h, char* re, char* im, size_t cap);
+/* public field sz (mandelbrot.h:25) */
+int nm_set_size(nm_mandelbrot* h, const char* re, const char* im);
+int nm_get_size(const nm_mandelbrot* h, char* re, char* im, size_t cap);
+int nm_get_size_d(const nm_mandelbrot* h, double* re, double* im);
+/* setPrecision (mandelbrot.cpp:37-56, protected in the reference; callers that
+ * write sz directly call it to re-derive the precision) */
+int nm_set_precision(nm_mandelbrot* h);
+long nm_precision_bits(const nm_mandelbrot* h);
+
+int nm_use_hardware(nm_mandelbrot* h);        /* useHardware(), mandelbrot.cpp:256-259: 1/0 */
+int nm_precompute(nm_mandelbrot* h);          /* precompute(), mandelbrot.cpp:261-267 */
+int nm_compute_row(nm_mandelbrot* h, int r);  /* computeRow(int), mandelbrot.cpp:269-283 */
+/* computeRow for every row, one launch (viewer.cpp:166-171 loops computeRow) */
+int nm_compute_all(nm_mandelbrot* h);
+
+int nm_at(nm_mandelbrot* h, int r, int c, nm_escape_value* out);         /* at(r,c), mandelbrot.cpp:318 */
+int nm_at_avg(nm_mandelbrot* h, int r, int c, int sc, nm_escape_value* out); /* at(r,c,sc), :320-332 */
+/* the whole grid, row-major (RenderGrid::values, grid.h:18) */
+int nm_get_grid(nm_mandelbrot* h, nm_escape_value* out, size_t n);
+
+/* pointAt(r,c,sc), mandelbrot.cpp:285-290, as decimal strings */
+int nm_point_at(const nm_mandelbrot* h, int r, int c, int sc, char* re, char* im, size_t cap);
+int nm_translate(nm_mandelbrot* h, int dr, int dc, int sc);    /* mandelbrot.cpp:292-295 */
+int nm_zoom(nm_mandelbrot* h, float scale);                    /* mandelbrot.cpp:297-302 */
+int nm_zoom_at(nm_mandelbrot* h, float scale, int r, int c, int sc); /* mandelbrot.cpp:304-316 */
+int nm_scale_up(nm_mandelbrot* h, int sc);                     /* mandelbrot.cpp:334-346 */
+int nm_scale_down(nm_mandelbrot* h, int sc);                   /* mandelbrot.cpp:348-360 */
+int nm_load_legacy(nm_mandelbrot* h, const char* path);        /* mandelbrot.cpp:19-35 */
+/* the view-file writer of viewer.cpp:6-29 (N, centre, size as decimal strings) */
+int nm_save_view(const nm_mandelbrot* h, const char* path);
+
+/* ---- extensions ----------------------------------------------------------- */
+/* bailout radius; the reference's constant is 1024 (mandelbrot.cpp:58-59) */
+int nm_set_bailout(nm_mandelbrot* h, double bailout);
+/* NM_DELTA_FP64 (default) or NM_DELTA_FP32X */
+int nm_set_delta_mode(nm_mandelbrot* h, int mode);
+/* a view by width: centre strings and width = cols * sz.re (square samples),
+ * parsed at the precision the width needs, then setPrecision */
+int nm_set_view(nm_mandelbrot* h, int N, const char* cre, const char* cim, const char* width);
+/* precompute around an explicit reference point instead of findProbe */
+int nm_precompute_at(nm_mandelbrot* h, const char* re, const char* im);
+int nm_reference_length(const nm_mandelbrot* h);  /* len(X) */
+int nm_probe_winner(const nm_mandelbrot* h);      /* index of findProbe's winner, -1 if none */
+/* reference series as the device sees it: X (nearest), A, B, C (truncated),
+ * 2*len doubles each (re, im interleaved); any pointer may be NULL */
+int nm_get_reference(const nm_mandelbrot* h, double* X, double* A, double* B, double* C);
+/* render every row with explicit engine settings and `repeats` timed launches;
+ * copy_back = 0 leaves the grid on the device (benchmarks) */
+int nm_render(nm_mandelbrot* h, int block_k, int waves_per_simd, int repeats, int copy_back,
+              nm_render_stats* stats);
+/* render a batch of frames that share h's reference orbit (zoom video): frame
+ * k has h's centre and size width_k = cols * sz; outputs are concatenated
+ * row-major grids.  widths are decimal strings. */
+int nm_render_frames(nm_mandelbrot* h, const char* const* widths, int nframes, int block_k, int waves_per_simd,
+                     int repeats, nm_escape_value* out, nm_render_stats* stats);
+/* the text of the last engine error */
+const char* nm_last_error(void);
+
+/* HP arithmetic probe for tests: op 0 add, 1 sub, 2 mul, 3 mul by int, 4 divide by
+ * unsigned int, 5 parse, 6 mul by double; writes sign, exponent (value = sign *
+ * 0.m * 2^exp) and the mantissa as 32-bit words, most significant first.
+ * Returns the number of words. */
+int nm_hp_op(int op, const char* a, const char* b, long iv, double dv, long bits, int* sign, long* exp,
+             uint32_t* words);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,324 @@
+"""newman_amd: MI355X-native per-pixel deep-zoom iteration for newman.
+
+Python mirror of newman's render core class Mandelbrot (mandelbrot.h:7-53) over
+the C ABI in include/newman_amd.h.  Every method forwards to the native library
+(newman_amd/libnewman_amd.so, HIP kernels for gfx950); there is no CPU fallback:
+rendering without a HIP device raises NewmanError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnewman_amd.so")
+
+NM_OK = 0
+NM_ERR_ARG = -1
+NM_ERR_HIP = -2
+NM_ERR_STATE = -3
+NM_ERR_NODEVICE = -4
+NM_ERR_RANGE = -5
+NM_ERR_IO = -6
+NM_ERR_NOMEM = -7
+NM_DELTA_FP64 = 0
+NM_DELTA_FP32X = 1
+
+_ERRNAMES = {NM_ERR_ARG: "NM_ERR_ARG", NM_ERR_HIP: "NM_ERR_HIP", NM_ERR_STATE: "NM_ERR_STATE",
+             NM_ERR_NODEVICE: "NM_ERR_NODEVICE", NM_ERR_RANGE: "NM_ERR_RANGE", NM_ERR_IO: "NM_ERR_IO",
+             NM_ERR_NOMEM: "NM_ERR_NOMEM"}
+
+ESCAPE_DTYPE = np.dtype([("iterations", "<i4"), ("smoothing", "<f4")])  # grid.h:8-15
+
+
+class NewmanError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__("%s failed: %s (%d) %s" % (what, _ERRNAMES.get(code, "?"), code, last_error()))
+
+
+class EscapeValue(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("smoothing", C.c_float)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("n_cardioid", C.c_uint64), ("n_sa_resolved", C.c_uint64),
+                ("n_iterated", C.c_uint64), ("executed", C.c_uint64), ("skipped", C.c_uint64),
+                ("n_rebased", C.c_uint64), ("credited", C.c_uint64), ("ms_start", C.c_double),
+                ("ms_iterate", C.c_double), ("ms_total", C.c_double), ("block_k", C.c_int32),
+                ("waves_per_simd", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+# (name, restype, argtypes) for every declaration of include/newman_amd.h
+_H = C.c_void_p
+_SIGS = [
+    ("nm_create", C.c_int, [C.POINTER(_H), C.c_int, C.c_int]),
+    ("nm_clone", C.c_int, [C.POINTER(_H), _H]),
+    ("nm_destroy", None, [_H]),
+    ("nm_rows", C.c_int, [_H]),
+    ("nm_cols", C.c_int, [_H]),
+    ("nm_set_N", C.c_int, [_H, C.c_int]),
+    ("nm_get_N", C.c_int, [_H]),
+    ("nm_set_error_tolerance", C.c_int, [_H, C.c_double]),
+    ("nm_get_error_tolerance", C.c_double, [_H]),
+    ("nm_set_center", C.c_int, [_H, C.c_char_p, C.c_char_p]),
+    ("nm_get_center", C.c_int, [_H, C.c_char_p, C.c_char_p, C.c_size_t]),
+    ("nm_set_size", C.c_int, [_H, C.c_char_p, C.c_char_p]),
+    ("nm_get_size", C.c_int, [_H, C.c_char_p, C.c_char_p, C.c_size_t]),
+    ("nm_get_size_d", C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("nm_set_precision", C.c_int, [_H]),
+    ("nm_precision_bits", C.c_long, [_H]),
+    ("nm_use_hardware", C.c_int, [_H]),
+    ("nm_precompute", C.c_int, [_H]),
+    ("nm_compute_row", C.c_int, [_H, C.c_int]),
+    ("nm_compute_all", C.c_int, [_H]),
+    ("nm_at", C.c_int, [_H, C.c_int, C.c_int, C.POINTER(EscapeValue)]),
+    ("nm_at_avg", C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.POINTER(EscapeValue)]),
+    ("nm_get_grid", C.c_int, [_H, C.c_void_p, C.c_size_t]),
+    ("nm_point_at", C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_size_t]),
+    ("nm_translate", C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
+    ("nm_zoom", C.c_int, [_H, C.c_float]),
+    ("nm_zoom_at", C.c_int, [_H, C.c_float, C.c_int, C.c_int, C.c_int]),
+    ("nm_scale_up", C.c_int, [_H, C.c_int]),
+    ("nm_scale_down", C.c_int, [_H, C.c_int]),
+    ("nm_load_legacy", C.c_int, [_H, C.c_char_p]),
+    ("nm_save_view", C.c_int, [_H, C.c_char_p]),
+    ("nm_set_bailout", C.c_int, [_H, C.c_double]),
+    ("nm_set_delta_mode", C.c_int, [_H, C.c_int]),
+    ("nm_set_view", C.c_int, [_H, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p]),
+    ("nm_precompute_at", C.c_int, [_H, C.c_char_p, C.c_char_p]),
+    ("nm_reference_length", C.c_int, [_H]),
+    ("nm_probe_winner", C.c_int, [_H]),
+    ("nm_get_reference", C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("nm_render", C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(RenderStats)]),
+    ("nm_render_frames", C.c_int, [_H, C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                   C.POINTER(RenderStats)]),
+    ("nm_last_error", C.c_char_p, []),
+    ("nm_hp_op", C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_long, C.c_double, C.c_long,
+                           C.POINTER(C.c_int), C.POINTER(C.c_long), C.c_void_p]),
+]
+EXPORTED = [s[0] for s in _SIGS]
+
+
+def lib():
+    """Load the native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NewmanError(NM_ERR_STATE, "loading %s (run python -m newman_amd.build)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    if _lib is None:
+        return ""
+    s = _lib.nm_last_error()
+    return s.decode() if s else ""
+
+
+def _check(code, what):
+    if code < 0:
+        raise NewmanError(code, what)
+    return code
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+def hp_op(op, a, b=None, iv=0, dv=0.0, bits=64):
+    """Native HP arithmetic probe (see nm_hp_op): returns (sign, exp, words)."""
+    L = lib()
+    words = (C.c_uint32 * 256)()
+    sign = C.c_int()
+    exp = C.c_long()
+    n = _check(L.nm_hp_op(op, _b(a), _b(b) if b is not None else None, iv, dv, bits, C.byref(sign), C.byref(exp),
+                          words), "nm_hp_op")
+    return sign.value, exp.value, list(words[:n])
+
+
+class Mandelbrot:
+    """newman's render core (mandelbrot.h:7-53) on the MI355X engine."""
+
+    def __init__(self, nr=1, nc=1, _handle=None):
+        L = lib()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            h = _H()
+            _check(L.nm_create(C.byref(h), nr, nc), "nm_create")
+            self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.nm_destroy(self._h)
+            self._h = None
+
+    def copy(self):
+        h = _H()
+        _check(lib().nm_clone(C.byref(h), self._h), "nm_clone")
+        return Mandelbrot(_handle=h)
+
+    # ---- public fields (mandelbrot.h:23-25)
+    @property
+    def N(self):
+        return lib().nm_get_N(self._h)
+
+    @N.setter
+    def N(self, v):
+        _check(lib().nm_set_N(self._h, int(v)), "nm_set_N")
+
+    @property
+    def error_tolerance(self):
+        return lib().nm_get_error_tolerance(self._h)
+
+    @error_tolerance.setter
+    def error_tolerance(self, v):
+        _check(lib().nm_set_error_tolerance(self._h, float(v)), "nm_set_error_tolerance")
+
+    @property
+    def center(self):
+        re = C.create_string_buffer(4096)
+        im = C.create_string_buffer(4096)
+        _check(lib().nm_get_center(self._h, re, im, 4096), "nm_get_center")
+        return re.value.decode(), im.value.decode()
+
+    @center.setter
+    def center(self, v):
+        _check(lib().nm_set_center(self._h, _b(str(v[0])), _b(str(v[1]))), "nm_set_center")
+
+    @property
+    def sz(self):
+        re = C.create_string_buffer(4096)
+        im = C.create_string_buffer(4096)
+        _check(lib().nm_get_size(self._h, re, im, 4096), "nm_get_size")
+        return re.value.decode(), im.value.decode()
+
+    @sz.setter
+    def sz(self, v):
+        _check(lib().nm_set_size(self._h, _b(str(v[0])), _b(str(v[1]))), "nm_set_size")
+
+    def sz_d(self):
+        re, im = C.c_double(), C.c_double()
+        _check(lib().nm_get_size_d(self._h, C.byref(re), C.byref(im)), "nm_get_size_d")
+        return re.value, im.value
+
+    # ---- methods
+    def rows(self):
+        return lib().nm_rows(self._h)
+
+    def cols(self):
+        return lib().nm_cols(self._h)
+
+    def setPrecision(self):
+        _check(lib().nm_set_precision(self._h), "nm_set_precision")
+
+    def precision_bits(self):
+        return lib().nm_precision_bits(self._h)
+
+    def useHardware(self):
+        return bool(_check(lib().nm_use_hardware(self._h), "nm_use_hardware"))
+
+    def precompute(self):
+        _check(lib().nm_precompute(self._h), "nm_precompute")
+
+    def computeRow(self, r):
+        _check(lib().nm_compute_row(self._h, int(r)), "nm_compute_row")
+
+    def computeAll(self):
+        _check(lib().nm_compute_all(self._h), "nm_compute_all")
+
+    def at(self, r, c, sc=None):
+        e = EscapeValue()
+        if sc is None:
+            _check(lib().nm_at(self._h, r, c, C.byref(e)), "nm_at")
+        else:
+            _check(lib().nm_at_avg(self._h, r, c, sc, C.byref(e)), "nm_at_avg")
+        return e.iterations, e.smoothing
+
+    def grid(self):
+        out = np.zeros(self.rows() * self.cols(), dtype=ESCAPE_DTYPE)
+        _check(lib().nm_get_grid(self._h, out.ctypes.data, out.size), "nm_get_grid")
+        return out.reshape(self.rows(), self.cols())
+
+    def pointAt(self, r, c, sc=1):
+        re = C.create_string_buffer(4096)
+        im = C.create_string_buffer(4096)
+        _check(lib().nm_point_at(self._h, r, c, sc, re, im, 4096), "nm_point_at")
+        return re.value.decode(), im.value.decode()
+
+    def translate(self, dr, dc, sc=1):
+        _check(lib().nm_translate(self._h, dr, dc, sc), "nm_translate")
+
+    def zoom(self, scale):
+        _check(lib().nm_zoom(self._h, scale), "nm_zoom")
+
+    def zoomAt(self, scale, r, c, sc=1):
+        _check(lib().nm_zoom_at(self._h, scale, r, c, sc), "nm_zoom_at")
+
+    def scaleUp(self, sc):
+        _check(lib().nm_scale_up(self._h, sc), "nm_scale_up")
+
+    def scaleDown(self, sc):
+        _check(lib().nm_scale_down(self._h, sc), "nm_scale_down")
+
+    def loadLegacy(self, path):
+        _check(lib().nm_load_legacy(self._h, _b(path)), "nm_load_legacy")
+
+    def save_view(self, path):
+        _check(lib().nm_save_view(self._h, _b(path)), "nm_save_view")
+
+    # ---- extensions
+    def set_bailout(self, b):
+        _check(lib().nm_set_bailout(self._h, float(b)), "nm_set_bailout")
+
+    def set_delta_mode(self, mode):
+        _check(lib().nm_set_delta_mode(self._h, int(mode)), "nm_set_delta_mode")
+
+    def set_view(self, N, cre, cim, width):
+        _check(lib().nm_set_view(self._h, int(N), _b(str(cre)), _b(str(cim)), _b(str(width))), "nm_set_view")
+
+    def precompute_at(self, re, im):
+        _check(lib().nm_precompute_at(self._h, _b(str(re)), _b(str(im))), "nm_precompute_at")
+
+    def reference_length(self):
+        return lib().nm_reference_length(self._h)
+
+    def probe_winner(self):
+        return lib().nm_probe_winner(self._h)
+
+    def reference(self):
+        n = self.reference_length()
+        X = np.zeros(2 * n)
+        A = np.zeros(2 * n)
+        B = np.zeros(2 * n)
+        Cc = np.zeros(2 * n)
+        _check(lib().nm_get_reference(self._h, X.ctypes.data, A.ctypes.data, B.ctypes.data, Cc.ctypes.data),
+               "nm_get_reference")
+        return X, A, B, Cc
+
+    def render(self, block_k=0, waves_per_simd=0, repeats=1, copy_back=True):
+        st = RenderStats()
+        _check(lib().nm_render(self._h, block_k, waves_per_simd, repeats, 1 if copy_back else 0, C.byref(st)),
+               "nm_render")
+        return st.as_dict()
+
+    def render_frames(self, widths, block_k=0, waves_per_simd=0, repeats=1):
+        n = len(widths)
+        arr = (C.c_char_p * n)(*[_b(str(w)) for w in widths])
+        out = np.zeros(n * self.rows() * self.cols(), dtype=ESCAPE_DTYPE)
+        st = RenderStats()
+        _check(lib().nm_render_frames(self._h, arr, n, block_k, waves_per_simd, repeats, out.ctypes.data,
+                                      C.byref(st)), "nm_render_frames")
+        return out.reshape(n, self.rows(), self.cols()), st.as_dict()

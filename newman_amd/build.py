@@ -1,0 +1,60 @@
+"""Build the native library newman_amd/libnewman_amd.so (HIP kernels for gfx950
+plus the C++ host side and the C ABI) and, when asked, the CPU oracle.
+
+Usage: python -m newman_amd.build [--oracle]
+
+The HIP file is compiled with -ffp-contract=off (the plain path must reproduce the
+reference's FMA-free double arithmetic bit for bit) and with packed FP32 disabled.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libnewman_amd.so")
+ARCH = os.environ.get("NEWMAN_AMD_ARCH", "gfx950")
+
+HOST_SRCS = ["hpfloat.cpp", "mandelbrot.cpp", "capi.cpp"]
+HIP_SRCS = ["kernels.hip"]
+
+CXXFLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-function",
+            "-pthread"]
+HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build_library(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in HOST_SRCS:
+        obj = os.path.join(BUILD, src + ".o")
+        _run(["g++"] + CXXFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    for src in HIP_SRCS:
+        obj = os.path.join(BUILD, src + ".o")
+        _run(["hipcc"] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    """The CPU oracle is test infrastructure (oracle/); built here for convenience."""
+    _run(["make", "-C", os.path.join(ROOT, "oracle")])
+    return os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+if __name__ == "__main__":
+    build_library()
+    if "--oracle" in sys.argv:
+        build_oracle()

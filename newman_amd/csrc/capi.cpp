@@ -1,0 +1,344 @@
+// capi.cpp - extern "C" boundary (include/newman_amd.h) over nm::Mandelbrot.
+// No exception crosses this boundary; every entry point returns a status.
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/newman_amd.h"
+#include "mandelbrot.hpp"
+
+using nm::Mandelbrot;
+
+struct nm_mandelbrot {
+  Mandelbrot m;
+  nm_mandelbrot(int nr, int nc) : m(nr, nc) {}
+  explicit nm_mandelbrot(const Mandelbrot& o) : m(o) {}
+};
+
+static thread_local std::string g_err;
+
+static int put_str(const std::string& s, char* out, size_t cap) {
+  if (!out || cap == 0) return NM_ERR_ARG;
+  if (s.size() + 1 > cap) return NM_ERR_RANGE;
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return NM_OK;
+}
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    g_err = "out of host memory";
+    return NM_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return NM_ERR_ARG;
+  } catch (...) {
+    g_err = "unknown error";
+    return NM_ERR_ARG;
+  }
+}
+
+static int engine_status(int st) {
+  if (st != NM_OK) {
+    nm::Engine* e = nullptr;
+    if (nm::Engine::instance(&e) == NM_OK && e) g_err = e->last_error();
+  }
+  return st;
+}
+
+static void copy_stats(const nm::RenderStats& s, nm_render_stats* o) {
+  if (!o) return;
+  o->samples = s.samples;
+  o->n_cardioid = s.n_cardioid;
+  o->n_sa_resolved = s.n_sa_resolved;
+  o->n_iterated = s.n_iterated;
+  o->executed = s.executed;
+  o->skipped = s.skipped;
+  o->n_rebased = s.n_rebased;
+  o->credited = s.credited;
+  o->ms_start = s.ms_start;
+  o->ms_iterate = s.ms_iterate;
+  o->ms_total = s.ms_total;
+  o->block_k = s.block_k;
+  o->waves_per_simd = s.waves_per_simd;
+}
+
+extern "C" {
+
+int nm_create(nm_mandelbrot** out, int nr, int nc) {
+  if (!out || nr <= 0 || nc <= 0) return NM_ERR_ARG;
+  return guard([&] {
+    *out = new nm_mandelbrot(nr, nc);
+    return NM_OK;
+  });
+}
+
+int nm_clone(nm_mandelbrot** out, const nm_mandelbrot* src) {
+  if (!out || !src) return NM_ERR_ARG;
+  return guard([&] {
+    *out = new nm_mandelbrot(src->m);
+    return NM_OK;
+  });
+}
+
+void nm_destroy(nm_mandelbrot* h) { delete h; }
+
+int nm_rows(const nm_mandelbrot* h) { return h ? h->m.rows() : NM_ERR_ARG; }
+int nm_cols(const nm_mandelbrot* h) { return h ? h->m.cols() : NM_ERR_ARG; }
+
+int nm_set_N(nm_mandelbrot* h, int N) {
+  if (!h || N <= 0) return NM_ERR_ARG;
+  h->m.N = N;
+  return NM_OK;
+}
+int nm_get_N(const nm_mandelbrot* h) { return h ? h->m.N : NM_ERR_ARG; }
+
+int nm_set_error_tolerance(nm_mandelbrot* h, double tol) {
+  if (!h || !(tol > 0)) return NM_ERR_ARG;
+  h->m.error_tolerance = tol;
+  return NM_OK;
+}
+double nm_get_error_tolerance(const nm_mandelbrot* h) { return h ? h->m.error_tolerance : -1.0; }
+
+int nm_set_center(nm_mandelbrot* h, const char* re, const char* im) {
+  if (!h || !re || !im) return NM_ERR_ARG;
+  return guard([&] {
+    nm::hpf r(h->m.center.re.L), i(h->m.center.im.L);
+    if (!nm::hp_set_str(r, re) || !nm::hp_set_str(i, im)) return (int)NM_ERR_ARG;
+    h->m.center.re = r;
+    h->m.center.im = i;
+    return (int)NM_OK;
+  });
+}
+
+int nm_get_center(const nm_mandelbrot* h, char* re, char* im, size_t cap) {
+  if (!h) return NM_ERR_ARG;
+  int a = put_str(h->m.center.re.to_string(), re, cap);
+  return a ? a : put_str(h->m.center.im.to_string(), im, cap);
+}
+
+int nm_set_size(nm_mandelbrot* h, const char* re, const char* im) {
+  if (!h || !re || !im) return NM_ERR_ARG;
+  return guard([&] {
+    nm::hpf r(h->m.sz.re.L), i(h->m.sz.im.L);
+    if (!nm::hp_set_str(r, re) || !nm::hp_set_str(i, im)) return (int)NM_ERR_ARG;
+    h->m.sz.re = r;
+    h->m.sz.im = i;
+    return (int)NM_OK;
+  });
+}
+
+int nm_get_size(const nm_mandelbrot* h, char* re, char* im, size_t cap) {
+  if (!h) return NM_ERR_ARG;
+  int a = put_str(h->m.sz.re.to_string(), re, cap);
+  return a ? a : put_str(h->m.sz.im.to_string(), im, cap);
+}
+
+int nm_get_size_d(const nm_mandelbrot* h, double* re, double* im) {
+  if (!h || !re || !im) return NM_ERR_ARG;
+  *re = h->m.sz.re.get_d();
+  *im = h->m.sz.im.get_d();
+  return NM_OK;
+}
+
+int nm_set_precision(nm_mandelbrot* h) {
+  if (!h) return NM_ERR_ARG;
+  h->m.setPrecision();
+  return NM_OK;
+}
+long nm_precision_bits(const nm_mandelbrot* h) { return h ? h->m.precisionBits() : NM_ERR_ARG; }
+
+int nm_use_hardware(nm_mandelbrot* h) { return h ? (h->m.useHardware() ? 1 : 0) : NM_ERR_ARG; }
+
+int nm_precompute(nm_mandelbrot* h) {
+  if (!h) return NM_ERR_ARG;
+  return guard([&] { return engine_status(h->m.precompute()); });
+}
+
+int nm_compute_row(nm_mandelbrot* h, int r) {
+  if (!h) return NM_ERR_ARG;
+  return guard([&] { return engine_status(h->m.computeRow(r)); });
+}
+
+int nm_compute_all(nm_mandelbrot* h) {
+  if (!h) return NM_ERR_ARG;
+  return guard([&] { return engine_status(h->m.computeAll()); });
+}
+
+int nm_at(nm_mandelbrot* h, int r, int c, nm_escape_value* out) {
+  if (!h || !out || r < 0 || c < 0 || r >= h->m.rows() || c >= h->m.cols()) return NM_ERR_ARG;
+  const nm::EscapeValue& e = h->m.at(r, c);
+  out->iterations = e.iterations;
+  out->smoothing = e.smoothing;
+  return NM_OK;
+}
+
+int nm_at_avg(nm_mandelbrot* h, int r, int c, int sc, nm_escape_value* out) {
+  if (!h || !out || sc <= 0 || r < 0 || c < 0 || (r + 1) * sc > h->m.rows() || (c + 1) * sc > h->m.cols())
+    return NM_ERR_ARG;
+  nm::EscapeValue e = h->m.at(r, c, sc);
+  out->iterations = e.iterations;
+  out->smoothing = e.smoothing;
+  return NM_OK;
+}
+
+int nm_get_grid(nm_mandelbrot* h, nm_escape_value* out, size_t n) {
+  if (!h || !out) return NM_ERR_ARG;
+  auto& v = h->m.grid_ref().values;
+  if (n < v.size()) return NM_ERR_RANGE;
+  std::memcpy(out, v.data(), v.size() * sizeof(nm_escape_value));
+  return NM_OK;
+}
+
+int nm_point_at(const nm_mandelbrot* h, int r, int c, int sc, char* re, char* im, size_t cap) {
+  if (!h) return NM_ERR_ARG;
+  nm::HPComplex p = h->m.pointAt(r, c, sc);
+  int a = put_str(p.re.to_string(), re, cap);
+  return a ? a : put_str(p.im.to_string(), im, cap);
+}
+
+int nm_translate(nm_mandelbrot* h, int dr, int dc, int sc) {
+  if (!h) return NM_ERR_ARG;
+  h->m.translate(dr, dc, sc);
+  return NM_OK;
+}
+
+int nm_zoom(nm_mandelbrot* h, float scale) {
+  if (!h || !(scale > 0)) return NM_ERR_ARG;
+  h->m.zoom(scale);
+  return NM_OK;
+}
+
+int nm_zoom_at(nm_mandelbrot* h, float scale, int r, int c, int sc) {
+  if (!h || !(scale > 0)) return NM_ERR_ARG;
+  h->m.zoomAt(scale, r, c, sc);
+  return NM_OK;
+}
+
+int nm_scale_up(nm_mandelbrot* h, int sc) {
+  if (!h || sc <= 0) return NM_ERR_ARG;
+  return guard([&] { return engine_status(h->m.scaleUp(sc)); });
+}
+
+int nm_scale_down(nm_mandelbrot* h, int sc) {
+  if (!h || sc <= 0) return NM_ERR_ARG;
+  return guard([&] { return engine_status(h->m.scaleDown(sc)); });
+}
+
+int nm_load_legacy(nm_mandelbrot* h, const char* path) {
+  if (!h || !path) return NM_ERR_ARG;
+  return guard([&] { return h->m.loadLegacy(path); });
+}
+
+// viewer.cpp:6-29: N, then centre.re, centre.im, sz.re, sz.im, one per line
+int nm_save_view(const nm_mandelbrot* h, const char* path) {
+  if (!h || !path) return NM_ERR_ARG;
+  FILE* fp = std::fopen(path, "w");
+  if (!fp) return NM_ERR_IO;
+  std::fprintf(fp, "%d\n%s\n%s\n%s\n%s\n", h->m.N, h->m.center.re.to_string().c_str(),
+               h->m.center.im.to_string().c_str(), h->m.sz.re.to_string().c_str(), h->m.sz.im.to_string().c_str());
+  std::fclose(fp);
+  return NM_OK;
+}
+
+int nm_set_bailout(nm_mandelbrot* h, double bailout) {
+  if (!h || !(bailout > 1.0)) return NM_ERR_ARG;
+  h->m.bailout = bailout;
+  return NM_OK;
+}
+
+int nm_set_delta_mode(nm_mandelbrot* h, int mode) {
+  if (!h || (mode != NM_DELTA_FP64 && mode != NM_DELTA_FP32X)) return NM_ERR_ARG;
+  h->m.delta_mode = mode;
+  return NM_OK;
+}
+
+int nm_set_view(nm_mandelbrot* h, int N, const char* cre, const char* cim, const char* width) {
+  if (!h) return NM_ERR_ARG;
+  return guard([&] { return h->m.setView(N, cre, cim, width); });
+}
+
+int nm_precompute_at(nm_mandelbrot* h, const char* re, const char* im) {
+  if (!h || !re || !im) return NM_ERR_ARG;
+  return guard([&] { return engine_status(h->m.precomputeAt(re, im)); });
+}
+
+int nm_reference_length(const nm_mandelbrot* h) { return h ? h->m.referenceLength() : NM_ERR_ARG; }
+int nm_probe_winner(const nm_mandelbrot* h) { return h ? h->m.probeWinner() : NM_ERR_ARG; }
+
+int nm_get_reference(const nm_mandelbrot* h, double* X, double* A, double* B, double* C) {
+  if (!h) return NM_ERR_ARG;
+  const nm::SaReferenceHost& r = h->m.reference();
+  if (r.len <= 0) return NM_ERR_STATE;
+  size_t n = 2 * (size_t)r.len;
+  if (X) std::memcpy(X, r.Zd.data() + 2, n * sizeof(double));
+  if (A) std::memcpy(A, r.A.data(), n * sizeof(double));
+  if (B) std::memcpy(B, r.B.data(), n * sizeof(double));
+  if (C) std::memcpy(C, r.C.data(), n * sizeof(double));
+  return NM_OK;
+}
+
+int nm_render(nm_mandelbrot* h, int block_k, int waves_per_simd, int repeats, int copy_back,
+              nm_render_stats* stats) {
+  if (!h || repeats < 1 || block_k < 0 || waves_per_simd < 0) return NM_ERR_ARG;
+  return guard([&] {
+    nm::RenderOpts o;
+    o.block_k = block_k;
+    o.waves_per_simd = waves_per_simd;
+    o.repeats = repeats;
+    int st = h->m.render(&o, copy_back != 0);
+    if (st == NM_OK) copy_stats(h->m.lastStats(), stats);
+    return engine_status(st);
+  });
+}
+
+int nm_render_frames(nm_mandelbrot* h, const char* const* widths, int nframes, int block_k, int waves_per_simd,
+                     int repeats, nm_escape_value* out, nm_render_stats* stats) {
+  if (!h || !widths || nframes <= 0 || repeats < 1) return NM_ERR_ARG;
+  return guard([&] {
+    nm::RenderOpts o;
+    o.block_k = block_k;
+    o.waves_per_simd = waves_per_simd;
+    o.repeats = repeats;
+    std::vector<std::string> w(widths, widths + nframes);
+    int st = h->m.renderFrames(w, &o, (nm::EscapeValue*)out);
+    if (st == NM_OK) copy_stats(h->m.lastStats(), stats);
+    return engine_status(st);
+  });
+}
+
+const char* nm_last_error(void) { return g_err.c_str(); }
+
+int nm_hp_op(int op, const char* a, const char* b, long iv, double dv, long bits, int* sign, long* exp,
+             uint32_t* words) {
+  if (!sign || !exp || !words) return NM_ERR_ARG;
+  return guard([&] {
+    int L = nm::hp_limbs_for_bits(bits);
+    nm::hpf x(L), y(L), r(L);
+    if (a && !nm::hp_set_str(x, a)) return (int)NM_ERR_ARG;
+    if (b && !nm::hp_set_str(y, b)) return (int)NM_ERR_ARG;
+    switch (op) {
+      case 0: nm::hp_add(r, x, y); break;
+      case 1: nm::hp_sub(r, x, y); break;
+      case 2: nm::hp_mul(r, x, y); break;
+      case 3: nm::hp_mul_si(r, x, iv); break;
+      case 4: nm::hp_div_ui(r, x, (unsigned long)iv); break;
+      case 5: nm::hp_set(r, x); break;
+      case 6: nm::hp_mul_d(r, x, dv); break;
+      default: return (int)NM_ERR_ARG;
+    }
+    *sign = r.sign;
+    *exp = (long)r.exp;
+    for (int i = 0; i < L; i++) {
+      uint64_t limb = r.m[L - 1 - i];
+      words[2 * i] = (uint32_t)(limb >> 32);
+      words[2 * i + 1] = (uint32_t)limb;
+    }
+    return 2 * L;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// engine.hpp - host-side interface of the HIP render engine (no HIP types here,
+// so the host classes compile with any C++ compiler).
+#pragma once
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+
+#include "../../include/newman_amd.h"
+
+namespace nm {
+
+// grid.h:8-15: {int iterations; float smoothing}, 8 bytes, row-major AoS.
+struct EscapeValue {
+  int32_t iterations;
+  float smoothing;
+};
+static_assert(sizeof(EscapeValue) == 8, "EscapeValue must keep grid.h's 8-byte layout");
+
+// Status codes (NM_OK, NM_ERR_*) are the C ABI's.
+// Cardioid/bulb pre-classification of a frame.
+enum CardMode : int32_t { CARD_NONE = 0, CARD_ALL = 1, CARD_TEST_DD = 2, CARD_MASK = 3 };
+
+// Arithmetic used for the deltas of the perturbation kernel.
+enum DeltaMode : int32_t { DELTA_FP64 = 0, DELTA_FP32X = 1 };
+
+// One frame of the plain-double path (mandelbrot.cpp:231-254): per-column and
+// per-row truncated coordinates (descend of the HP pixel map) plus the remainder
+// as a double-double low part for the cardioid test.
+struct HwFrameHost {
+  int nr = 0, nc = 0;
+  int card_mode = CARD_TEST_DD;
+  std::vector<double> cre, crelo;  // nc
+  std::vector<double> cim, cimlo;  // nr
+};
+
+// One frame of the series-approximation path (mandelbrot.cpp:144-229): per-column
+// and per-row epsilon = descend(Y0 - X[0]), and the cardioid classification.
+struct SaFrameHost {
+  int nr = 0, nc = 0;
+  int card_mode = CARD_NONE;
+  std::vector<double> epsre;      // nc
+  std::vector<double> epsim;      // nr
+  std::vector<uint32_t> mask;     // CARD_MASK: nr*nc bits, 1 = in cardioid/bulb
+};
+
+// Reference orbit and series coefficients in device-ready form.
+struct SaReferenceHost {
+  int len = 0;                 // len(X)
+  std::vector<double> Zd;      // 2*(len+2): Zd[0] = 0, Zd[k] = nearest(X[k-1]); Zd[len+1] = X[len]
+  std::vector<double> Xlo;     // 2*len: nearest(X[k] - Zd[k+1])
+  std::vector<double> A, B, C; // 2*len: descend(A[i]) etc. (mandelbrot.cpp:166-168)
+  // Stop-index tables (prefix over j in [1, i]):
+  std::vector<double> PM;      // min log2(tol |B_j|^2 / |C_j|^2)
+  std::vector<double> BX, BN;  // max / min log2 |B_j|^2
+  std::vector<double> CX, CN;  // max / min log2 |C_j|^2
+  std::vector<int32_t> SP;     // count of special j (zero or non-finite B_j or C_j)
+  // floatexp copies for the FP32X kernel: mantissa pairs and shared exponent
+  std::vector<float> Zf;       // 2*(len+2) Zd as float with per-entry exponent in Ze
+  std::vector<int32_t> Ze;     // len+2
+};
+
+struct RenderOpts {
+  int N = 256;
+  double bailout = 1024.0;
+  double tol = 1e-10;
+  int delta_mode = DELTA_FP64;
+  int block_k = 0;        // iterations per block between lane checks (0 = tuned default)
+  int waves_per_simd = 0; // persistent waves per SIMD (0 = tuned default)
+  int repeats = 1;        // timing: launch the whole frame set this many times
+};
+
+struct RenderStats {
+  uint64_t samples = 0;
+  uint64_t n_cardioid = 0;   // samples resolved by the cardioid/bulb test
+  uint64_t n_sa_resolved = 0; // escaped inside the series skip, or the skip reached N
+  uint64_t n_iterated = 0;   // samples handed to the iteration kernel
+  uint64_t executed = 0;     // iteration steps executed by the iteration kernel
+  uint64_t skipped = 0;      // sum of series-skip start indices of iterated samples
+  uint64_t n_rebased = 0;    // perturbation rebases (glitch avoidance)
+  uint64_t credited = 0;     // sum of reported iterations over non-cardioid samples
+  double ms_start = 0;       // classification / series start kernel (last repeat)
+  double ms_iterate = 0;     // iteration kernel (average over repeats)
+  double ms_total = 0;       // both kernels, average over repeats
+  int block_k = 0, waves_per_simd = 0;
+};
+
+// Device engine: one per process per device.  All calls are synchronous.
+class Engine {
+ public:
+  static int instance(Engine** out);
+  virtual ~Engine() = default;
+  // Make `ref` the device reference; `serial` identifies it so an unchanged
+  // reference is not uploaded again.
+  virtual int set_reference(const SaReferenceHost& ref, uint64_t serial) = 0;
+  virtual int render_hw(const std::vector<const HwFrameHost*>& frames, const RenderOpts& opt,
+                        EscapeValue* host_out, RenderStats* stats) = 0;
+  virtual int render_sa(const std::vector<const SaFrameHost*>& frames, const RenderOpts& opt,
+                        EscapeValue* host_out, RenderStats* stats) = 0;
+  // multisample average (mandelbrot.cpp:320-332) of a fine grid on the device
+  virtual int downsample(const EscapeValue* fine, int nr, int nc, int sc, EscapeValue* out) = 0;
+  virtual const char* last_error() const = 0;
+};
+
+}  // namespace nm

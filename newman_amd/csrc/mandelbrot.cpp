@@ -1,0 +1,728 @@
+// mandelbrot.cpp - host side of the drop-in render core (see mandelbrot.hpp).
+// Each method cites the reference method it replaces.
+#include "mandelbrot.hpp"
+
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <thread>
+
+namespace nm {
+
+static constexpr double kMinPreview = 1.5e-16;  // mandelbrot.cpp:257
+
+// ------------------------------------------------------------------ threading
+template <class F>
+static void parallel_for(long total, F&& fn, long chunk = 4) {
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt == 0) nt = 1;
+  if (nt > 64) nt = 64;
+  if (total <= chunk || nt == 1) {
+    for (long i = 0; i < total; i++) fn(i);
+    return;
+  }
+  std::atomic<long> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      long s = next.fetch_add(chunk);
+      if (s >= total) break;
+      long e = std::min(total, s + chunk);
+      for (long i = s; i < e; i++) fn(i);
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nt; t++) th.emplace_back(worker);
+  worker();
+  for (auto& t : th) t.join();
+}
+
+// ------------------------------------------------------------------ helpers
+static inline void times2(hpf& x) {
+  if (x.sign) x.exp += 1;  // 2.0 * x is exact
+}
+
+// nearest-rounded double of an HP value (the perturbation kernels' reference
+// orbit), built on the truncating conversion
+static double get_d_nearest(const hpf& a) {
+  double t = a.get_d();
+  if (a.sign == 0 || !std::isfinite(t)) return t;
+  hpf td(2), r(a.L);
+  hp_set_d(td, t);
+  hp_sub(r, a, td);  // remainder, same sign as a or zero
+  if (r.sign == 0) return t;
+  double up = std::nextafter(t, a.sign > 0 ? INFINITY : -INFINITY);
+  hpf half(2);
+  hp_set_d(half, (up - t) * 0.5);
+  r.sign = 1;
+  half.sign = 1;
+  int c = hp_cmp(r, half);
+  if (c > 0) return up;
+  if (c < 0) return t;
+  uint64_t bits;
+  std::memcpy(&bits, &t, 8);
+  return (bits & 1) ? up : t;
+}
+
+static inline bool bailedOut(const hpf& re, const hpf& im, double b2) {
+  LPComplex z(re.get_d(), im.get_d());
+  return sqMag(z) > b2;  // mandelbrot.cpp:60
+}
+
+// mandelbrot.cpp:63-71, in HP at the working precision
+bool inCardioidHP(const hpf& re, const hpf& im) {
+  int L = std::max(re.L, im.L);
+  hpf fourth(L), xmf(L), y2(L), q(L), t(L), lhs(L), rhs(L);
+  hp_set_d(fourth, 0.25);
+  hp_sub(xmf, re, fourth);
+  hp_mul(y2, im, im);
+  hp_mul(q, xmf, xmf);
+  hp_add(q, q, y2);
+  hp_add(t, q, xmf);
+  hp_mul(lhs, q, t);
+  hp_mul(rhs, fourth, y2);
+  if (hp_cmp(lhs, rhs) < 0) return true;  // cardioid
+  hpf one(2);
+  hp_set_d(one, 1.0);
+  hp_add(q, re, one);
+  hp_mul(lhs, q, q);
+  hp_add(lhs, lhs, y2);
+  hp_mul(rhs, fourth, fourth);
+  return hp_cmp(lhs, rhs) < 0;  // second disk
+}
+
+// The pixel map of computeRow / findProbe / pointAt (mandelbrot.cpp:271, 275,
+// 86-87, 287-288): re = center.re + k*sz.re, im = center.im + j*sz.im, each step
+// truncated at the working precision.
+static void pixel_re(hpf& out, const hpf& cre, const hpf& szre, long k) {
+  hpf t(out.L);
+  hp_mul_si(t, szre, k);
+  hp_add(out, cre, t);
+}
+
+// ------------------------------------------------------------------ class
+Mandelbrot::Mandelbrot() : Mandelbrot(1, 1) {}
+
+// mandelbrot.cpp:8-17
+Mandelbrot::Mandelbrot(int nr, int nc) : grid(nr, nc) {
+  error_tolerance = 1e-10;
+  N = 256;
+  center.re = -0.5;
+  center.im = 0.0;
+  sz.re = 4.0 / nc;
+  sz.im = 3.0 / nr;
+  setPrecision();
+}
+
+// mandelbrot.cpp:37-56
+void Mandelbrot::setPrecision() {
+  const double log_alpha = std::log2(1.0e-20);
+  const int beta = 64;
+  long e;
+  sz.re.get_d_2exp(&e);
+  int bits = (int)(beta - e + log_alpha);
+  if (bits < 64) bits = 64;
+  bits_ = bits;
+  hp_set_default_bits(bits);  // process-wide, as mpf_set_default_prec
+  center.re.set_prec_bits(bits);
+  center.im.set_prec_bits(bits);
+  sz.re.set_prec_bits(bits);
+  sz.im.set_prec_bits(bits);
+  X.clear();
+  ref_ = SaReferenceHost();
+  have_ref_ = false;
+  rendered_ = false;
+}
+
+// mandelbrot.cpp:19-35: values parsed at the current precision, then sz rescaled
+// by 800/cols, 600/rows and setPrecision.
+int Mandelbrot::loadLegacy(const char* fn) {
+  FILE* fp = std::fopen(fn, "r");
+  if (!fp) return NM_ERR_IO;
+  char buf[4096];
+  int n;
+  if (std::fscanf(fp, "%d\n", &n) != 1) {
+    std::fclose(fp);
+    return NM_ERR_IO;
+  }
+  N = n;
+  if (std::fscanf(fp, "%4095s\n", buf) == 1) center.re = buf;
+  if (std::fscanf(fp, "%4095s\n", buf) == 1) center.im = buf;
+  if (std::fscanf(fp, "%4095s\n", buf) == 1) sz.re = buf;
+  if (std::fscanf(fp, "%4095s\n", buf) == 1) sz.im = buf;
+  std::fclose(fp);
+  sz.re = sz.re * (800.0 / cols());
+  sz.im = sz.im * (600.0 / rows());
+  setPrecision();
+  return NM_OK;
+}
+
+// Parse a view at the precision the target width needs: N, centre strings and
+// width = cols * sz.re (square samples).  A harness convenience, not a reference
+// method: loadLegacy would parse the centre at the previous precision.
+int Mandelbrot::setView(int n, const char* cre, const char* cim, const char* width) {
+  if (n <= 0 || !cre || !cim || !width) return NM_ERR_ARG;
+  hpf w(hp_limbs_for_bits(256));
+  if (!hp_set_str(w, width) || w.sign <= 0) return NM_ERR_ARG;
+  hpf s(w.L);
+  hp_div_ui(s, w, (unsigned long)cols());
+  long e;
+  s.get_d_2exp(&e);
+  int bits = (int)(64 - e + std::log2(1.0e-20));
+  if (bits < 64) bits = 64;
+  int L = hp_limbs_for_bits(bits);
+  hpf r(L), i(L), wl(L), szl(L);
+  if (!hp_set_str(r, cre) || !hp_set_str(i, cim)) return NM_ERR_ARG;
+  hp_set_str(wl, width);
+  hp_div_ui(szl, wl, (unsigned long)cols());
+  N = n;
+  hp_copy_prec(center.re, r);
+  hp_copy_prec(center.im, i);
+  hp_copy_prec(sz.re, szl);
+  hp_copy_prec(sz.im, szl);
+  setPrecision();
+  return NM_OK;
+}
+
+// mandelbrot.cpp:256-259
+bool Mandelbrot::useHardware() { return sz.re.get_d() >= kMinPreview && sz.im.get_d() >= kMinPreview; }
+
+// mandelbrot.cpp:97-110.  X[0] = X0; X[i] = X[i-1]^2 + X0, truncated at the first
+// i with bailedOut(X[i]) (size i) or at N.  With store, X keeps len+1 entries:
+// the extra X[len] (the escaping element, or one more step) feeds the device
+// orbit so the perturbation kernel can rebase at the reference's end.
+int Mandelbrot::computeOrbit(const HPComplex& X0, bool store) {
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  const double b2 = bailout * bailout;
+  hpf xr(L), xi(L), nr(L), ni(L), t1(L), t2(L);
+  hp_set(xr, X0.re);
+  hp_set(xi, X0.im);
+  if (store) {
+    X.clear();
+    X.reserve((size_t)N + 1);
+    HPComplex z;
+    z.re = hpf(L); z.im = hpf(L);
+    hp_set(z.re, xr);
+    hp_set(z.im, xi);
+    X.push_back(z);
+  }
+  int len = N;
+  for (int i = 1; i <= N; i++) {
+    if (!store && i == N) break;
+    hp_mul(t1, xr, xr);
+    hp_mul(t2, xi, xi);
+    hp_sub(t1, t1, t2);
+    hp_add(nr, t1, X0.re);
+    hp_mul(t1, xr, xi);
+    times2(t1);
+    hp_add(ni, t1, X0.im);
+    if (store) {
+      HPComplex z;
+      z.re = hpf(L); z.im = hpf(L);
+      hp_set(z.re, nr);
+      hp_set(z.im, ni);
+      X.push_back(z);
+    }
+    if (i == N) break;  // the extra element past a full-length orbit
+    if (bailedOut(nr, ni, b2)) {
+      len = i;
+      break;
+    }
+    hp_copy_prec(xr, nr);
+    hp_copy_prec(xi, ni);
+  }
+  return len;
+}
+
+// mandelbrot.cpp:73-95.  The probe orbits are independent: their lengths are
+// computed in parallel and the first longest one is kept, as the serial
+// swap-if-longer loop does; its orbit is then recomputed and stored.
+void Mandelbrot::findProbe() {
+  std::vector<std::pair<int, int>> pts;
+  for (int c = 0; c < cols(); c += 2) {
+    pts.emplace_back(rows() / 4, c);
+    pts.emplace_back(rows() / 2, c);
+    pts.emplace_back(3 * rows() / 4, c);
+  }
+  for (int r = 0; r < rows(); r += 2) pts.emplace_back(r, cols() / 2);
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  std::vector<int> lens(pts.size(), 0);
+  parallel_for((long)pts.size(), [&](long i) {
+    HPComplex p;
+    p.re = hpf(L); p.im = hpf(L);
+    pixel_re(p.re, center.re, sz.re, (long)(pts[i].second - cols() / 2));
+    pixel_re(p.im, center.im, sz.im, (long)(rows() / 2 - pts[i].first - 1));
+    Mandelbrot* self = this;
+    // length only: a private orbit loop (computeOrbit's non-storing path is
+    // reentrant; it touches no member state)
+    lens[i] = self->computeOrbit(p, false);
+  }, 2);
+  int best = -1, bestlen = 0;
+  for (size_t i = 0; i < pts.size(); i++)
+    if (lens[i] > bestlen) {
+      bestlen = lens[i];
+      best = (int)i;
+    }
+  winner_ = best;
+  HPComplex p;
+  p.re = hpf(L); p.im = hpf(L);
+  pixel_re(p.re, center.re, sz.re, (long)(pts[best].second - cols() / 2));
+  pixel_re(p.im, center.im, sz.im, (long)(rows() / 2 - pts[best].first - 1));
+  int len = computeOrbit(p, true);
+  X.resize((size_t)len + 1);
+  ref_.len = len;
+}
+
+// mandelbrot.cpp:112-131, the recurrence exactly as written (A[i], B[i] on the
+// right-hand side of B[i], C[i]).  Only the truncated doubles the per-pixel work
+// reads are kept.
+void Mandelbrot::computeSeries() {
+  const int len = ref_.len;
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  ref_.A.assign(2 * (size_t)len, 0.0);
+  ref_.B.assign(2 * (size_t)len, 0.0);
+  ref_.C.assign(2 * (size_t)len, 0.0);
+  hpf Ar(L), Ai(L), Br(L), Bi(L), Cr(L), Ci(L), nAr(L), nAi(L), nBr(L), nBi(L), nCr(L), nCi(L), t(L);
+  hpf one(2);
+  hp_set_d(one, 1.0);
+  hp_set_d(Ar, 1.0);
+  ref_.A[0] = 1.0;
+  for (int i = 1; i < len; i++) {
+    const hpf& xr = X[i - 1].re;
+    const hpf& xi = X[i - 1].im;
+    hp_mul(nAr, xr, Ar); hp_mul(t, xi, Ai); hp_sub(nAr, nAr, t); times2(nAr); hp_add(nAr, nAr, one);
+    hp_mul(nAi, xr, Ai); hp_mul(t, xi, Ar); hp_add(nAi, nAi, t); times2(nAi);
+    hp_mul(nBr, xr, Br); hp_mul(t, xi, Bi); hp_sub(nBr, nBr, t); times2(nBr);
+    hp_mul(t, nAr, nAr); hp_add(nBr, nBr, t);
+    hp_mul(t, nAi, nAi); hp_sub(nBr, nBr, t);
+    hp_mul(nBi, xr, Bi); hp_mul(t, xi, Br); hp_add(nBi, nBi, t);
+    hp_mul(t, nAr, nAi); hp_add(nBi, nBi, t); times2(nBi);
+    hp_mul(nCr, xr, Cr); hp_mul(t, xi, Ci); hp_sub(nCr, nCr, t);
+    hp_mul(t, nAr, nBr); hp_add(nCr, nCr, t);
+    hp_mul(t, nAi, nBi); hp_sub(nCr, nCr, t); times2(nCr);
+    hp_mul(nCi, xr, Ci); hp_mul(t, xi, Cr); hp_add(nCi, nCi, t);
+    hp_mul(t, nAr, nBi); hp_add(nCi, nCi, t);
+    hp_mul(t, nAi, nBr); hp_add(nCi, nCi, t); times2(nCi);
+    hp_copy_prec(Ar, nAr); hp_copy_prec(Ai, nAi);
+    hp_copy_prec(Br, nBr); hp_copy_prec(Bi, nBi);
+    hp_copy_prec(Cr, nCr); hp_copy_prec(Ci, nCi);
+    ref_.A[2 * i] = Ar.get_d(); ref_.A[2 * i + 1] = Ai.get_d();
+    ref_.B[2 * i] = Br.get_d(); ref_.B[2 * i + 1] = Bi.get_d();
+    ref_.C[2 * i] = Cr.get_d(); ref_.C[2 * i + 1] = Ci.get_d();
+  }
+}
+
+// log2 |z|^2 without overflow (long double has a 15-bit exponent)
+static double log2_sqmag(double re, double im) {
+  long double r = re, i = im;
+  long double m = r * r + i * i;
+  if (m == 0) return -INFINITY;
+  return (double)log2l(m);
+}
+
+// Tables for the O(log N) per-sample stop-index search that reproduces the
+// reference's SA scan (mandelbrot.cpp:165-181).  The stop test isUnstable
+// (mandelbrot.cpp:138-142), |B e^2|^2 tol < |C e^3|^2, is |e|^2 > T_i with
+// T_i = tol |B_i|^2 / |C_i|^2 up to rounding; the first unstable i is the first
+// index whose prefix-minimum of T drops below |e|^2.  The kernels resolve the
+// rounding band around each threshold with the exact double test, and use the
+// magnitude prefixes to prove that no intermediate of the exact test under- or
+// overflows (otherwise they fall back to the reference's linear scan).
+void Mandelbrot::buildThresholds() {
+  const int len = ref_.len;
+  const double ltol = std::log2(error_tolerance);
+  ref_.PM.assign(len, INFINITY);
+  ref_.BX.assign(len, -INFINITY);
+  ref_.BN.assign(len, INFINITY);
+  ref_.CX.assign(len, -INFINITY);
+  ref_.CN.assign(len, INFINITY);
+  ref_.SP.assign(len, 0);
+  double pm = INFINITY, bx = -INFINITY, bn = INFINITY, cx = -INFINITY, cn = INFINITY;
+  int sp = 0;
+  for (int i = 1; i < len; i++) {
+    double br = ref_.B[2 * i], bi = ref_.B[2 * i + 1], cr = ref_.C[2 * i], ci = ref_.C[2 * i + 1];
+    bool special = !std::isfinite(br) || !std::isfinite(bi) || !std::isfinite(cr) || !std::isfinite(ci) ||
+                   (br == 0.0 && bi == 0.0) || (cr == 0.0 && ci == 0.0);
+    double lb = special ? 0.0 : log2_sqmag(br, bi);
+    double lc = special ? 0.0 : log2_sqmag(cr, ci);
+    if (special) sp++;
+    else {
+      double T = ltol + lb - lc;
+      pm = std::min(pm, T);
+      bx = std::max(bx, lb); bn = std::min(bn, lb);
+      cx = std::max(cx, lc); cn = std::min(cn, lc);
+    }
+    ref_.PM[i] = pm; ref_.BX[i] = bx; ref_.BN[i] = bn; ref_.CX[i] = cx; ref_.CN[i] = cn; ref_.SP[i] = sp;
+  }
+}
+
+// mandelbrot.cpp:261-267
+int Mandelbrot::precompute() {
+  if (useHardware()) return NM_OK;
+  if (N <= 0 || rows() <= 0 || cols() <= 0) return NM_ERR_ARG;
+  X.clear();
+  ref_ = SaReferenceHost();
+  findProbe();
+  computeSeries();
+  return uploadReference();
+}
+
+int Mandelbrot::precomputeAt(const char* re, const char* im) {
+  if (N <= 0) return NM_ERR_ARG;
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  HPComplex p;
+  p.re = hpf(L); p.im = hpf(L);
+  if (!hp_set_str(p.re, re) || !hp_set_str(p.im, im)) return NM_ERR_ARG;
+  X.clear();
+  ref_ = SaReferenceHost();
+  int len = computeOrbit(p, true);
+  X.resize((size_t)len + 1);
+  ref_.len = len;
+  winner_ = -1;
+  computeSeries();
+  return uploadReference();
+}
+
+static std::atomic<uint64_t> g_ref_serial{1};
+
+int Mandelbrot::uploadReference() {
+  const int len = ref_.len;
+  ref_.Zd.assign(2 * (size_t)(len + 2), 0.0);
+  ref_.Xlo.assign(2 * (size_t)len, 0.0);
+  ref_.Zf.assign(2 * (size_t)(len + 2), 0.0f);
+  ref_.Ze.assign((size_t)(len + 2), 0);
+  for (int k = 0; k <= len; k++) {
+    for (int p = 0; p < 2; p++) {
+      const hpf& x = p ? X[k].im : X[k].re;
+      double nd = get_d_nearest(x);
+      ref_.Zd[2 * (k + 1) + p] = nd;
+      if (k < len) {
+        hpf td(2), rem(x.L);
+        hp_set_d(td, nd);
+        hp_sub(rem, x, td);
+        ref_.Xlo[2 * k + p] = get_d_nearest(rem);
+      }
+    }
+  }
+  // floatexp orbit: Z = (fr, fi) * 2^e with max(|fr|, |fi|) in [0.5, 1)
+  for (int k = 0; k < len + 2; k++) {
+    double zr = ref_.Zd[2 * k], zi = ref_.Zd[2 * k + 1];
+    double m = std::max(std::fabs(zr), std::fabs(zi));
+    int e = 0;
+    if (m > 0) std::frexp(m, &e);
+    ref_.Ze[k] = e;
+    ref_.Zf[2 * k] = (float)std::ldexp(zr, -e);
+    ref_.Zf[2 * k + 1] = (float)std::ldexp(zi, -e);
+  }
+  buildThresholds();
+  have_ref_ = true;
+  rendered_ = false;
+  ref_serial_ = g_ref_serial.fetch_add(1);
+  return NM_OK;
+}
+
+// ------------------------------------------------------------ frame builders
+// Frame-level cardioid/bulb classification: returns CARD_NONE or CARD_ALL when
+// the sign of both tests is provably constant over the frame, otherwise -1.
+// For |x|, |y| <= 4 the gradient of q(q + x - 1/4) - y^2/4 is bounded by 4000
+// and that of (x + 1)^2 + y^2 - 1/16 by 40 (mean value bound).
+static int classify_frame(const HPComplex& center, const HPComplex& sz, int nr, int nc) {
+  double cx = center.re.get_d(), cy = center.im.get_d();
+  double hx = (std::fabs(sz.re.get_d()) * (nc / 2 + 2)), hy = (std::fabs(sz.im.get_d()) * (nr / 2 + 2));
+  if (cx + hx < -1.3 || cx - hx > 0.4 || cy + hy < -0.7 || cy - hy > 0.7) return CARD_NONE;
+  if (std::fabs(cx) + hx > 4.0 || std::fabs(cy) + hy > 4.0) return -1;
+  int L = std::max(center.re.L, 4);
+  hpf fourth(L), xmf(L), y2(L), q(L), t(L), lhs(L), rhs(L), f(L);
+  hp_set_d(fourth, 0.25);
+  hp_sub(xmf, center.re, fourth);
+  hp_mul(y2, center.im, center.im);
+  hp_mul(q, xmf, xmf);
+  hp_add(q, q, y2);
+  hp_add(t, q, xmf);
+  hp_mul(lhs, q, t);
+  hp_mul(rhs, fourth, y2);
+  hp_sub(f, lhs, rhs);
+  double fcard = f.get_d();
+  hpf one(2);
+  hp_set_d(one, 1.0);
+  hp_add(q, center.re, one);
+  hp_mul(lhs, q, q);
+  hp_add(lhs, lhs, y2);
+  hp_mul(rhs, fourth, fourth);
+  hp_sub(f, lhs, rhs);
+  double fdisk = f.get_d();
+  double h = hx + hy;
+  if (std::fabs(fcard) > 4000.0 * h * 1.01 && std::fabs(fdisk) > 40.0 * h * 1.01)
+    return (fcard < 0 || fdisk < 0) ? CARD_ALL : CARD_NONE;
+  return -1;
+}
+
+// Per-column / per-row truncated coordinates of the plain-double path
+// (descend(Y0) in getIterationsHW, mandelbrot.cpp:234) plus the double-double
+// remainder used by the device cardioid test.
+int Mandelbrot::buildHwFrame(HwFrameHost& f, const HPComplex& cen, const HPComplex& s) {
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  f.nr = rows();
+  f.nc = cols();
+  f.cre.resize(f.nc); f.crelo.resize(f.nc);
+  f.cim.resize(f.nr); f.cimlo.resize(f.nr);
+  hpf p(L), rem(L), td(2);
+  for (int c = 0; c < f.nc; c++) {
+    pixel_re(p, cen.re, s.re, (long)(c - cols() / 2));
+    double t = p.get_d();
+    hp_set_d(td, t);
+    hp_sub(rem, p, td);
+    f.cre[c] = t;
+    f.crelo[c] = get_d_nearest(rem);
+  }
+  for (int r = 0; r < f.nr; r++) {
+    pixel_re(p, cen.im, s.im, (long)(rows() / 2 - r - 1));
+    double t = p.get_d();
+    hp_set_d(td, t);
+    hp_sub(rem, p, td);
+    f.cim[r] = t;
+    f.cimlo[r] = get_d_nearest(rem);
+  }
+  int m = classify_frame(cen, s, f.nr, f.nc);
+  f.card_mode = m < 0 ? CARD_TEST_DD : m;
+  return NM_OK;
+}
+
+// Per-column / per-row epsilon = descend(Y0 - X[0]) (mandelbrot.cpp:155-159).
+int Mandelbrot::buildSaFrame(SaFrameHost& f, const HPComplex& cen, const HPComplex& s) {
+  if (!have_ref_) return NM_ERR_STATE;
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  f.nr = rows();
+  f.nc = cols();
+  f.epsre.resize(f.nc);
+  f.epsim.resize(f.nr);
+  hpf p(L), y(L);
+  for (int c = 0; c < f.nc; c++) {
+    pixel_re(p, cen.re, s.re, (long)(c - cols() / 2));
+    hp_sub(y, p, X[0].re);
+    f.epsre[c] = y.get_d();
+  }
+  for (int r = 0; r < f.nr; r++) {
+    pixel_re(p, cen.im, s.im, (long)(rows() / 2 - r - 1));
+    hp_sub(y, p, X[0].im);
+    f.epsim[r] = y.get_d();
+  }
+  int m = classify_frame(cen, s, f.nr, f.nc);
+  if (m >= 0) {
+    f.card_mode = m;
+    f.mask.clear();
+  } else {
+    // the cardioid or bulb boundary crosses this deep frame: test every sample
+    // in HP (mandelbrot.cpp:149) and hand the device a bit mask
+    f.card_mode = CARD_MASK;
+    size_t n = (size_t)f.nr * f.nc;
+    f.mask.assign((n + 31) / 32, 0u);
+    std::vector<uint8_t> in(n, 0);
+    parallel_for((long)f.nr, [&](long r) {
+      hpf pr(L), pi(L);
+      pixel_re(pi, cen.im, s.im, (long)(rows() / 2 - r - 1));
+      for (int c = 0; c < f.nc; c++) {
+        pixel_re(pr, cen.re, s.re, (long)(c - cols() / 2));
+        in[(size_t)r * f.nc + c] = inCardioidHP(pr, pi) ? 1 : 0;
+      }
+    }, 1);
+    for (size_t i = 0; i < n; i++)
+      if (in[i]) f.mask[i / 32] |= 1u << (i % 32);
+  }
+  return NM_OK;
+}
+
+// ------------------------------------------------------------------ render
+std::string Mandelbrot::snapshot() const {
+  std::string s;
+  auto add = [&](const void* p, size_t n) { s.append((const char*)p, n); };
+  for (const hpf* h : {&center.re, &center.im, &sz.re, &sz.im}) {
+    add(&h->sign, sizeof(h->sign));
+    add(&h->exp, sizeof(h->exp));
+    add(h->m, sizeof(uint64_t) * h->L);
+  }
+  add(&N, sizeof N);
+  add(&error_tolerance, sizeof error_tolerance);
+  add(&bailout, sizeof bailout);
+  add(&delta_mode, sizeof delta_mode);
+  add(&grid.nr, sizeof grid.nr);
+  add(&grid.nc, sizeof grid.nc);
+  add(&ref_serial_, sizeof ref_serial_);
+  return s;
+}
+
+int Mandelbrot::render(RenderOpts* opts, bool copy_back) {
+  if (N <= 0 || rows() <= 0 || cols() <= 0) return NM_ERR_ARG;
+  Engine* eng = nullptr;
+  int st = Engine::instance(&eng);
+  if (st != NM_OK) return st;
+  RenderOpts o = opts ? *opts : RenderOpts();
+  o.N = N;
+  o.bailout = bailout;
+  o.tol = error_tolerance;
+  o.delta_mode = delta_mode;
+  EscapeValue* out = copy_back ? grid.values.data() : nullptr;
+  if (useHardware()) {
+    HwFrameHost f;
+    buildHwFrame(f);
+    st = eng->render_hw({&f}, o, out, &stats_);
+  } else {
+    if (!have_ref_) return NM_ERR_STATE;  // precompute() first (mandelbrot.cpp:161)
+    if (std::fabs(o.tol - error_tolerance) > 0) return NM_ERR_ARG;
+    st = eng->set_reference(ref_, ref_serial_);
+    if (st != NM_OK) return st;
+    SaFrameHost f;
+    st = buildSaFrame(f);
+    if (st != NM_OK) return st;
+    st = eng->render_sa({&f}, o, out, &stats_);
+  }
+  if (st == NM_OK) {
+    rendered_ = copy_back;
+    snap_ = snapshot();
+  }
+  return st;
+}
+
+// mandelbrot.cpp:269-283.  The first call for a view renders the whole frame;
+// later calls for the same view find their row already in the host grid.
+int Mandelbrot::computeRow(int r) {
+  if (r < 0 || r >= rows()) return NM_ERR_ARG;
+  if (rendered_ && snap_ == snapshot()) return NM_OK;
+  return render(nullptr, true);
+}
+
+int Mandelbrot::computeAll() { return render(nullptr, true); }
+
+// A zoom-video batch: every frame shares this object's centre, reference orbit
+// and grid size; frame k has sz = width_k / cols (square samples).  All frames go
+// to the device in one start launch and one persistent iterate launch.
+int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts* opts, EscapeValue* out) {
+  if (widths.empty() || N <= 0) return NM_ERR_ARG;
+  Engine* eng = nullptr;
+  int st = Engine::instance(&eng);
+  if (st != NM_OK) return st;
+  RenderOpts o = opts ? *opts : RenderOpts();
+  o.N = N;
+  o.bailout = bailout;
+  o.tol = error_tolerance;
+  o.delta_mode = delta_mode;
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  std::vector<HPComplex> sizes(widths.size());
+  bool all_hw = true, any_hw = false;
+  for (size_t k = 0; k < widths.size(); k++) {
+    hpf w(L);
+    if (!hp_set_str(w, widths[k].c_str()) || w.sign <= 0) return NM_ERR_ARG;
+    sizes[k].re = hpf(L);
+    hp_div_ui(sizes[k].re, w, (unsigned long)cols());
+    sizes[k].im = sizes[k].re;
+    bool hw = sizes[k].re.get_d() >= kMinPreview;
+    all_hw = all_hw && hw;
+    any_hw = any_hw || hw;
+  }
+  if (any_hw && !all_hw) return NM_ERR_ARG;  // one path per batch
+  if (all_hw) {
+    std::vector<HwFrameHost> fs(widths.size());
+    std::vector<const HwFrameHost*> ptr;
+    for (size_t k = 0; k < fs.size(); k++) {
+      buildHwFrame(fs[k], center, sizes[k]);
+      ptr.push_back(&fs[k]);
+    }
+    return eng->render_hw(ptr, o, out, &stats_);
+  }
+  if (!have_ref_) return NM_ERR_STATE;
+  st = eng->set_reference(ref_, ref_serial_);
+  if (st != NM_OK) return st;
+  std::vector<SaFrameHost> fs(widths.size());
+  std::vector<const SaFrameHost*> ptr;
+  for (size_t k = 0; k < fs.size(); k++) {
+    st = buildSaFrame(fs[k], center, sizes[k]);
+    if (st != NM_OK) return st;
+    ptr.push_back(&fs[k]);
+  }
+  return eng->render_sa(ptr, o, out, &stats_);
+}
+
+// mandelbrot.cpp:285-290
+HPComplex Mandelbrot::pointAt(int r, int c, int sc) const {
+  HPComplex pt;
+  pixel_re(pt.re, center.re, sz.re, (long)(sc * c - cols() / 2));
+  pixel_re(pt.im, center.im, sz.im, (long)(rows() / 2 - sc * r - 1));
+  return pt;
+}
+
+// mandelbrot.cpp:292-295
+void Mandelbrot::translate(int dr, int dc, int sc) {
+  center.re = center.re - sz.re * (long)(dc * sc);
+  center.im = center.im + sz.im * (long)(dr * sc);
+}
+
+// mandelbrot.cpp:297-302
+void Mandelbrot::zoom(float scale) {
+  sz.re *= (1.0 / scale);
+  sz.im *= (1.0 / scale);
+  setPrecision();
+}
+
+// mandelbrot.cpp:304-316
+void Mandelbrot::zoomAt(float scale, int r, int c, int sc) {
+  HPComplex pt;
+  pixel_re(pt.re, center.re, sz.re, (long)(sc * c - cols() / 2));
+  pixel_re(pt.im, center.im, sz.im, (long)(rows() / 2 - sc * r - 1));
+  sz.re *= (1.0 / scale);
+  sz.im *= (1.0 / scale);
+  center.re = pt.re - sz.re * (long)(sc * c - cols() / 2);
+  center.im = pt.im - sz.im * (long)(rows() / 2 - sc * r - 1);
+  setPrecision();
+}
+
+// mandelbrot.cpp:318
+const EscapeValue& Mandelbrot::at(int r, int c) { return grid.at(r, c); }
+
+// mandelbrot.cpp:320-332 (float sum, r1 outer, c1 inner)
+EscapeValue Mandelbrot::at(int r, int c, int sc) {
+  float sum = 0.0f;
+  for (int r1 = 0; r1 < sc; r1++)
+    for (int c1 = 0; c1 < sc; c1++) {
+      const EscapeValue& e = grid.at(sc * r + r1, sc * c + c1);
+      sum += e.iterations + e.smoothing;
+    }
+  sum /= sc * sc;
+  EscapeValue o;
+  o.iterations = (int)sum;
+  o.smoothing = sum - o.iterations;
+  return o;
+}
+
+// mandelbrot.cpp:334-346
+int Mandelbrot::scaleUp(int sc) {
+  if (sc <= 0) return NM_ERR_ARG;
+  RenderGrid scaled(rows() * sc, cols() * sc);
+  for (int r = 0; r < scaled.nr; r++)
+    for (int c = 0; c < scaled.nc; c++) scaled.at(r, c) = grid.at(r / sc, c / sc);
+  grid = std::move(scaled);
+  sz.re = sz.re / sc;
+  sz.im = sz.im / sc;
+  setPrecision();
+  return NM_OK;
+}
+
+// mandelbrot.cpp:348-360, the averaging done by the device kernel
+int Mandelbrot::scaleDown(int sc) {
+  if (sc <= 0) return NM_ERR_ARG;
+  RenderGrid scaled(rows() / sc, cols() / sc);
+  if (scaled.nr > 0 && scaled.nc > 0) {
+    Engine* eng = nullptr;
+    int st = Engine::instance(&eng);
+    if (st != NM_OK) return st;
+    st = eng->downsample(grid.values.data(), rows(), cols(), sc, scaled.values.data());
+    if (st != NM_OK) return st;
+  }
+  grid = std::move(scaled);
+  sz.re = sz.re * (long)sc;
+  sz.im = sz.im * (long)sc;
+  setPrecision();
+  return NM_OK;
+}
+
+}  // namespace nm

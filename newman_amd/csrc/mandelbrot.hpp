@@ -1,0 +1,111 @@
+// mandelbrot.hpp - drop-in for newman's render core (mandelbrot.h:7-53).
+//
+// Same public surface: error_tolerance, N, center, sz; rows/cols, useHardware,
+// precompute, computeRow, pointAt, translate, zoom, zoomAt, at(r,c), at(r,c,sc),
+// scaleUp, scaleDown, loadLegacy, load, save.  The reference orbit and series are
+// computed serially on the host in HP (findProbe is threaded over probes, with
+// the same "longest orbit, first on ties" choice) and uploaded once; computeRow
+// renders the whole frame on the GPU on its first call for a view and then
+// serves rows from the host copy of the grid.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "hpfloat.hpp"
+
+namespace nm {
+
+// grid.h:6-25
+class RenderGrid {
+ public:
+  using EscapeValue = nm::EscapeValue;
+  int nr = 0, nc = 0;
+  std::vector<EscapeValue> values;
+  RenderGrid() = default;
+  RenderGrid(int nr_, int nc_) : nr(nr_), nc(nc_), values((size_t)nr_ * nc_, EscapeValue{0, 0.0f}) {}
+  EscapeValue& at(int r, int c) { return values[(size_t)r * nc + c]; }
+  const EscapeValue& at(int r, int c) const { return values[(size_t)r * nc + c]; }
+};
+
+class Mandelbrot {
+ public:
+  double error_tolerance;  // mandelbrot.h:23
+  int N;                   // mandelbrot.h:24
+  HPComplex center, sz;    // mandelbrot.h:25
+
+  // Extensions of this engine (not in the reference): bailout radius (the
+  // reference's constant 1024, mandelbrot.cpp:58) and the delta arithmetic.
+  double bailout = 1024.0;
+  int delta_mode = DELTA_FP64;
+
+  Mandelbrot();
+  Mandelbrot(int nr, int nc);
+
+  int loadLegacy(const char* fn);  // mandelbrot.cpp:19-35 (returns a status)
+  int rows() const { return grid.nr; }
+  int cols() const { return grid.nc; }
+
+  bool useHardware();
+  int precompute();
+  int computeRow(int r);
+  int computeAll();  // every row in one launch
+
+  HPComplex pointAt(int r, int c, int sc = 1) const;
+  void translate(int dr, int dc, int sc = 1);
+  void zoom(float scale);
+  void zoomAt(float scale, int r, int c, int sc = 1);
+
+  const EscapeValue& at(int r, int c);
+  EscapeValue at(int r, int c, int sc);
+
+  int scaleUp(int sc);
+  int scaleDown(int sc);
+
+  void load(const char* fn) { (void)fn; }  // mandelbrot.cpp:362 (stub)
+  void save(const char* fn) { (void)fn; }  // mandelbrot.cpp:364 (stub)
+
+  // ---- extensions used by the C ABI, tests and benchmarks
+  void setPrecision();                          // mandelbrot.cpp:37-56
+  long precisionBits() const { return bits_; }
+  int setView(int N, const char* cre, const char* cim, const char* width);
+  int precomputeAt(const char* re, const char* im);  // explicit reference point
+  int referenceLength() const { return (int)ref_.len; }
+  int probeWinner() const { return winner_; }
+  const SaReferenceHost& reference() const { return ref_; }
+  RenderGrid& grid_ref() { return grid; }
+  const RenderStats& lastStats() const { return stats_; }
+  int render(RenderOpts* opts = nullptr, bool copy_back = true);
+  // frames with this object's centre, reference and grid size, one per width
+  // (width = cols * sz); outputs concatenated
+  int renderFrames(const std::vector<std::string>& widths, RenderOpts* opts, EscapeValue* out);
+  int buildHwFrame(HwFrameHost& f) { return buildHwFrame(f, center, sz); }
+  int buildSaFrame(SaFrameHost& f) { return buildSaFrame(f, center, sz); }
+  int buildHwFrame(HwFrameHost& f, const HPComplex& c, const HPComplex& s);
+  int buildSaFrame(SaFrameHost& f, const HPComplex& c, const HPComplex& s);
+  void invalidate() { rendered_ = false; }
+
+ protected:
+  RenderGrid grid;
+  std::vector<HPComplex> X;  // reference orbit, plus one extra element X[len]
+  SaReferenceHost ref_;
+  long bits_ = 64;
+  int winner_ = -1;
+  bool have_ref_ = false;
+  bool rendered_ = false;
+  uint64_t ref_serial_ = 0;  // identifies this object's uploaded reference
+  std::string snap_;  // view snapshot of the last render
+  RenderStats stats_;
+
+  void findProbe();
+  int computeOrbit(const HPComplex& X0, bool store);
+  void computeSeries();
+  void buildThresholds();
+  std::string snapshot() const;
+  int uploadReference();
+};
+
+// per-pixel helpers shared by the frame builders and tests
+bool inCardioidHP(const hpf& re, const hpf& im);  // mandelbrot.cpp:63-71
+
+}  // namespace nm
