@@ -128,7 +128,8 @@ int nm_render(nm_mandelbrot* h, int block_k, int waves_per_simd, int repeats, in
               nm_render_stats* stats);
 /* render a batch of frames that share h's reference orbit (zoom video): frame
  * k has h's centre and size width_k = cols * sz; outputs are concatenated
- * row-major grids.  widths are decimal strings. */
+ * row-major grids (out = NULL keeps them on the device).  widths are decimal
+ * strings. */
 int nm_render_frames(nm_mandelbrot* h, const char* const* widths, int nframes, int block_k, int waves_per_simd,
                      int repeats, nm_escape_value* out, nm_render_stats* stats);
 /* the text of the last engine error */

@@ -314,11 +314,13 @@ class Mandelbrot:
                "nm_render")
         return st.as_dict()
 
-    def render_frames(self, widths, block_k=0, waves_per_simd=0, repeats=1):
+    def render_frames(self, widths, block_k=0, waves_per_simd=0, repeats=1, copy_back=True):
+        """Frames sharing this view's centre and reference; copy_back=False leaves
+        the grids on the device and returns (None, stats)."""
         n = len(widths)
         arr = (C.c_char_p * n)(*[_b(str(w)) for w in widths])
-        out = np.zeros(n * self.rows() * self.cols(), dtype=ESCAPE_DTYPE)
+        out = np.zeros(n * self.rows() * self.cols(), dtype=ESCAPE_DTYPE) if copy_back else None
         st = RenderStats()
-        _check(lib().nm_render_frames(self._h, arr, n, block_k, waves_per_simd, repeats, out.ctypes.data,
-                                      C.byref(st)), "nm_render_frames")
-        return out.reshape(n, self.rows(), self.cols()), st.as_dict()
+        _check(lib().nm_render_frames(self._h, arr, n, block_k, waves_per_simd, repeats,
+                                      out.ctypes.data if copy_back else None, C.byref(st)), "nm_render_frames")
+        return (out.reshape(n, self.rows(), self.cols()) if copy_back else None), st.as_dict()

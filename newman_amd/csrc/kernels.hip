@@ -6,20 +6,25 @@
 //   col/row tables  plain path: truncated coordinate + double-double remainder;
 //                   series path: epsilon = descend(Y0 - X[0]) per column / row
 //   out[]           EscapeValue {int32, float} row-major per frame (grid.h layout)
-//   active[]        u32 sample ids that still need iterating (compacted)
-//   state[]         per active slot: delta (2 x f64), epsilon (2 x f64), it, m
+//   flag[]          u8 per sample: 1 = handed to the iteration kernel
+//   state[]         per sample (series path): delta, epsilon (4 x f64); it, m
+//   active[]        u32 sample ids that iterate, compacted in sample order
 //   Zd[]            reference orbit as f64 pairs, Zd[0] = 0, padded past its end
 //
-// Two kernels per path:
-//   *_start   one thread per sample: cardioid/bulb test (plain path) or the
+// Pipeline per render, with no device-scope atomics on the hot path (one
+// contended counter serialises at ~10 ns per atomic on this part):
+//   *_start   tiles of 1024 samples: cardioid/bulb test (plain path) or the
 //             reference's per-sample series start (series path); resolves what it
-//             can and appends the rest to active[] with wave ballots (one atomic
-//             per wave).
-//   *_iterate persistent waves: each lane holds one sample; K iterations run
-//             without exits, escapes are found afterwards by a per-lane replay of
-//             the block (escape is absorbing, so |z| at the block end is enough);
-//             finished lanes are refilled from a wave-owned chunk of active[]
-//             (ballot + popcount ranks, one atomic per 64 samples).
+//             can, flags the rest and counts them per tile
+//   scan      exclusive scan of the tile counts (one workgroup)
+//   scatter   compacts flagged sample ids into active[] in sample order
+//   *_iterate persistent waves: each lane holds one sample; blocks of K steps
+//             run without exits (the first escape of a lane records its step and
+//             |z|^2 and poisons the lane with NaN: escape is absorbing); lanes are
+//             refilled (ballot + popcount ranks) from 64-sample chunks of
+//             active[], chunk c owned by wave c mod W (interleaved, so every wave
+//             gets an even share of the frame)
+// Counters go to per-workgroup partial slots summed on the host.
 // This file is compiled with -ffp-contract=off: the plain path must reproduce the
 // reference's double arithmetic bit for bit (complex.h:19-31, makefile:3 has no
 // FMA); the perturbation path writes its FMAs explicitly.
@@ -48,49 +53,64 @@ struct DevFrame {
 };
 
 enum CounterIdx {
-  C_ACTIVE = 0,   // entries appended to active[] (u32 view of the low word)
-  C_WORK = 1,     // chunk cursor of the iterate kernel
-  C_CARDIOID = 2,
-  C_SA_ESC = 3,
-  C_EXECUTED = 4,
-  C_SKIPPED = 5,
-  C_REBASED = 6,
-  C_CREDITED = 7,
-  C_COUNT = 8
+  C_CARDIOID = 0,
+  C_SA_RES = 1,
+  C_EXECUTED = 2,
+  C_SKIPPED = 3,
+  C_REBASED = 4,
+  C_CREDITED = 5,
+  C_COUNT = 6
 };
 
-constexpr int kChunk = 64;  // samples a wave takes per atomic
+constexpr int kThreads = 256;           // workgroup size of every kernel
+constexpr int kPerThread = 4;           // samples per thread per start tile
+constexpr int kTile = kThreads * kPerThread;
+constexpr int kChunk = 64;              // active samples per chunk (one per lane)
+constexpr int kHead = 64;               // orbit entries kept in LDS for rebased lanes (>= max K)
+constexpr int kPartSlots = 8192;        // per-workgroup partial counter slots
 
-struct PlainArgs {
+// partial-slot ranges of the kernels of one render
+constexpr int kSlotStart = 0;
+constexpr int kSlotIter = 4096;
+constexpr int kSlotSum = 6144;
+
+struct Common {
   const DevFrame* frames;
   int nframes;
+  uint32_t frame_size;  // samples per frame (all frames of a set have one size)
+  double inv_frame;     // 1.0 / frame_size
   uint32_t total;
-  const double* cre;
-  const double* crelo;
-  const double* cim;
-  const double* cimlo;
   EscapeValue* out;
+  double* r2;  // |z|^2 at the escape of iterated samples (per sample)
+  double4* state;  // per sample: plain (x, y, cx, cy); series (dr, di, er, ei)
+  int2* state_i;   // per sample: plain (k, -); series (it, m)
+  uint8_t* flag;
+  uint32_t* tile_count;
   uint32_t* active;
-  unsigned long long* cnt;
+  const uint32_t* nactive;  // written by the scan
+  unsigned long long* part;
   int N;
   double b2, b2m, log_b;
 };
 
+struct PlainArgs {
+  Common c;
+  uint32_t nrows, ncols;  // frame size
+  double inv_ncols;
+  const double* cre;
+  const double* crelo;
+  const double* cim;
+  const double* cimlo;
+};
+
 struct SeriesArgs {
-  const DevFrame* frames;
-  int nframes;
-  uint32_t total;
+  Common c;
   const double* epsre;
   const double* epsim;
   const uint32_t* mask;
-  EscapeValue* out;
-  uint32_t* active;
-  double4* state;    // dr, di, er, ei
-  int2* state_i;     // it, m
-  unsigned long long* cnt;
   // reference
-  const double2* Zd;  // len + 2 (+ padding)
-  const double2* Xlo; // len
+  const double2* Zd;   // len + 2 (+ padding)
+  const double2* Xlo;  // len
   const double2* A;
   const double2* B;
   const double2* C;
@@ -102,45 +122,43 @@ struct SeriesArgs {
   const int* SP;
   int len;
   int mend;  // len + 1
-  int N;
-  double tol, b2, b2m, log_b;
+  double tol;
 };
 
 // ------------------------------------------------------------ device helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-__device__ __forceinline__ int find_frame(const DevFrame* f, int n, uint32_t gid) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (f[mid].sample_base <= gid) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
+// x / d for x < 2^32 with inv = 1.0 / d: (x + 1/2) / d is at least 1/(2d) away
+// from an integer and the product errs by at most (x / d) 2^-52 < 1/(2d), so the
+// truncated product is the exact quotient
+__device__ __forceinline__ uint32_t udiv(uint32_t x, double inv) { return (uint32_t)(((double)x + 0.5) * inv); }
+
+// every frame of a frame set has the same size: frame = sample id / frame size
+__device__ __forceinline__ uint32_t frame_of(uint32_t gid, double inv_frame) { return udiv(gid, inv_frame); }
 
 // mandelbrot.cpp:133-136 (log(bailout) comes from the host)
 __device__ __forceinline__ float smooth_of(double r2, double log_b) {
   return (float)(1.0 - log2(0.5 * log(r2) / log_b));
 }
 
-// wave-aggregated u64 counter add
-__device__ __forceinline__ void wave_add(unsigned long long* p, unsigned long long v) {
-  // butterfly reduction over the 64 lanes
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  if (lane_id() == 0 && v) atomicAdd(p, v);
+  return v;
 }
 
-// Append `want` lanes to a list: returns the slot for wanting lanes.
-__device__ __forceinline__ uint32_t wave_append(unsigned long long* counter, bool want) {
-  unsigned long long m = __ballot(want);
-  if (m == 0) return 0;
-  int leader = __ffsll((long long)m) - 1;
-  uint32_t base = 0;
-  if (lane_id() == leader) base = (uint32_t)atomicAdd(counter, (unsigned long long)__popcll(m));
-  base = __shfl(base, leader);
-  unsigned long long below = m & ((1ull << lane_id()) - 1ull);
-  return base + (uint32_t)__popcll(below);
+// Adds v (per thread) into the workgroup's partial slot: wave reduction, LDS
+// accumulation, one plain store per workgroup.  Called by all threads.
+__device__ __forceinline__ void block_partial(unsigned long long* part, int slot, const unsigned long long* v,
+                                              const int* idx, int n) {
+  __shared__ unsigned long long acc[C_COUNT];
+  if (threadIdx.x < C_COUNT) acc[threadIdx.x] = 0ull;
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    unsigned long long s = wave_sum(v[k]);
+    if (lane_id() == 0 && s) atomicAdd(&acc[idx[k]], s);
+  }
+  __syncthreads();
+  if (threadIdx.x < C_COUNT) part[(size_t)slot * C_COUNT + threadIdx.x] += acc[threadIdx.x];
 }
 
 // ---- double-double arithmetic for the plain path's cardioid test -----------
@@ -165,7 +183,6 @@ __device__ __forceinline__ dd dd_add(dd a, dd b) {
   s.lo += t.lo;
   return quick_two_sum(s.hi, s.lo);
 }
-__device__ __forceinline__ dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
 __device__ __forceinline__ dd dd_mul(dd a, dd b) {
   double p = a.hi * b.hi;
   double e = fma(a.hi, b.hi, -p);  // exact low part of the product
@@ -205,113 +222,247 @@ __device__ __forceinline__ double trunc_sum3(double hi, double lo, double d) {
   return r;
 }
 
-// ================================================================ plain path
-// getIterationsHW (mandelbrot.cpp:231-254), start kernel: cardioid/bulb test,
-// compaction of the samples that iterate.
-__global__ __launch_bounds__(256) void k_plain_start(PlainArgs a) {
-  unsigned long long ncard = 0;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < a.total; base += stride) {
-    uint32_t gid = base + threadIdx.x;
-    bool valid = gid < a.total;
-    bool in = false;
-    if (valid) {
-      const DevFrame& f = a.frames[find_frame(a.frames, a.nframes, gid)];
-      uint32_t local = gid - f.sample_base;
-      uint32_t r = local / f.nc, c = local - r * f.nc;
-      if (f.card_mode == CARD_ALL) in = true;
-      else if (f.card_mode == CARD_TEST_DD)
-        in = dd_in_cardioid(a.cre[f.tab_c + c], a.crelo[f.tab_c + c], a.cim[f.tab_r + r], a.cimlo[f.tab_r + r]);
-      if (in) {
-        a.out[f.out_base + local] = EscapeValue{a.N, 0.0f};
-        ncard++;
-      }
-    }
-    bool want = valid && !in;
-    uint32_t slot = wave_append(&a.cnt[C_ACTIVE], want);
-    if (want) a.active[slot] = gid;
+// Each workgroup walks its tiles; the block-wide count of flagged samples of a
+// tile goes to tile_count[tile].
+__device__ __forceinline__ void tile_count_store(uint32_t* tile_count, uint32_t tile, uint32_t mine) {
+  __shared__ uint32_t wsum[kThreads / 64];
+  unsigned long long s = wave_sum((unsigned long long)mine);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = (uint32_t)s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kThreads / 64; w++) t += wsum[w];
+    tile_count[tile] = t;
   }
-  wave_add(&a.cnt[C_CARDIOID], ncard);
+  __syncthreads();
 }
 
-// Iterate kernel: z <- sq(z) + z0 with the reference's operation order,
-// count = index of the update that escapes (mandelbrot.cpp:243-246).
+// ================================================================ plain path
+// getIterationsHW (mandelbrot.cpp:231-254), start kernel: the cardioid/bulb
+// test, then up to kPreIter updates per sample with the exact escape test (most
+// samples of a shallow view escape here, converged); the rest are flagged for
+// the persistent kernel with their state (z, c, updates done).
+constexpr int kPreIter = 64;
+
+__global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
+  const Common& c = a.c;
+  unsigned long long ncard = 0, nres = 0, executed = 0;
+  const uint32_t ntiles = (c.total + kTile - 1) / kTile;
+  const int N = c.N;
+  const int pre = min(kPreIter, N);
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t mine = 0;
+    for (int j = 0; j < kPerThread; j++) {
+      uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
+      if (gid >= c.total) continue;
+      const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
+      uint32_t local = gid - f.sample_base;
+      uint32_t r = udiv(local, a.inv_ncols), col = local - r * a.ncols;
+      bool in = false;
+      const double cx = a.cre[f.tab_c + col], cy = a.cim[f.tab_r + r];
+      if (f.card_mode == CARD_ALL) in = true;
+      else if (f.card_mode == CARD_TEST_DD)
+        in = dd_in_cardioid(cx, a.crelo[f.tab_c + col], cy, a.cimlo[f.tab_r + r]);
+      bool want = false;
+      if (in) {
+        c.out[gid] = EscapeValue{N, 0.0f};
+        ncard++;
+      } else {
+        // z <- sq(z) + z0 in the reference's operation order (complex.h:19-27)
+        double x = cx, y = cy, r2 = x * x, i2 = y * y, mg = 0.0;
+        int k = 0;
+        for (; k < pre; k++) {
+          double t = x + x;
+          y = t * y + cy;
+          x = r2 - i2 + cx;
+          r2 = x * x;
+          i2 = y * y;
+          mg = r2 + i2;
+          if (mg > c.b2) break;
+        }
+        executed += (unsigned long long)min(k + 1, pre);
+        if (k < pre) {
+          c.out[gid] = EscapeValue{k, smooth_of(mg, c.log_b)};
+          nres++;
+        } else if (k >= N) {
+          c.out[gid] = EscapeValue{N, 0.0f};
+          nres++;
+        } else {
+          c.state[gid] = make_double4(x, y, cx, cy);
+          c.state_i[gid] = make_int2(k, 0);
+          want = true;
+        }
+      }
+      c.flag[gid] = want ? 1 : 0;
+      mine += want ? 1 : 0;
+    }
+    tile_count_store(c.tile_count, tile, mine);
+  }
+  const unsigned long long v[3] = {ncard, nres, executed};
+  const int idx[3] = {C_CARDIOID, C_SA_RES, C_EXECUTED};
+  block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 3);
+}
+
+// ============================================================ compaction
+// exclusive scan of tile_count (in place) by one workgroup; total -> *nactive
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint32_t ntiles, uint32_t* nactive) {
+  __shared__ uint32_t sums[1024];
+  const uint32_t per = (ntiles + 1023) / 1024;
+  const uint32_t b = threadIdx.x * per, e = min(b + per, ntiles);
+  uint32_t s = 0;
+  for (uint32_t i = b; i < e; i++) s += tile_count[i];
+  sums[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+    uint32_t v = threadIdx.x >= (unsigned)off ? sums[threadIdx.x - off] : 0u;
+    __syncthreads();
+    sums[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = sums[threadIdx.x] - s;
+  for (uint32_t i = b; i < e; i++) {
+    uint32_t t = tile_count[i];
+    tile_count[i] = run;
+    run += t;
+  }
+  if (threadIdx.x == 1023) *nactive = sums[1023];
+}
+
+// flagged sample ids -> active[] in sample order
+__global__ __launch_bounds__(kThreads) void k_scatter(const uint8_t* flag, const uint32_t* tile_off, uint32_t total,
+                                                      uint32_t* active) {
+  __shared__ uint32_t wcount[kThreads / 64];
+  const uint32_t ntiles = (total + kTile - 1) / kTile;
+  const int wave = threadIdx.x >> 6;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t run = tile_off[tile];
+    for (int j = 0; j < kPerThread; j++) {
+      uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
+      bool f = gid < total && flag[gid];
+      unsigned long long m = __ballot(f);
+      if (lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
+      __syncthreads();
+      uint32_t before = 0, all = 0;
+      for (int w = 0; w < kThreads / 64; w++) {
+        before += w < wave ? wcount[w] : 0u;
+        all += wcount[w];
+      }
+      if (f) active[run + before + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = gid;
+      run += all;
+      __syncthreads();
+    }
+  }
+}
+
+// ---- work pipeline of a persistent wave -------------------------------------
+// Chunk c (64 consecutive entries of active[]) belongs to wave c mod W.  Each
+// lane holds one entry of three chunks: CUR (decoded: handed to lanes that need
+// work through cross-lane shuffles), NEXT (its decode loads in flight) and the
+// chunk after (its sample-id load in flight), so a refill reads registers only.
+__device__ __forceinline__ uint32_t chunk_items(uint32_t chunk, uint32_t total) {
+  uint64_t b = (uint64_t)chunk * kChunk;
+  return b >= total ? 0u : (uint32_t)min((uint64_t)kChunk, (uint64_t)total - b);
+}
+
+__device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t chunk, uint32_t total) {
+  uint32_t n = chunk_items(chunk, total);
+  return (uint32_t)lane_id() < n ? active[chunk * kChunk + lane_id()] : 0u;
+}
+
+// Iterate kernel for the samples that outlive the start kernel: z <- sq(z) + z0
+// with the reference's operation order, count = index of the update that
+// escapes (mandelbrot.cpp:243-246).  A block runs K updates with no test; escape
+// is absorbing (|z| > 1024 keeps growing, then overflows to inf/NaN), so a block
+// whose last |z|^2 is not <= bailout^2 (or that crosses N) is replayed from its
+// saved z with the test, in registers.
 template <int K>
-__global__ __launch_bounds__(256) void k_plain_iterate(PlainArgs a) {
-  const uint32_t total = (uint32_t)a.cnt[C_ACTIVE];
-  unsigned int* work = (unsigned int*)&a.cnt[C_WORK];
-  uint32_t cpos = 0, cend = 0;
-  bool exhausted = false;
+__global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
+  const Common& c = a.c;
+  const uint32_t total = *c.nactive;
+  const uint32_t W = gridDim.x * (blockDim.x >> 6);
+  uint32_t ccur = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  uint32_t ncur = chunk_items(ccur, total), nnext = chunk_items(ccur + W, total);
+  uint32_t cur_pix = load_pix(c.active, ccur, total), nxt_pix = load_pix(c.active, ccur + W, total);
+  uint32_t far_pix = load_pix(c.active, ccur + 2 * W, total);
+  double4 cur_s = make_double4(0.0, 0.0, 0.0, 0.0), nxt_s = cur_s;
+  int cur_k = 0, nxt_k = 0;
+  if ((uint32_t)lane_id() < ncur) {
+    cur_s = c.state[cur_pix];
+    cur_k = c.state_i[cur_pix].x;
+  }
+  if ((uint32_t)lane_id() < nnext) {
+    nxt_s = c.state[nxt_pix];
+    nxt_k = c.state_i[nxt_pix].x;
+  }
+  uint32_t pos = 0;
   bool act = false;
   uint32_t pix = 0;
   double cx = 0.0, cy = 0.0, x = 0.0, y = 0.0, r2 = 0.0, i2 = 0.0;
   int k = 0;
   unsigned long long executed = 0;
+  const int N = c.N;
   for (;;) {
     unsigned long long need = __ballot(!act);
-    while (need && !exhausted) {
-      if (cpos >= cend) {
-        uint32_t b = 0;
-        if (lane_id() == 0) b = atomicAdd(work, (unsigned int)kChunk);
-        b = __shfl(b, 0);
-        cpos = b;
-        cend = min(b + (uint32_t)kChunk, total);
-        if (cpos >= total) {
-          exhausted = true;
-          break;
-        }
-      }
-      uint32_t avail = cend - cpos;
-      uint32_t nneed = (uint32_t)__popcll(need);
-      uint32_t take = min(nneed, avail);
+    while (need && ncur > 0) {
+      uint32_t take = min((uint32_t)__popcll(need), ncur - pos);
       uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane_id()) - 1ull));
+      int src = (int)min(pos + rank, ncur - 1);
+      uint32_t g_pix = __shfl(cur_pix, src);
+      double g0 = __shfl(cur_s.x, src), g1 = __shfl(cur_s.y, src), g2 = __shfl(cur_s.z, src),
+             g3 = __shfl(cur_s.w, src);
+      int g4 = __shfl(cur_k, src);
       if (!act && rank < take) {
-        pix = a.active[cpos + rank];
-        const DevFrame& f = a.frames[find_frame(a.frames, a.nframes, pix)];
-        uint32_t local = pix - f.sample_base;
-        uint32_t r = local / f.nc, c = local - r * f.nc;
-        cx = a.cre[f.tab_c + c];
-        cy = a.cim[f.tab_r + r];
-        x = cx;
-        y = cy;
+        pix = g_pix;
+        x = g0;
+        y = g1;
+        cx = g2;
+        cy = g3;
+        k = g4;
         r2 = x * x;
         i2 = y * y;
-        k = 0;
         act = true;
       }
-      cpos += take;
+      pos += take;
+      if (pos == ncur) {  // rotate: NEXT becomes CUR, the far chunk is decoded
+        ccur += W;
+        ncur = nnext;
+        cur_pix = nxt_pix;
+        cur_s = nxt_s;
+        cur_k = nxt_k;
+        nnext = chunk_items(ccur + W, total);
+        nxt_pix = far_pix;
+        if ((uint32_t)lane_id() < nnext) {
+          nxt_s = c.state[nxt_pix];
+          nxt_k = c.state_i[nxt_pix].x;
+        }
+        far_pix = load_pix(c.active, ccur + 2 * W, total);
+        pos = 0;
+      }
       need = __ballot(!act);
     }
     if (__ballot(act) == 0) break;
-    const bool fast = __ballot(act && k + K > a.N) == 0;
     const double x0 = x, y0 = y;
     const int k0 = k;
-    bool replay = act;
-    if (fast) {
 #pragma unroll
-      for (int j = 0; j < K; j++) {
-        double t = x + x;
-        y = t * y + cy;
-        x = r2 - i2 + cx;
-        r2 = x * x;
-        i2 = y * y;
-      }
-      k += K;
-      replay = act && !(r2 + i2 <= a.b2m);
-      if (act && !replay) executed += K;
+    for (int j = 0; j < K; j++) {
+      double t = x + x;
+      y = t * y + cy;
+      x = r2 - i2 + cx;
+      r2 = x * x;
+      i2 = y * y;
     }
-    if (replay) {
+    k += K;
+    if (act && (!(r2 + i2 <= c.b2) || k > N)) {  // an escape in the block, or N inside it
       x = x0;
       y = y0;
       k = k0;
       r2 = x * x;
       i2 = y * y;
       bool done = false;
-      int count = a.N;
-      float sm = 0.0f;
       for (int j = 0; j < K; j++) {
-        if (k >= a.N) {
+        if (k >= N) {
           done = true;
+          c.out[pix] = EscapeValue{N, 0.0f};
           break;
         }
         double t = x + x;
@@ -320,24 +471,30 @@ __global__ __launch_bounds__(256) void k_plain_iterate(PlainArgs a) {
         r2 = x * x;
         i2 = y * y;
         executed++;
-        double mg = r2 + i2;
-        if (mg > a.b2) {
+        double mg = r2 + i2;  // sqMag (mandelbrot.cpp:245)
+        if (mg > c.b2) {
           done = true;
-          count = k;
-          sm = smooth_of(mg, a.log_b);
+          c.out[pix] = EscapeValue{k, 0.0f};
+          c.r2[pix] = mg;  // smoothing: k_finalize
           break;
         }
         k++;
       }
-      if (!done && k >= a.N) done = true;
-      if (done) {
-        const DevFrame& f = a.frames[find_frame(a.frames, a.nframes, pix)];
-        a.out[f.out_base + (pix - f.sample_base)] = EscapeValue{count, sm};
+      if (!done && k >= N) {
+        done = true;
+        c.out[pix] = EscapeValue{N, 0.0f};
+      }
+      if (done) act = false;
+    } else if (act) {
+      executed += K;
+      if (k == N) {
+        c.out[pix] = EscapeValue{N, 0.0f};
         act = false;
       }
     }
   }
-  wave_add(&a.cnt[C_EXECUTED], executed);
+  const int idx[1] = {C_EXECUTED};
+  block_partial(c.part, kSlotIter + blockIdx.x, &executed, idx, 1);
 }
 
 // =============================================================== series path
@@ -382,20 +539,52 @@ __device__ __forceinline__ int first_le(const double* PM, int lo, int hi, double
 // exact test, and the magnitude prefixes prove the skipped exact tests would not
 // under- or overflow.  Otherwise the reference's linear scan runs.
 constexpr double kMarg = 4.0e-9;
-__device__ int sa_stop_index(const SeriesArgs& a, double2 e, double2 e2, double2 e3) {
-  const int last = a.len - 1;
-  if (last < 1) return 0;
+
+// log2 |e|^2 without under/overflow; -inf for e = 0
+__device__ __forceinline__ double log2_sqmag(double2 e) {
   double mx = fmax(fabs(e.x), fabs(e.y));
-  if (mx == 0.0) return 0;  // e = 0: b = c = 0 and the test never fires
+  if (mx == 0.0) return -INFINITY;
   int ex = ilogb(mx);
   double sx = ldexp(e.x, -ex), sy = ldexp(e.y, -ex);
-  double L = log2(sx * sx + sy * sy) + 2.0 * ex;
-  int ia = first_le(a.PM, 1, last, L + kMarg);
+  return log2(sx * sx + sy * sy) + 2.0 * ex;
+}
+
+__device__ __forceinline__ double wave_uniform(double x) {  // value of lane 0, in SGPRs
+  long long b = __double_as_longlong(x);
+  int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// [lo, hi]: the range that holds first_le(PM, 1, last, L + kMarg) for every lane
+// of the wave with need set.  Called by all lanes; PM is searched with scalar
+// loads at the wave's largest and smallest L.
+__device__ __forceinline__ int2 wave_stop_bracket(const SeriesArgs& a, double L, bool need) {
+  double lmax = need ? L : -INFINITY, lmin = need ? L : INFINITY;
+  for (int off = 32; off > 0; off >>= 1) {
+    lmax = fmax(lmax, __shfl_xor(lmax, off));
+    lmin = fmin(lmin, __shfl_xor(lmin, off));
+  }
+  lmax = wave_uniform(lmax);
+  lmin = wave_uniform(lmin);
+  const int last = a.len - 1;
+  if (last < 1 || !(lmax >= lmin)) return make_int2(1, last);
+  int lo = first_le(a.PM, 1, last, lmax + kMarg);
+  int hi = first_le(a.PM, lo, last, lmin + kMarg);
+  return make_int2(lo, min(hi, last));
+}
+
+__device__ int sa_stop_index(const SeriesArgs& a, double L, int2 br, double2 e2, double2 e3) {
+  const int last = a.len - 1;
+  if (last < 1) return 0;
+  if (L == -INFINITY) return 0;  // e = 0: b = c = 0 and the test never fires
+  int ia = first_le(a.PM, br.x, br.y, L + kMarg);
+  if (ia > br.y && br.y < last) ia = first_le(a.PM, br.y + 1, last, L + kMarg);
   int safe_idx = ia > last ? last : ia - 1;
   bool safe = L > -600.0 && L < 600.0;
   if (safe && safe_idx >= 1) {
-    safe = a.SP[safe_idx] == 0 && a.BX[safe_idx] + 2.0 * L < 960.0 && a.BN[safe_idx] + 2.0 * L > -960.0 &&
-           a.CX[safe_idx] + 3.0 * L < 960.0 && a.CN[safe_idx] + 3.0 * L > -960.0 && a.BN[safe_idx] + 2.0 * L > -900.0;
+    safe = a.SP[safe_idx] == 0 && a.BX[safe_idx] + 2.0 * L < 960.0 && a.BN[safe_idx] + 2.0 * L > -900.0 &&
+           a.CX[safe_idx] + 3.0 * L < 960.0 && a.CN[safe_idx] + 3.0 * L > -960.0;
   }
   int from = safe ? ia : 1;
   for (int i = from; i <= last; i++)
@@ -403,47 +592,61 @@ __device__ int sa_stop_index(const SeriesArgs& a, double2 e, double2 e2, double2
   return 0;
 }
 
-__global__ __launch_bounds__(256) void k_series_start(SeriesArgs a) {
-  unsigned long long ncard = 0, nesc = 0, skipped = 0;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < a.total; base += stride) {
-    uint32_t gid = base + threadIdx.x;
-    bool valid = gid < a.total;
-    bool want = false;
-    double2 dl = make_double2(0.0, 0.0), e = make_double2(0.0, 0.0);
-    int it = 0, m = 0;
-    if (valid) {
-      const DevFrame& f = a.frames[find_frame(a.frames, a.nframes, gid)];
-      uint32_t local = gid - f.sample_base;
-      uint32_t r = local / f.nc, c = local - r * f.nc;
-      bool in = f.card_mode == CARD_ALL;
-      if (f.card_mode == CARD_MASK) {
-        uint64_t bit = f.mask_base + local;
-        in = (a.mask[bit >> 5] >> (bit & 31)) & 1u;
+__global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
+  const Common& c = a.c;
+  unsigned long long ncard = 0, nres = 0, skipped = 0;
+  const uint32_t ntiles = (c.total + kTile - 1) / kTile;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t mine = 0;
+    for (int j = 0; j < kPerThread; j++) {
+      uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
+      bool valid = gid < c.total;
+      // phase 1: locate the sample, cardioid class, epsilon and log2 |e|^2
+      bool in = false;
+      uint64_t out_idx = 0;
+      double L = -INFINITY;
+      double2 e = make_double2(0.0, 0.0);
+      if (valid) {
+        const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
+        uint32_t local = gid - f.sample_base;
+        uint32_t r = local / f.nc, col = local - r * f.nc;
+        in = f.card_mode == CARD_ALL;
+        if (f.card_mode == CARD_MASK) {
+          uint64_t bit = f.mask_base + local;
+          in = (a.mask[bit >> 5] >> (bit & 31)) & 1u;
+        }
+        out_idx = f.out_base + local;
+        if (!in) {
+          e = make_double2(a.epsre[f.tab_c + col], a.epsim[f.tab_r + r]);
+          L = log2_sqmag(e);
+        }
       }
-      EscapeValue res{a.N, 0.0f};
-      bool resolved = true;
+      // phase 2 (all lanes): the wave's bracket of the stop-index search
+      const int2 br = wave_stop_bracket(a, L, valid && !in && L > -INFINITY);
+      // phase 3: the reference's series start for this sample
+      if (!valid) continue;
+      bool want = false;
+      EscapeValue res{c.N, 0.0f};
       if (in) {
         ncard++;
       } else {
-        e = make_double2(a.epsre[f.tab_c + c], a.epsim[f.tab_r + r]);
         // eps2 = sq(eps), eps3 = eps * eps2 (complex.h:19-21, mandelbrot.cpp:160-161)
         double2 e2 = make_double2(e.x * e.x - e.y * e.y, 2.0 * e.x * e.y);
         double2 e3 = cmul(e, e2);
-        int ua = sa_stop_index(a, e, e2, e3);
+        int ua = sa_stop_index(a, L, br, e2, e3);
         int dsize = ua ? max(ua - 3, 1) : a.len;
         int found = dsize - 1;
         double2 d = sa_term(a, found, e, e2, e3);
         double2 zf = a.Zd[found + 1], xl = a.Xlo[found];
         double yr = trunc_sum3(zf.x, xl.x, d.x), yi = trunc_sum3(zf.y, xl.y, d.y);
-        if (yr * yr + yi * yi > a.b2) {
+        if (yr * yr + yi * yi > c.b2) {
           // escape inside the skip: the reference's binary search (mandelbrot.cpp:186-206)
           int low = 0, high = dsize - 1, mid = dsize / 2;
           while (low <= high) {
             double2 dm = sa_term(a, mid, e, e2, e3);
             double2 zm = a.Zd[mid + 1], xm = a.Xlo[mid];
             double ym = trunc_sum3(zm.x, xm.x, dm.x), yn = trunc_sum3(zm.y, xm.y, dm.y);
-            if (!(ym * ym + yn * yn > a.b2)) low = mid + 1;
+            if (!(ym * ym + yn * yn > c.b2)) low = mid + 1;
             else {
               high = mid - 1;
               found = mid;
@@ -455,197 +658,211 @@ __global__ __launch_bounds__(256) void k_series_start(SeriesArgs a) {
           xl = a.Xlo[found];
           yr = trunc_sum3(zf.x, xl.x, d.x);
           yi = trunc_sum3(zf.y, xl.y, d.y);
-          res = EscapeValue{found, smooth_of(yr * yr + yi * yi, a.log_b)};
-          nesc++;
-        } else if (found + 1 < a.N) {
+          res = EscapeValue{found, smooth_of(yr * yr + yi * yi, c.log_b)};
+          nres++;
+        } else if (found + 1 < c.N) {
           // first perturbation step from Y[found] = X[found] + d (its escape
           // test was the one above); rebase if |Y| < |d|
           double dr = d.x, di = d.y;
-          m = found + 1;
+          int m = found + 1;
           double zr = zf.x, zi = zf.y;
           double pr = zr + dr, pi = zi + di;
-          double mag = fma(pr, pr, pi * pi);
+          double r2 = pr * pr + pi * pi;
           double dmag = fma(dr, dr, di * di);
-          if (mag < dmag || m == a.mend) {
+          double wr = zr + pr, wi = zi + pi;
+          if (r2 < dmag || m == a.mend) {
             dr = pr;
             di = pi;
-            zr = 0.0;
-            zi = 0.0;
+            wr = pr;
+            wi = pi;
             m = 0;
           }
-          double wr = zr + pr, wi = zi + pi;
           double ndr = fma(dr, wr, fma(-di, wi, e.x));
           double ndi = fma(dr, wi, fma(di, wr, e.y));
-          dl = make_double2(ndr, ndi);
-          it = found + 1;
-          m += 1;
+          c.state[gid] = make_double4(ndr, ndi, e.x, e.y);
+          c.state_i[gid] = make_int2(found + 1, m + 1);
           want = true;
-          resolved = false;
           skipped += (unsigned long long)(found + 1);
         } else {
-          nesc++;  // the skip reached N: (N, 0) as the empty HP loop returns
+          nres++;  // the skip reached N: (N, 0) as the empty HP loop returns
         }
       }
-      if (resolved) a.out[f.out_base + local] = res;
+      if (!want) c.out[out_idx] = res;
+      c.flag[gid] = want ? 1 : 0;
+      mine += want ? 1 : 0;
     }
-    uint32_t slot = wave_append(&a.cnt[C_ACTIVE], want);
-    if (want) {
-      a.active[slot] = gid;
-      a.state[slot] = make_double4(dl.x, dl.y, e.x, e.y);
-      a.state_i[slot] = make_int2(it, m);
-    }
+    tile_count_store(c.tile_count, tile, mine);
   }
-  wave_add(&a.cnt[C_CARDIOID], ncard);
-  wave_add(&a.cnt[C_SA_ESC], nesc);
-  wave_add(&a.cnt[C_SKIPPED], skipped);
+  const unsigned long long v[3] = {ncard, nres, skipped};
+  const int idx[3] = {C_CARDIOID, C_SA_RES, C_SKIPPED};
+  block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 3);
+}
+
+// One block of K perturbation steps for a lane (Y_it = Zd[m] + d):
+//   d' = d (Zd[m] + Y) + e, rebase to the orbit start (m = 0, d = Y) when
+//   |Y| < |d| (or, in END blocks, when m reaches the end of the reference).
+// zb holds Zd[m0 .. m0+K-1]; a lane that rebased reads the orbit head from LDS.
+// The first escape (exact sqMag > bailout^2, mandelbrot.cpp:60) records its step
+// and |Y|^2 and poisons the lane with NaN so later steps cannot escape again.
+template <int K, bool END>
+__device__ __forceinline__ void pert_block(double& dr, double& di, const double er, const double ei, int& m,
+                                           const int it, const double2* zb, const double2* zhead, const int endj,
+                                           const int mend, const double b2, int& esc, double& esc_r2,
+                                           unsigned int& rb) {
+  const double qnan = __longlong_as_double(0x7ff8000000000000ll);
+  bool moved = false;
+  unsigned int hoff = 0;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const double2 zh = *(const double2*)((const char*)zhead + hoff);
+    const double zr = moved ? zh.x : zb[j].x;
+    const double zi = moved ? zh.y : zb[j].y;
+    const double yr = zr + dr, yi = zi + di;
+    const double r2 = yr * yr + yi * yi;  // sqMag (mandelbrot.cpp:60)
+    if (r2 > b2) {  // first escape: NaN in dr reaches every later step
+      esc = it + j;
+      esc_r2 = r2;
+      dr = qnan;
+    }
+    const double dmag = fma(dr, dr, di * di);
+    double wr = zr + yr, wi = zi + yi;
+    bool reb = r2 < dmag;
+    // the forced rebase at the orbit's end, m == mend: m = m0 + j until a
+    // rebase, then hoff / 16
+    if (END) reb = reb || (moved ? (int)hoff == 16 * mend : j == endj);
+    if (reb) {  // rebase: d = Y against Zd[0] = 0, so W = Y
+      dr = yr;
+      di = yi;
+      wr = yr;
+      wi = yi;
+      hoff = 0;
+      moved = true;
+    }
+    const double ndr = fma(dr, wr, fma(-di, wi, er));
+    const double ndi = fma(dr, wi, fma(di, wr, ei));
+    dr = ndr;
+    di = ndi;
+    hoff += 16;
+  }
+  m = moved ? (int)(hoff >> 4) : m + K;
+  rb += moved ? 1u : 0u;
 }
 
 // FP64 perturbation with rebasing (replaces the HP continuation of
-// mandelbrot.cpp:209-224): Y = Zd[m] + d, d' = d (Zd[m] + Y) + e, rebase to
-// m = 0 when |Y| < |d| or at the end of the reference orbit.
+// mandelbrot.cpp:209-224).  Persistent waves, lanes refilled from 64-sample
+// chunks; blocks of K steps with the block's orbit entries loaded up front.
 template <int K>
-__global__ __launch_bounds__(256) void k_series_iterate(SeriesArgs a) {
-  const uint32_t total = (uint32_t)a.cnt[C_ACTIVE];
-  unsigned int* work = (unsigned int*)&a.cnt[C_WORK];
-  uint32_t cpos = 0, cend = 0;
-  bool exhausted = false;
+__global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
+  const Common& c = a.c;
+  const uint32_t total = *c.nactive;
+  const uint32_t W = gridDim.x * (blockDim.x >> 6);
+  unsigned long long executed = 0, rebased = 0;
+  const int N = c.N, mend = a.mend;
+  // orbit head for lanes that rebase inside a block (Zd is padded past it)
+  __shared__ double2 zhead[kHead + 1];
+  for (int i = threadIdx.x; i <= kHead; i += blockDim.x) zhead[i] = a.Zd[i];
+  __syncthreads();
+  uint32_t ccur = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  uint32_t ncur = chunk_items(ccur, total), nnext = chunk_items(ccur + W, total);
+  uint32_t cur_pix = load_pix(c.active, ccur, total), nxt_pix = load_pix(c.active, ccur + W, total);
+  uint32_t far_pix = load_pix(c.active, ccur + 2 * W, total);
+  double4 cur_s = make_double4(0.0, 0.0, 0.0, 0.0), nxt_s = cur_s;
+  int2 cur_i = make_int2(0, 0), nxt_i = cur_i;
+  if ((uint32_t)lane_id() < ncur) {
+    cur_s = c.state[cur_pix];
+    cur_i = c.state_i[cur_pix];
+  }
+  if ((uint32_t)lane_id() < nnext) {
+    nxt_s = c.state[nxt_pix];
+    nxt_i = c.state_i[nxt_pix];
+  }
+  uint32_t pos = 0;
   bool act = false;
   uint32_t pix = 0;
   double dr = 0.0, di = 0.0, er = 0.0, ei = 0.0;
   int it = 0, m = 0;
-  unsigned long long executed = 0, rebased = 0;
-  const int N = a.N, mend = a.mend;
   for (;;) {
     unsigned long long need = __ballot(!act);
-    while (need && !exhausted) {
-      if (cpos >= cend) {
-        uint32_t b = 0;
-        if (lane_id() == 0) b = atomicAdd(work, (unsigned int)kChunk);
-        b = __shfl(b, 0);
-        cpos = b;
-        cend = min(b + (uint32_t)kChunk, total);
-        if (cpos >= total) {
-          exhausted = true;
-          break;
-        }
-      }
-      uint32_t avail = cend - cpos;
-      uint32_t nneed = (uint32_t)__popcll(need);
-      uint32_t take = min(nneed, avail);
+    while (need && ncur > 0) {
+      uint32_t take = min((uint32_t)__popcll(need), ncur - pos);
       uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane_id()) - 1ull));
+      int src = (int)min(pos + rank, ncur - 1);
+      uint32_t g_pix = __shfl(cur_pix, src);
+      double g0 = __shfl(cur_s.x, src), g1 = __shfl(cur_s.y, src), g2 = __shfl(cur_s.z, src),
+             g3 = __shfl(cur_s.w, src);
+      int g4 = __shfl(cur_i.x, src), g5 = __shfl(cur_i.y, src);
       if (!act && rank < take) {
-        uint32_t slot = cpos + rank;
-        pix = a.active[slot];
-        double4 s = a.state[slot];
-        int2 si = a.state_i[slot];
-        dr = s.x;
-        di = s.y;
-        er = s.z;
-        ei = s.w;
-        it = si.x;
-        m = si.y;
+        pix = g_pix;
+        dr = g0;
+        di = g1;
+        er = g2;
+        ei = g3;
+        it = g4;
+        m = g5;
         act = true;
       }
-      cpos += take;
+      pos += take;
+      if (pos == ncur) {  // rotate: NEXT becomes CUR, the far chunk is decoded
+        ccur += W;
+        ncur = nnext;
+        cur_pix = nxt_pix;
+        cur_s = nxt_s;
+        cur_i = nxt_i;
+        nnext = chunk_items(ccur + W, total);
+        nxt_pix = far_pix;
+        if ((uint32_t)lane_id() < nnext) {
+          nxt_s = c.state[nxt_pix];
+          nxt_i = c.state_i[nxt_pix];
+        }
+        far_pix = load_pix(c.active, ccur + 2 * W, total);
+        pos = 0;
+      }
       need = __ballot(!act);
     }
     if (__ballot(act) == 0) break;
     if (!act) {  // idle lanes run the block on a harmless state
       dr = di = er = ei = 0.0;
       m = 0;
+      it = 0;
     }
-    const bool fast = __ballot(act && (it + K > N || m + K > mend)) == 0;
-    const double dr0 = dr, di0 = di;
-    const int m0 = m, it0 = it;
-    bool replay = act;
-    if (fast) {
-      double mag = 0.0;
-      unsigned long long rb = 0;
+    // lanes whose block crosses the reference's end rebase there (END variant)
+    const bool end_zone = __ballot(act && m + K > mend) != 0 || mend <= K;
+    double2 zb[K];
 #pragma unroll
-      for (int j = 0; j < K; j++) {
-        double2 z = a.Zd[m];
-        double zr = z.x, zi = z.y;
-        double yr = zr + dr, yi = zi + di;
-        mag = fma(yr, yr, yi * yi);
-        double dmag = fma(dr, dr, di * di);
-        if (mag < dmag) {
-          dr = yr;
-          di = yi;
-          zr = 0.0;
-          zi = 0.0;
-          m = 0;
-          rb++;
-        }
-        double wr = zr + yr, wi = zi + yi;
-        double ndr = fma(dr, wr, fma(-di, wi, er));
-        double ndi = fma(dr, wi, fma(di, wr, ei));
-        dr = ndr;
-        di = ndi;
-        m++;
-      }
-      it += K;
-      replay = act && !(mag <= a.b2m);
-      if (act && !replay) {
-        executed += K;
-        rebased += rb;
-        if (it >= N) {  // tops up to N-1 did not escape
-          const DevFrame& f = a.frames[find_frame(a.frames, a.nframes, pix)];
-          a.out[f.out_base + (pix - f.sample_base)] = EscapeValue{N, 0.0f};
-          act = false;
-        }
-      }
-    }
-    if (replay) {
-      dr = dr0;
-      di = di0;
-      m = m0;
-      it = it0;
-      bool done = false;
-      int count = N;
-      float sm = 0.0f;
-      for (int j = 0; j < K; j++) {
-        double2 z = a.Zd[m];
-        double zr = z.x, zi = z.y;
-        double yr = zr + dr, yi = zi + di;
-        double r2 = yr * yr + yi * yi;  // sqMag, as bailedOut does
-        executed++;
-        if (r2 > a.b2) {
-          done = true;
-          count = it;
-          sm = smooth_of(r2, a.log_b);
-          break;
-        }
-        if (it == N - 1) {
-          done = true;
-          break;
-        }
-        double mag = fma(yr, yr, yi * yi);
-        double dmag = fma(dr, dr, di * di);
-        if (mag < dmag || m == mend) {
-          dr = yr;
-          di = yi;
-          zr = 0.0;
-          zi = 0.0;
-          m = 0;
-          rebased++;
-        }
-        double wr = zr + yr, wi = zi + yi;
-        double ndr = fma(dr, wr, fma(-di, wi, er));
-        double ndi = fma(dr, wi, fma(di, wr, ei));
-        dr = ndr;
-        di = ndi;
-        m++;
-        it++;
-      }
-      if (done) {
-        const DevFrame& f = a.frames[find_frame(a.frames, a.nframes, pix)];
-        a.out[f.out_base + (pix - f.sample_base)] = EscapeValue{count, sm};
+    for (int j = 0; j < K; j++) zb[j] = a.Zd[m + j];
+    int esc = -1;
+    double esc_r2 = 0.0;
+    unsigned int rb = 0;
+    const int it0 = it;
+    if (end_zone)
+      pert_block<K, true>(dr, di, er, ei, m, it, zb, zhead, mend - m, mend, c.b2, esc, esc_r2, rb);
+    else
+      pert_block<K, false>(dr, di, er, ei, m, it, zb, zhead, 0, mend, c.b2, esc, esc_r2, rb);
+    if (act) {
+      it = it0 + K;
+      const bool escaped = esc >= 0 && esc <= N - 1;
+      executed += escaped ? (unsigned)(esc - it0 + 1) : (unsigned)min(K, N - it0);
+      rebased += rb;
+      if (escaped || it >= N) {  // tops up to N-1 decided
+        c.out[pix] = EscapeValue{escaped ? esc : N, 0.0f};
+        if (escaped) c.r2[pix] = esc_r2;  // smoothing: k_finalize
         act = false;
       }
     }
   }
-  wave_add(&a.cnt[C_EXECUTED], executed);
-  wave_add(&a.cnt[C_REBASED], rebased);
+  const unsigned long long v[2] = {executed, rebased};
+  const int idx[2] = {C_EXECUTED, C_REBASED};
+  block_partial(c.part, kSlotIter + blockIdx.x, v, idx, 2);
+}
+
+// smoothing of the iterated samples (mandelbrot.cpp:133-136), converged: one
+// thread per entry of active[]
+__global__ __launch_bounds__(kThreads) void k_finalize(Common c) {
+  const uint32_t total = *c.nactive;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    uint32_t pix = c.active[i];
+    if (c.out[pix].iterations < c.N) c.out[pix].smoothing = smooth_of(c.r2[pix], c.log_b);
+  }
 }
 
 // ============================================================ grid utilities
@@ -668,32 +885,27 @@ __global__ void k_downsample(const EscapeValue* fine, int nr, int nc, int sc, Es
   }
 }
 
-// sum of reported iterations over an output range (credited work)
-__global__ void k_sum_iterations(const EscapeValue* out, unsigned long long n, unsigned long long* dst) {
+// sum of reported iterations over the output (credited work)
+__global__ __launch_bounds__(kThreads) void k_sum_iterations(const EscapeValue* out, unsigned long long n,
+                                                             unsigned long long* part) {
   unsigned long long s = 0;
   for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
        i += (unsigned long long)gridDim.x * blockDim.x)
     s += (unsigned long long)out[i].iterations;
-  wave_add(dst, s);
+  const int idx[1] = {C_CREDITED};
+  block_partial(part, kSlotSum + blockIdx.x, &s, idx, 1);
 }
 
 __global__ void k_zero(unsigned long long* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0ull;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
 // ================================================================== engine
-#define NM_HIP(expr)                                   \
-  do {                                                 \
-    hipError_t e_ = (expr);                            \
-    if (e_ != hipSuccess) return fail(#expr, e_);      \
+#define NM_HIP(expr)                              \
+  do {                                            \
+    hipError_t e_ = (expr);                       \
+    if (e_ != hipSuccess) return fail(#expr, e_); \
   } while (0)
-
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-};
 
 class EngineImpl : public Engine {
  public:
@@ -707,12 +919,17 @@ class EngineImpl : public Engine {
   void* d_tabs = nullptr; size_t n_tabs_bytes = 0;
   void* d_mask = nullptr; size_t n_mask_bytes = 0;
   void* d_out = nullptr; size_t n_out_bytes = 0;
+  void* d_flag = nullptr; size_t n_flag_bytes = 0;
+  void* d_r2 = nullptr; size_t n_r2_bytes = 0;
+  void* d_tiles = nullptr; size_t n_tiles_bytes = 0;
   void* d_active = nullptr; size_t n_active_bytes = 0;
   void* d_state = nullptr; size_t n_state_bytes = 0;
   void* d_state_i = nullptr; size_t n_state_i_bytes = 0;
   void* d_ref = nullptr; size_t n_ref_bytes = 0;
   void* d_grid = nullptr; size_t n_grid_bytes = 0;  // downsample staging
-  unsigned long long* d_cnt = nullptr;
+  unsigned long long* d_part = nullptr;               // kPartSlots x C_COUNT
+  uint32_t* d_nactive = nullptr;
+  std::vector<unsigned long long> h_part;
   uint64_t ref_serial = 0;
   // reference layout (offsets into d_ref)
   int ref_len = 0;
@@ -738,7 +955,9 @@ class EngineImpl : public Engine {
     num_cu = prop.multiProcessorCount;
     NM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) NM_HIP(hipEventCreate(&e));
-    NM_HIP(hipMalloc((void**)&d_cnt, sizeof(unsigned long long) * C_COUNT));
+    NM_HIP(hipMalloc((void**)&d_part, sizeof(unsigned long long) * kPartSlots * C_COUNT));
+    NM_HIP(hipMalloc((void**)&d_nactive, 256));
+    h_part.resize((size_t)kPartSlots * C_COUNT);
     return NM_OK;
   }
 
@@ -757,8 +976,9 @@ class EngineImpl : public Engine {
     if (serial == ref_serial && serial != 0) return NM_OK;
     const int len = ref.len;
     if (len < 1) return NM_ERR_ARG;
-    // Zd padded with zeros so idle lanes can run a block from index 0
-    const size_t nz = (size_t)len + 2 + 64;
+    // Zd padded with zeros so idle lanes can run a block from index 0 and the
+    // LDS orbit head (kHead + 1 entries) always exists
+    const size_t nz = (size_t)len + 2 + kHead + 64;
     size_t off = 0;
     auto place = [&](size_t bytes) {
       size_t o = off;
@@ -802,63 +1022,95 @@ class EngineImpl : public Engine {
     return NM_OK;
   }
 
-  int zero_counters() {
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, d_cnt, (int)C_COUNT);
-    NM_HIP(hipGetLastError());
+  int grid_for_tiles(uint64_t total) const {
+    uint64_t ntiles = (total + kTile - 1) / kTile;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, (uint64_t)num_cu * 8));
+  }
+
+  // buffers and the common kernel arguments for `total` samples
+  int prepare_common(Common& c, uint64_t total, const RenderOpts& opt) {
+    if (total == 0 || total >= 0xFFFFF000ull) return NM_ERR_RANGE;
+    uint64_t ntiles = (total + kTile - 1) / kTile;
+    int st;
+    if ((st = grow(&d_out, &n_out_bytes, total * sizeof(EscapeValue)))) return st;
+    if ((st = grow(&d_flag, &n_flag_bytes, total))) return st;
+    if ((st = grow(&d_r2, &n_r2_bytes, total * 8))) return st;
+    if ((st = grow(&d_state, &n_state_bytes, total * sizeof(double4)))) return st;
+    if ((st = grow(&d_state_i, &n_state_i_bytes, total * sizeof(int2)))) return st;
+    if ((st = grow(&d_tiles, &n_tiles_bytes, ntiles * 4))) return st;
+    if ((st = grow(&d_active, &n_active_bytes, total * 4))) return st;
+    c.frames = (const DevFrame*)d_frames;
+    c.total = (uint32_t)total;
+    c.out = (EscapeValue*)d_out;
+    c.r2 = (double*)d_r2;
+    c.state = (double4*)d_state;
+    c.state_i = (int2*)d_state_i;
+    c.flag = (uint8_t*)d_flag;
+    c.tile_count = (uint32_t*)d_tiles;
+    c.active = (uint32_t*)d_active;
+    c.nactive = d_nactive;
+    c.part = d_part;
+    c.N = opt.N;
+    c.b2 = opt.bailout * opt.bailout;
+    c.b2m = c.b2 * (1.0 - 1.0 / 1099511627776.0);
+    c.log_b = std::log(opt.bailout);
     return NM_OK;
   }
 
-  int read_counters(unsigned long long* h) {
-    NM_HIP(hipMemcpyAsync(h, d_cnt, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToHost, stream));
-    NM_HIP(hipStreamSynchronize(stream));
-    return NM_OK;
-  }
-
-  // frames + tables + output/active/state sizing shared by both paths
-  struct Layout {
-    std::vector<DevFrame> frames;
-    uint64_t total = 0;
-  };
-
-  int start_grid() const { return num_cu * 16; }
-
+  // start -> scan -> scatter -> iterate, `repeats` times; then counters and output
   template <class Args, class StartFn, class IterFn>
   int run(Args& args, StartFn start, IterFn iter, int waves, const RenderOpts& opt, RenderStats* stats,
           uint64_t nout, EscapeValue* host_out) {
+    const Common& c = args.c;
     const int repeats = std::max(1, opt.repeats);
+    const uint32_t ntiles = (uint32_t)((c.total + kTile - 1) / kTile);
+    const int gtile = grid_for_tiles(c.total);
+    const int giter = std::min(num_cu * waves, kSlotSum - kSlotIter);
     float ms_start = 0, ms_iter = 0, ms_total = 0;
-    unsigned long long h[C_COUNT] = {};
     for (int rep = 0; rep < repeats; rep++) {
-      int st = zero_counters();
-      if (st) return st;
+      hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartSlots * C_COUNT);
+      NM_HIP(hipGetLastError());
       NM_HIP(hipEventRecord(ev[0], stream));
-      hipLaunchKernelGGL(start, dim3(start_grid()), dim3(256), 0, stream, args);
+      hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args);
+      NM_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, c.tile_count, ntiles, d_nactive);
+      NM_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint8_t*)c.flag,
+                         (const uint32_t*)c.tile_count, c.total, c.active);
       NM_HIP(hipGetLastError());
       NM_HIP(hipEventRecord(ev[1], stream));
-      hipLaunchKernelGGL(iter, dim3(num_cu * waves), dim3(256), 0, stream, args);
+      hipLaunchKernelGGL(iter, dim3(giter), dim3(kThreads), 0, stream, args);
       NM_HIP(hipGetLastError());
       NM_HIP(hipEventRecord(ev[2], stream));
-      NM_HIP(hipEventSynchronize(ev[2]));
-      float t1 = 0, t2 = 0;
+      hipLaunchKernelGGL(k_finalize, dim3(gtile), dim3(kThreads), 0, stream, c);
+      NM_HIP(hipGetLastError());
+      NM_HIP(hipEventRecord(ev[3], stream));
+      NM_HIP(hipEventSynchronize(ev[3]));
+      float t1 = 0, t2 = 0, t3 = 0;
       NM_HIP(hipEventElapsedTime(&t1, ev[0], ev[1]));
       NM_HIP(hipEventElapsedTime(&t2, ev[1], ev[2]));
-      ms_start = t1;
+      NM_HIP(hipEventElapsedTime(&t3, ev[2], ev[3]));
+      ms_start = t1 + t3;
       ms_iter += t2;
-      ms_total += t1 + t2;
+      ms_total += t1 + t2 + t3;
     }
-    hipLaunchKernelGGL(k_sum_iterations, dim3(num_cu * 4), dim3(256), 0, stream, (const EscapeValue*)d_out,
-                       (unsigned long long)nout, d_cnt + C_CREDITED);
+    hipLaunchKernelGGL(k_sum_iterations, dim3(std::min(num_cu * 4, kPartSlots - kSlotSum)), dim3(kThreads), 0,
+                       stream, (const EscapeValue*)d_out, (unsigned long long)nout, d_part);
     NM_HIP(hipGetLastError());
-    int st = read_counters(h);
-    if (st) return st;
+    uint32_t nact = 0;
+    NM_HIP(hipMemcpyAsync(h_part.data(), d_part, h_part.size() * 8, hipMemcpyDeviceToHost, stream));
+    NM_HIP(hipMemcpyAsync(&nact, d_nactive, 4, hipMemcpyDeviceToHost, stream));
     if (host_out)
       NM_HIP(hipMemcpyAsync(host_out, d_out, nout * sizeof(EscapeValue), hipMemcpyDeviceToHost, stream));
     NM_HIP(hipStreamSynchronize(stream));
+    unsigned long long h[C_COUNT] = {};
+    for (int s = 0; s < kPartSlots; s++)
+      for (int k = 0; k < C_COUNT; k++) h[k] += h_part[(size_t)s * C_COUNT + k];
     if (stats) {
       stats->samples = nout;
       stats->n_cardioid = h[C_CARDIOID];
-      stats->n_sa_resolved = h[C_SA_ESC];
-      stats->n_iterated = h[C_ACTIVE];
+      stats->n_sa_resolved = h[C_SA_RES];
+      stats->n_iterated = nact;
       stats->executed = h[C_EXECUTED];
       stats->skipped = h[C_SKIPPED];
       stats->n_rebased = h[C_REBASED];
@@ -874,12 +1126,11 @@ class EngineImpl : public Engine {
                 RenderStats* stats) override {
     if (frames.empty()) return NM_ERR_ARG;
     std::vector<DevFrame> df;
-    std::vector<double> tabs;
     uint64_t total = 0;
-    // table layout: [cre | crelo | cim | cimlo] concatenated per kind
     size_t ncol = 0, nrow = 0;
     for (auto* f : frames) {
       if (!f || f->nr <= 0 || f->nc <= 0) return NM_ERR_ARG;
+      if (f->nr != frames[0]->nr || f->nc != frames[0]->nc) return NM_ERR_ARG;  // one frame size per set
       ncol += f->nc;
       nrow += f->nr;
     }
@@ -902,41 +1153,34 @@ class EngineImpl : public Engine {
       total += (uint64_t)f->nr * f->nc;
       df.push_back(d);
     }
-    if (total >= 0xFFFFFFF0ull) return NM_ERR_RANGE;
     int st;
     if ((st = grow(&d_frames, &n_frames_bytes, df.size() * sizeof(DevFrame)))) return st;
-    size_t tb = (2 * ncol + 2 * nrow) * sizeof(double);
-    if ((st = grow(&d_tabs, &n_tabs_bytes, tb))) return st;
-    if ((st = grow(&d_out, &n_out_bytes, total * sizeof(EscapeValue)))) return st;
-    if ((st = grow(&d_active, &n_active_bytes, total * sizeof(uint32_t)))) return st;
+    if ((st = grow(&d_tabs, &n_tabs_bytes, (2 * ncol + 2 * nrow) * sizeof(double)))) return st;
+    PlainArgs a;
+    if ((st = prepare_common(a.c, total, opt))) return st;
+    a.c.nframes = (int)df.size();
+    a.c.frame_size = (uint32_t)(total / df.size());
+    a.c.inv_frame = 1.0 / (double)a.c.frame_size;
     double* t = (double*)d_tabs;
     NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(t, cre.data(), ncol * 8, hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(t + ncol, crelo.data(), ncol * 8, hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(t + 2 * ncol, cim.data(), nrow * 8, hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(t + 2 * ncol + nrow, cimlo.data(), nrow * 8, hipMemcpyHostToDevice, stream));
-    PlainArgs a;
-    a.frames = (const DevFrame*)d_frames;
-    a.nframes = (int)df.size();
-    a.total = (uint32_t)total;
+    a.nrows = (uint32_t)frames[0]->nr;
+    a.ncols = (uint32_t)frames[0]->nc;
+    a.inv_ncols = 1.0 / (double)a.ncols;
     a.cre = t;
     a.crelo = t + ncol;
     a.cim = t + 2 * ncol;
     a.cimlo = t + 2 * ncol + nrow;
-    a.out = (EscapeValue*)d_out;
-    a.active = (uint32_t*)d_active;
-    a.cnt = d_cnt;
-    a.N = opt.N;
-    a.b2 = opt.bailout * opt.bailout;
-    a.b2m = a.b2 * (1.0 - 1.0 / 1099511627776.0);
-    a.log_b = std::log(opt.bailout);
     const int K = opt.block_k > 0 ? opt.block_k : 32;
     const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 4;
     if (stats) {
       stats->block_k = K;
       stats->waves_per_simd = waves;
     }
-    // 256-thread blocks = 4 waves = one per SIMD: num_cu * waves blocks
+    // 256-thread workgroups = 4 waves = one per SIMD: num_cu * waves workgroups
     switch (K) {
       case 8: return run(a, k_plain_start, k_plain_iterate<8>, waves, opt, stats, total, host_out);
       case 16: return run(a, k_plain_start, k_plain_iterate<16>, waves, opt, stats, total, host_out);
@@ -953,6 +1197,7 @@ class EngineImpl : public Engine {
     size_t ncol = 0, nrow = 0, nmask = 0;
     for (auto* f : frames) {
       if (!f || f->nr <= 0 || f->nc <= 0) return NM_ERR_ARG;
+      if (f->nr != frames[0]->nr || f->nc != frames[0]->nc) return NM_ERR_ARG;  // one frame size per set
       ncol += f->nc;
       nrow += f->nr;
       if (f->card_mode == CARD_MASK) nmask += f->mask.size();
@@ -980,33 +1225,24 @@ class EngineImpl : public Engine {
       total += (uint64_t)f->nr * f->nc;
       df.push_back(d);
     }
-    if (total >= 0xFFFFFFF0ull) return NM_ERR_RANGE;
     int st;
     if ((st = grow(&d_frames, &n_frames_bytes, df.size() * sizeof(DevFrame)))) return st;
     if ((st = grow(&d_tabs, &n_tabs_bytes, (ncol + nrow) * sizeof(double)))) return st;
     if ((st = grow(&d_mask, &n_mask_bytes, std::max<size_t>(mask.size(), 1) * 4))) return st;
-    if ((st = grow(&d_out, &n_out_bytes, total * sizeof(EscapeValue)))) return st;
-    if ((st = grow(&d_active, &n_active_bytes, total * sizeof(uint32_t)))) return st;
-    if ((st = grow(&d_state, &n_state_bytes, total * sizeof(double4)))) return st;
-    if ((st = grow(&d_state_i, &n_state_i_bytes, total * sizeof(int2)))) return st;
+    SeriesArgs a;
+    if ((st = prepare_common(a.c, total, opt))) return st;
+    a.c.nframes = (int)df.size();
+    a.c.frame_size = (uint32_t)(total / df.size());
+    a.c.inv_frame = 1.0 / (double)a.c.frame_size;
     double* t = (double*)d_tabs;
     NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(t, er.data(), ncol * 8, hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(t + ncol, ei.data(), nrow * 8, hipMemcpyHostToDevice, stream));
     if (!mask.empty())
       NM_HIP(hipMemcpyAsync(d_mask, mask.data(), mask.size() * 4, hipMemcpyHostToDevice, stream));
-    SeriesArgs a;
-    a.frames = (const DevFrame*)d_frames;
-    a.nframes = (int)df.size();
-    a.total = (uint32_t)total;
     a.epsre = t;
     a.epsim = t + ncol;
     a.mask = (const uint32_t*)d_mask;
-    a.out = (EscapeValue*)d_out;
-    a.active = (uint32_t*)d_active;
-    a.state = (double4*)d_state;
-    a.state_i = (int2*)d_state_i;
-    a.cnt = d_cnt;
     a.Zd = rZd;
     a.Xlo = rXlo;
     a.A = rA;
@@ -1020,13 +1256,9 @@ class EngineImpl : public Engine {
     a.SP = rSP;
     a.len = ref_len;
     a.mend = ref_len + 1;
-    a.N = opt.N;
     a.tol = opt.tol;
-    a.b2 = opt.bailout * opt.bailout;
-    a.b2m = a.b2 * (1.0 - 1.0 / 1099511627776.0);
-    a.log_b = std::log(opt.bailout);
-    const int K = opt.block_k > 0 ? opt.block_k : 16;
-    const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 4;
+    const int K = opt.block_k > 0 ? opt.block_k : 8;
+    const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
     if (stats) {
       stats->block_k = K;
       stats->waves_per_simd = waves;
@@ -1067,8 +1299,6 @@ int Engine::instance(Engine** out) {
     int st = e->init();
     if (st != NM_OK) {
       g_engine_status = st;
-      static std::string msg;
-      msg = e->err;
       delete e;
       *out = nullptr;
       return st;

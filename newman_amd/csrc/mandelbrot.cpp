@@ -624,10 +624,8 @@ int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts*
   if (all_hw) {
     std::vector<HwFrameHost> fs(widths.size());
     std::vector<const HwFrameHost*> ptr;
-    for (size_t k = 0; k < fs.size(); k++) {
-      buildHwFrame(fs[k], center, sizes[k]);
-      ptr.push_back(&fs[k]);
-    }
+    parallel_for((long)fs.size(), [&](long k) { buildHwFrame(fs[k], center, sizes[k]); }, 1);
+    for (size_t k = 0; k < fs.size(); k++) ptr.push_back(&fs[k]);
     return eng->render_hw(ptr, o, out, &stats_);
   }
   if (!have_ref_) return NM_ERR_STATE;
@@ -635,9 +633,10 @@ int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts*
   if (st != NM_OK) return st;
   std::vector<SaFrameHost> fs(widths.size());
   std::vector<const SaFrameHost*> ptr;
+  std::vector<int> sts(fs.size(), NM_OK);
+  parallel_for((long)fs.size(), [&](long k) { sts[k] = buildSaFrame(fs[k], center, sizes[k]); }, 1);
   for (size_t k = 0; k < fs.size(); k++) {
-    st = buildSaFrame(fs[k], center, sizes[k]);
-    if (st != NM_OK) return st;
+    if (sts[k] != NM_OK) return sts[k];
     ptr.push_back(&fs[k]);
   }
   return eng->render_sa(ptr, o, out, &stats_);

@@ -796,24 +796,24 @@ static orc_escape pert_from(const pert_ctx* P, double dr, double di, double er, 
     double zr = P->Zd[2 * m], zi = P->Zd[2 * m + 1];
     double yr = zr + dr, yi = zi + di;
     if (it > it0) {
-      double r2 = yr * yr + yi * yi; /* sqMag, no FMA */
-      if (r2 > P->b2) {
+      double r2e = yr * yr + yi * yi; /* sqMag, no FMA */
+      if (r2e > P->b2) {
         esc.iterations = it;
-        esc.smoothing = (float)smoothing_of(r2, P->bailout);
+        esc.smoothing = (float)smoothing_of(r2e, P->bailout);
         return esc;
       }
       if (it == N - 1) break;
     }
-    double mag = fma(yr, yr, yi * yi);
+    double r2 = yr * yr + yi * yi; /* sqMag, no FMA: also the rebase test */
     double dmag = fma(dr, dr, di * di);
-    if (mag < dmag || m == P->mend) {
+    double wr = zr + yr, wi = zi + yi;
+    if (r2 < dmag || m == P->mend) { /* rebase: d = Y against Zd[0] = 0, so W = Y */
       dr = yr;
       di = yi;
-      zr = 0.0;
-      zi = 0.0;
+      wr = yr;
+      wi = yi;
       m = 0;
     }
-    double wr = zr + yr, wi = zi + yi;
     double ndr = fma(dr, wr, fma(-di, wi, er));
     double ndi = fma(dr, wi, fma(di, wr, ei));
     dr = ndr;

@@ -156,3 +156,67 @@ def test_downsample_matches_at_avg(gpu):
     got = m.grid()
     assert np.array_equal(got["iterations"], want["iterations"])
     assert np.array_equal(got["smoothing"], want["smoothing"])
+
+
+@pytest.mark.parametrize("N,width", [(12, "1e-24"), (40, "1e-24"), (300, "3e-23")])
+def test_series_path_short_orbits(gpu, N, width):
+    """Reference orbits shorter than a block (len(X) <= N < K): every block is an
+    end-zone block and rebased lanes can reach the orbit's end again."""
+    v = _VIEWS["t13"]
+    m, o = _pair(40, 56, N, v["re"], v["im"], width)
+    m.precompute()
+    o.precompute()
+    m.render()
+    rc = np.array([(r, c) for r in range(40) for c in range(56)], dtype=np.int32)
+    ref = o.render_samples(rc, pert=True).reshape(40, 56)
+    g = m.grid()
+    assert np.array_equal(g["iterations"], ref["iterations"])
+    assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
+
+
+@pytest.mark.parametrize("rre,rim", [("0.5", "0.5"), ("-1.8", "0.02"), ("-0.8", "0.2")])
+def test_series_path_reference_escapes_early(gpu, rre, rim):
+    """An explicit reference point whose orbit escapes after a few steps: samples
+    outlive it and rebase at its end (the forced rebase of the iteration)."""
+    v = _VIEWS["t13"]
+    m, o = _pair(48, 64, 3000, v["re"], v["im"], "1e-23")
+    m.precompute_at(rre, rim)
+    o.precompute_at(rre, rim)
+    assert m.reference_length() == o.ref_len() < 50
+    st = m.render()
+    rc = np.array([(r, c) for r in range(48) for c in range(64)], dtype=np.int32)
+    ref = o.render_samples(rc, pert=True).reshape(48, 64)
+    g = m.grid()
+    assert np.array_equal(g["iterations"], ref["iterations"])
+    assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
+    assert st["n_rebased"] > 0
+
+
+def test_frame_batch_equals_single_renders(gpu):
+    """A zoom-video frame set (shared reference, widths 1.1x apart) in one launch
+    equals the frames rendered one by one."""
+    import newman_amd as nm
+    v = _VIEWS["t13"]
+    widths = ["%.6e" % (1e-24 * 1.1 ** k) for k in range(6)]
+    m = nm.Mandelbrot(36, 48)
+    m.set_view(v["N"], v["re"], v["im"], widths[0])
+    m.precompute()
+    batch, st = m.render_frames(widths)
+    for k, w in enumerate(widths):
+        s = nm.Mandelbrot(36, 48)
+        s.set_view(v["N"], v["re"], v["im"], widths[0])
+        s.precompute()
+        one, _ = s.render_frames([w])
+        assert np.array_equal(batch[k]["iterations"], one[0]["iterations"])
+        assert np.array_equal(batch[k]["smoothing"], one[0]["smoothing"])
+
+
+def test_plain_batch_equals_single_frame(gpu):
+    import newman_amd as nm
+    m = nm.Mandelbrot(48, 64)
+    m.set_view(500, "-0.75", "0", "3.0")
+    batch, _ = m.render_frames(["3.0"] * 5)
+    m.render()
+    g = m.grid()
+    for k in range(5):
+        assert np.array_equal(batch[k], g)
