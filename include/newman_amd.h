@@ -46,15 +46,18 @@ typedef struct nm_render_stats {
   uint64_t n_cardioid;   /* resolved by the cardioid/bulb test (mandelbrot.cpp:63-71) */
   uint64_t n_sa_resolved; /* resolved inside the series skip: escaped (mandelbrot.cpp:186-206) or skip reached N */
   uint64_t n_iterated;   /* handed to the iteration kernel */
-  uint64_t executed;     /* iteration steps executed by the iteration kernel */
+  uint64_t executed;     /* iteration steps executed (start and iteration kernels) */
   uint64_t skipped;      /* sum of series-skip start indices of iterated samples */
-  uint64_t n_rebased;    /* perturbation rebases */
+  uint64_t n_rebased;    /* blocks of block_k steps in which a sample rebased its delta */
   uint64_t credited;     /* sum of reported iterations over non-cardioid samples */
-  double ms_start;       /* start kernel, ms (HIP events) */
+  double ms_start;       /* start kernel, ms (HIP events), mean over repeats */
   double ms_iterate;     /* iteration kernel, ms, mean over repeats */
-  double ms_total;       /* both, ms, mean over repeats */
+  double ms_total;       /* all kernels of one pass, ms, mean over repeats */
   int32_t block_k;       /* iterations per block used */
   int32_t waves_per_simd;
+  uint64_t executed_start; /* of executed: steps run inside the start kernel */
+  double ms_compact;     /* tile scan + scatter of the active list, ms */
+  double ms_finalize;    /* smoothing / output kernel, ms */
 } nm_render_stats;
 
 typedef struct nm_mandelbrot nm_mandelbrot; /* class Mandelbrot (mandelbrot.h:7) */

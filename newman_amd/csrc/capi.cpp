@@ -64,6 +64,9 @@ static void copy_stats(const nm::RenderStats& s, nm_render_stats* o) {
   o->ms_total = s.ms_total;
   o->block_k = s.block_k;
   o->waves_per_simd = s.waves_per_simd;
+  o->executed_start = s.executed_start;
+  o->ms_compact = s.ms_compact;
+  o->ms_finalize = s.ms_finalize;
 }
 
 extern "C" {

@@ -74,13 +74,16 @@ struct RenderStats {
   uint64_t n_cardioid = 0;   // samples resolved by the cardioid/bulb test
   uint64_t n_sa_resolved = 0; // escaped inside the series skip, or the skip reached N
   uint64_t n_iterated = 0;   // samples handed to the iteration kernel
-  uint64_t executed = 0;     // iteration steps executed by the iteration kernel
+  uint64_t executed = 0;     // iteration steps executed (start and iteration kernels)
+  uint64_t executed_start = 0; // of executed: steps run inside the start kernel
   uint64_t skipped = 0;      // sum of series-skip start indices of iterated samples
-  uint64_t n_rebased = 0;    // perturbation rebases (glitch avoidance)
+  uint64_t n_rebased = 0;    // blocks of block_k steps in which a sample rebased (glitch avoidance)
   uint64_t credited = 0;     // sum of reported iterations over non-cardioid samples
-  double ms_start = 0;       // classification / series start kernel (last repeat)
-  double ms_iterate = 0;     // iteration kernel (average over repeats)
-  double ms_total = 0;       // both kernels, average over repeats
+  double ms_start = 0;       // classification / series start kernel
+  double ms_compact = 0;     // tile scan + scatter
+  double ms_iterate = 0;     // iteration kernel
+  double ms_finalize = 0;    // smoothing / output kernel
+  double ms_total = 0;       // all of one pass; every time is a mean over repeats
   int block_k = 0, waves_per_simd = 0;
 };
 

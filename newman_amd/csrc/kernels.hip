@@ -59,7 +59,8 @@ enum CounterIdx {
   C_SKIPPED = 3,
   C_REBASED = 4,
   C_CREDITED = 5,
-  C_COUNT = 6
+  C_EXEC_START = 6,
+  C_COUNT = 7
 };
 
 constexpr int kThreads = 256;           // workgroup size of every kernel
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
     tile_count_store(c.tile_count, tile, mine);
   }
   const unsigned long long v[3] = {ncard, nres, executed};
-  const int idx[3] = {C_CARDIOID, C_SA_RES, C_EXECUTED};
+  const int idx[3] = {C_CARDIOID, C_SA_RES, C_EXEC_START};
   block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 3);
 }
 
@@ -910,7 +911,7 @@ __global__ void k_zero(unsigned long long* p, int n) {
 class EngineImpl : public Engine {
  public:
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {};
+  hipEvent_t ev[5] = {};
   int device = 0;
   int num_cu = 256;
   std::string err;
@@ -1066,33 +1067,32 @@ class EngineImpl : public Engine {
     const uint32_t ntiles = (uint32_t)((c.total + kTile - 1) / kTile);
     const int gtile = grid_for_tiles(c.total);
     const int giter = std::min(num_cu * waves, kSlotSum - kSlotIter);
-    float ms_start = 0, ms_iter = 0, ms_total = 0;
+    double ms[4] = {};
     for (int rep = 0; rep < repeats; rep++) {
       hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartSlots * C_COUNT);
       NM_HIP(hipGetLastError());
       NM_HIP(hipEventRecord(ev[0], stream));
       hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args);
       NM_HIP(hipGetLastError());
+      NM_HIP(hipEventRecord(ev[1], stream));
       hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, c.tile_count, ntiles, d_nactive);
       NM_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint8_t*)c.flag,
                          (const uint32_t*)c.tile_count, c.total, c.active);
       NM_HIP(hipGetLastError());
-      NM_HIP(hipEventRecord(ev[1], stream));
+      NM_HIP(hipEventRecord(ev[2], stream));
       hipLaunchKernelGGL(iter, dim3(giter), dim3(kThreads), 0, stream, args);
       NM_HIP(hipGetLastError());
-      NM_HIP(hipEventRecord(ev[2], stream));
+      NM_HIP(hipEventRecord(ev[3], stream));
       hipLaunchKernelGGL(k_finalize, dim3(gtile), dim3(kThreads), 0, stream, c);
       NM_HIP(hipGetLastError());
-      NM_HIP(hipEventRecord(ev[3], stream));
-      NM_HIP(hipEventSynchronize(ev[3]));
-      float t1 = 0, t2 = 0, t3 = 0;
-      NM_HIP(hipEventElapsedTime(&t1, ev[0], ev[1]));
-      NM_HIP(hipEventElapsedTime(&t2, ev[1], ev[2]));
-      NM_HIP(hipEventElapsedTime(&t3, ev[2], ev[3]));
-      ms_start = t1 + t3;
-      ms_iter += t2;
-      ms_total += t1 + t2 + t3;
+      NM_HIP(hipEventRecord(ev[4], stream));
+      NM_HIP(hipEventSynchronize(ev[4]));
+      for (int k = 0; k < 4; k++) {
+        float t = 0;
+        NM_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+        ms[k] += t;
+      }
     }
     hipLaunchKernelGGL(k_sum_iterations, dim3(std::min(num_cu * 4, kPartSlots - kSlotSum)), dim3(kThreads), 0,
                        stream, (const EscapeValue*)d_out, (unsigned long long)nout, d_part);
@@ -1111,13 +1111,16 @@ class EngineImpl : public Engine {
       stats->n_cardioid = h[C_CARDIOID];
       stats->n_sa_resolved = h[C_SA_RES];
       stats->n_iterated = nact;
-      stats->executed = h[C_EXECUTED];
+      stats->executed = h[C_EXECUTED] + h[C_EXEC_START];
+      stats->executed_start = h[C_EXEC_START];
       stats->skipped = h[C_SKIPPED];
       stats->n_rebased = h[C_REBASED];
       stats->credited = h[C_CREDITED] - h[C_CARDIOID] * (unsigned long long)opt.N;
-      stats->ms_start = ms_start;
-      stats->ms_iterate = ms_iter / repeats;
-      stats->ms_total = ms_total / repeats;
+      stats->ms_start = ms[0] / repeats;
+      stats->ms_compact = ms[1] / repeats;
+      stats->ms_iterate = ms[2] / repeats;
+      stats->ms_finalize = ms[3] / repeats;
+      stats->ms_total = (ms[0] + ms[1] + ms[2] + ms[3]) / repeats;
     }
     return NM_OK;
   }
