@@ -58,6 +58,8 @@ typedef struct nm_render_stats {
   uint64_t executed_start; /* of executed: steps run inside the start kernel */
   double ms_compact;     /* tile scan + scatter of the active list, ms */
   double ms_finalize;    /* smoothing / output kernel, ms */
+  int32_t graph;         /* 1: ms_total is the mean of HIP graph replays of the pass */
+  int32_t tile;          /* sample order used: 0 row-major, T = T x T tiles */
 } nm_render_stats;
 
 typedef struct nm_mandelbrot nm_mandelbrot; /* class Mandelbrot (mandelbrot.h:7) */
@@ -135,6 +137,26 @@ int nm_render(nm_mandelbrot* h, int block_k, int waves_per_simd, int repeats, in
  * strings. */
 int nm_render_frames(nm_mandelbrot* h, const char* const* widths, int nframes, int block_k, int waves_per_simd,
                      int repeats, nm_escape_value* out, nm_render_stats* stats);
+/* extension: launch settings of later renders on h.  0 keeps the engine's
+ * default; tile = sample order (0 row-major, or T x T tiles of 8 x 8 blocks, T in
+ * 8..256, a power of two); graph = 1 replays repeated passes from one captured
+ * HIP graph; ms = the kernel time the autotuner measured (output only). */
+typedef struct nm_launch {
+  int32_t block_k;
+  int32_t waves_per_simd;
+  int32_t tile;
+  int32_t graph;
+  double ms;
+} nm_launch;
+int nm_set_launch(nm_mandelbrot* h, const nm_launch* l);
+int nm_get_launch(const nm_mandelbrot* h, nm_launch* l);
+/* extension: the launch autotuner for a frame set (see nm_render_frames):
+ * sweeps sample order x waves per SIMD x block size on an evenly spaced
+ * subsample of at most 16 frames, makes the fastest h's launch settings and
+ * writes it to *best; up to cap sweep rows go to table (NULL ok), their count
+ * to *n_table. */
+int nm_autotune_frames(nm_mandelbrot* h, const char* const* widths, int nframes, nm_launch* best,
+                       nm_launch* table, int cap, int* n_table);
 /* the text of the last engine error */
 const char* nm_last_error(void);
 

@@ -47,7 +47,15 @@ class RenderStats(C.Structure):
                 ("n_rebased", C.c_uint64), ("credited", C.c_uint64), ("ms_start", C.c_double),
                 ("ms_iterate", C.c_double), ("ms_total", C.c_double), ("block_k", C.c_int32),
                 ("waves_per_simd", C.c_int32), ("executed_start", C.c_uint64), ("ms_compact", C.c_double),
-                ("ms_finalize", C.c_double)]
+                ("ms_finalize", C.c_double), ("graph", C.c_int32), ("tile", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Launch(C.Structure):
+    _fields_ = [("block_k", C.c_int32), ("waves_per_simd", C.c_int32), ("tile", C.c_int32), ("graph", C.c_int32),
+                ("ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -99,6 +107,10 @@ _SIGS = [
     ("nm_render", C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(RenderStats)]),
     ("nm_render_frames", C.c_int, [_H, C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                    C.POINTER(RenderStats)]),
+    ("nm_set_launch", C.c_int, [_H, C.c_void_p]),
+    ("nm_get_launch", C.c_int, [_H, C.c_void_p]),
+    ("nm_autotune_frames", C.c_int, [_H, C.POINTER(C.c_char_p), C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.POINTER(C.c_int)]),
     ("nm_last_error", C.c_char_p, []),
     ("nm_hp_op", C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_long, C.c_double, C.c_long,
                            C.POINTER(C.c_int), C.POINTER(C.c_long), C.c_void_p]),
@@ -314,6 +326,26 @@ class Mandelbrot:
         _check(lib().nm_render(self._h, block_k, waves_per_simd, repeats, 1 if copy_back else 0, C.byref(st)),
                "nm_render")
         return st.as_dict()
+
+    def set_launch(self, block_k=0, waves_per_simd=0, tile=0, graph=False):
+        l = Launch(block_k, waves_per_simd, tile, 1 if graph else 0, 0.0)
+        _check(lib().nm_set_launch(self._h, C.byref(l)), "nm_set_launch")
+
+    def get_launch(self):
+        l = Launch()
+        _check(lib().nm_get_launch(self._h, C.byref(l)), "nm_get_launch")
+        return l.as_dict()
+
+    def autotune_frames(self, widths):
+        """Sweep the launch settings on the frame set; returns (best, table)."""
+        n = len(widths)
+        arr = (C.c_char_p * n)(*[_b(str(w)) for w in widths])
+        best = Launch()
+        table = (Launch * 64)()
+        nt = C.c_int(0)
+        _check(lib().nm_autotune_frames(self._h, arr, n, C.byref(best), table, 64, C.byref(nt)),
+               "nm_autotune_frames")
+        return best.as_dict(), [table[i].as_dict() for i in range(nt.value)]
 
     def render_frames(self, widths, block_k=0, waves_per_simd=0, repeats=1, copy_back=True):
         """Frames sharing this view's centre and reference; copy_back=False leaves

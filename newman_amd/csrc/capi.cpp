@@ -67,6 +67,8 @@ static void copy_stats(const nm::RenderStats& s, nm_render_stats* o) {
   o->executed_start = s.executed_start;
   o->ms_compact = s.ms_compact;
   o->ms_finalize = s.ms_finalize;
+  o->graph = s.graph;
+  o->tile = s.tile;
 }
 
 extern "C" {
@@ -310,6 +312,47 @@ int nm_render_frames(nm_mandelbrot* h, const char* const* widths, int nframes, i
     int st = h->m.renderFrames(w, &o, (nm::EscapeValue*)out);
     if (st == NM_OK) copy_stats(h->m.lastStats(), stats);
     return engine_status(st);
+  });
+}
+
+int nm_set_launch(nm_mandelbrot* h, const nm_launch* l) {
+  if (!h || !l || l->block_k < 0 || l->waves_per_simd < 0 || l->waves_per_simd > 16) return NM_ERR_ARG;
+  const int t = l->tile;
+  if (t != 0 && t != 8 && t != 16 && t != 32 && t != 64 && t != 128 && t != 256) return NM_ERR_ARG;
+  h->m.launch.block_k = l->block_k;
+  h->m.launch.waves_per_simd = l->waves_per_simd;
+  h->m.launch.tile = t;
+  h->m.launch.graph = l->graph != 0;
+  h->m.launch.ms = 0.0;
+  return NM_OK;
+}
+
+int nm_get_launch(const nm_mandelbrot* h, nm_launch* l) {
+  if (!h || !l) return NM_ERR_ARG;
+  l->block_k = h->m.launch.block_k;
+  l->waves_per_simd = h->m.launch.waves_per_simd;
+  l->tile = h->m.launch.tile;
+  l->graph = h->m.launch.graph;
+  l->ms = h->m.launch.ms;
+  return NM_OK;
+}
+
+int nm_autotune_frames(nm_mandelbrot* h, const char* const* widths, int nframes, nm_launch* best,
+                       nm_launch* table, int cap, int* n_table) {
+  if (!h || !widths || nframes <= 0 || cap < 0) return NM_ERR_ARG;
+  return guard([&] {
+    std::vector<std::string> w(widths, widths + nframes);
+    std::vector<nm::Mandelbrot::Launch> rows;
+    int st = h->m.autotuneFrames(w, &rows);
+    if (st != NM_OK) return engine_status(st);
+    if (best) nm_get_launch(h, best);
+    int n = 0;
+    for (const auto& r : rows) {
+      if (table && n < cap) table[n] = nm_launch{r.block_k, r.waves_per_simd, r.tile, r.graph, r.ms};
+      n++;
+    }
+    if (n_table) *n_table = std::min(n, cap);
+    return (int)NM_OK;
   });
 }
 

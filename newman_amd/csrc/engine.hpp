@@ -67,6 +67,8 @@ struct RenderOpts {
   int block_k = 0;        // iterations per block between lane checks (0 = tuned default)
   int waves_per_simd = 0; // persistent waves per SIMD (0 = tuned default)
   int repeats = 1;        // timing: launch the whole frame set this many times
+  int tile = 0;           // sample order: 0 row-major, T = T x T tiles of 8 x 8 blocks (8..256)
+  bool graph = false;     // replay repeats 2.. from one captured HIP graph
 };
 
 struct RenderStats {
@@ -84,6 +86,8 @@ struct RenderStats {
   double ms_iterate = 0;     // iteration kernel
   double ms_finalize = 0;    // smoothing / output kernel
   double ms_total = 0;       // all of one pass; every time is a mean over repeats
+  int graph = 0;             // 1: ms_total is the mean of HIP graph replays
+  int tile = 0;              // the sample order used
   int block_k = 0, waves_per_simd = 0;
 };
 

@@ -48,8 +48,13 @@ struct DevFrame {
   uint32_t nr, nc;
   uint32_t tab_c, tab_r;  // offsets into the column / row tables
   int32_t card_mode;
-  uint64_t out_base;   // first output element
-  uint64_t mask_base;  // first bit in the cardioid mask
+  uint64_t out_base;   // first output element (row-major grid of this frame)
+  uint64_t mask_base;  // first bit in the cardioid mask (row-major)
+  // sample order: tile == 0 is row-major; otherwise T x T tiles (row-major tile
+  // grid, edge tiles padded) of 8 x 8 micro-blocks, one wave per micro-block
+  uint32_t tile, ntc;     // T, tiles per frame row
+  uint32_t tshift, mshift;  // log2(T*T), log2(T/8)
+  double inv_nc, inv_ntc;
 };
 
 enum CounterIdx {
@@ -136,6 +141,29 @@ __device__ __forceinline__ uint32_t udiv(uint32_t x, double inv) { return (uint3
 
 // every frame of a frame set has the same size: frame = sample id / frame size
 __device__ __forceinline__ uint32_t frame_of(uint32_t gid, double inv_frame) { return udiv(gid, inv_frame); }
+
+// the (row, column) of a frame-local sample id; padding samples of edge tiles
+// are not valid
+struct Loc {
+  uint32_t r, c;
+  bool valid;
+};
+__device__ __forceinline__ Loc locate(const DevFrame& f, uint32_t local) {
+  Loc l;
+  if (f.tile == 0) {
+    l.r = udiv(local, f.inv_nc);
+    l.c = local - l.r * f.nc;
+    l.valid = true;
+    return l;
+  }
+  const uint32_t t = local >> f.tshift, w = local & ((1u << f.tshift) - 1u);
+  const uint32_t tr = udiv(t, f.inv_ntc), tc = t - tr * f.ntc;
+  const uint32_t mb = w >> 6, in = w & 63u;
+  l.r = tr * f.tile + ((mb >> f.mshift) << 3) + (in >> 3);
+  l.c = tc * f.tile + ((mb & ((1u << f.mshift) - 1u)) << 3) + (in & 7u);
+  l.valid = l.r < f.nr && l.c < f.nc;
+  return l;
+}
 
 // mandelbrot.cpp:133-136 (log(bailout) comes from the host)
 __device__ __forceinline__ float smooth_of(double r2, double log_b) {
@@ -257,8 +285,12 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
       uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
       if (gid >= c.total) continue;
       const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
-      uint32_t local = gid - f.sample_base;
-      uint32_t r = udiv(local, a.inv_ncols), col = local - r * a.ncols;
+      const Loc lc = locate(f, gid - f.sample_base);
+      if (!lc.valid) {
+        c.flag[gid] = 0;
+        continue;
+      }
+      const uint32_t r = lc.r, col = lc.c;
       bool in = false;
       const double cx = a.cre[f.tab_c + col], cy = a.cim[f.tab_r + r];
       if (f.card_mode == CARD_ALL) in = true;
@@ -604,20 +636,19 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
       bool valid = gid < c.total;
       // phase 1: locate the sample, cardioid class, epsilon and log2 |e|^2
       bool in = false;
-      uint64_t out_idx = 0;
       double L = -INFINITY;
       double2 e = make_double2(0.0, 0.0);
       if (valid) {
         const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
-        uint32_t local = gid - f.sample_base;
-        uint32_t r = local / f.nc, col = local - r * f.nc;
+        const Loc lc = locate(f, gid - f.sample_base);
+        valid = lc.valid;
+        const uint32_t r = lc.r, col = lc.c;
         in = f.card_mode == CARD_ALL;
-        if (f.card_mode == CARD_MASK) {
-          uint64_t bit = f.mask_base + local;
+        if (valid && f.card_mode == CARD_MASK) {
+          uint64_t bit = f.mask_base + (uint64_t)r * f.nc + col;
           in = (a.mask[bit >> 5] >> (bit & 31)) & 1u;
         }
-        out_idx = f.out_base + local;
-        if (!in) {
+        if (valid && !in) {
           e = make_double2(a.epsre[f.tab_c + col], a.epsim[f.tab_r + r]);
           L = log2_sqmag(e);
         }
@@ -625,7 +656,10 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
       // phase 2 (all lanes): the wave's bracket of the stop-index search
       const int2 br = wave_stop_bracket(a, L, valid && !in && L > -INFINITY);
       // phase 3: the reference's series start for this sample
-      if (!valid) continue;
+      if (!valid) {
+        if (gid < c.total) c.flag[gid] = 0;
+        continue;
+      }
       bool want = false;
       EscapeValue res{c.N, 0.0f};
       if (in) {
@@ -688,7 +722,7 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
           nres++;  // the skip reached N: (N, 0) as the empty HP loop returns
         }
       }
-      if (!want) c.out[out_idx] = res;
+      if (!want) c.out[gid] = res;
       c.flag[gid] = want ? 1 : 0;
       mine += want ? 1 : 0;
     }
@@ -858,6 +892,15 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
 
 // smoothing of the iterated samples (mandelbrot.cpp:133-136), converged: one
 // thread per entry of active[]
+// tiled sample order: results (sample order) -> the row-major grids
+__global__ __launch_bounds__(kThreads) void k_emit(Common c, EscapeValue* grid) {
+  for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < c.total; gid += gridDim.x * blockDim.x) {
+    const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
+    const Loc lc = locate(f, gid - f.sample_base);
+    if (lc.valid) grid[f.out_base + (uint64_t)lc.r * f.nc + lc.c] = c.out[gid];
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_finalize(Common c) {
   const uint32_t total = *c.nactive;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
@@ -911,7 +954,7 @@ __global__ void k_zero(unsigned long long* p, int n) {
 class EngineImpl : public Engine {
  public:
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {};
+  hipEvent_t ev[5] = {};  // stage boundaries of the eager pass
   int device = 0;
   int num_cu = 256;
   std::string err;
@@ -928,6 +971,7 @@ class EngineImpl : public Engine {
   void* d_state_i = nullptr; size_t n_state_i_bytes = 0;
   void* d_ref = nullptr; size_t n_ref_bytes = 0;
   void* d_grid = nullptr; size_t n_grid_bytes = 0;  // downsample staging
+  void* d_emit = nullptr; size_t n_emit_bytes = 0;  // row-major grids of a tiled sample order
   unsigned long long* d_part = nullptr;               // kPartSlots x C_COUNT
   uint32_t* d_nactive = nullptr;
   std::vector<unsigned long long> h_part;
@@ -1058,50 +1102,109 @@ class EngineImpl : public Engine {
     return NM_OK;
   }
 
-  // start -> scan -> scatter -> iterate, `repeats` times; then counters and output
+  // The kernels of one pass: zero the counters, start, scan, scatter,
+  // iterate, finalize (and emit for tiled sample orders).  ev != nullptr
+  // records an event before each stage (eager timing); under stream capture
+  // it is null.
   template <class Args, class StartFn, class IterFn>
-  int run(Args& args, StartFn start, IterFn iter, int waves, const RenderOpts& opt, RenderStats* stats,
-          uint64_t nout, EscapeValue* host_out) {
+  int launch_pass(Args& args, StartFn start, IterFn iter, int waves, EscapeValue* emit_to, hipEvent_t* evs) {
     const Common& c = args.c;
-    const int repeats = std::max(1, opt.repeats);
     const uint32_t ntiles = (uint32_t)((c.total + kTile - 1) / kTile);
     const int gtile = grid_for_tiles(c.total);
     const int giter = std::min(num_cu * waves, kSlotSum - kSlotIter);
-    double ms[4] = {};
-    for (int rep = 0; rep < repeats; rep++) {
-      hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartSlots * C_COUNT);
+    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartSlots * C_COUNT);
+    NM_HIP(hipGetLastError());
+    if (evs) NM_HIP(hipEventRecord(evs[0], stream));
+    hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args);
+    NM_HIP(hipGetLastError());
+    if (evs) NM_HIP(hipEventRecord(evs[1], stream));
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, c.tile_count, ntiles, d_nactive);
+    NM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint8_t*)c.flag,
+                       (const uint32_t*)c.tile_count, c.total, c.active);
+    NM_HIP(hipGetLastError());
+    if (evs) NM_HIP(hipEventRecord(evs[2], stream));
+    hipLaunchKernelGGL(iter, dim3(giter), dim3(kThreads), 0, stream, args);
+    NM_HIP(hipGetLastError());
+    if (evs) NM_HIP(hipEventRecord(evs[3], stream));
+    hipLaunchKernelGGL(k_finalize, dim3(gtile), dim3(kThreads), 0, stream, c);
+    NM_HIP(hipGetLastError());
+    if (emit_to) {
+      hipLaunchKernelGGL(k_emit, dim3(gtile), dim3(kThreads), 0, stream, c, emit_to);
       NM_HIP(hipGetLastError());
-      NM_HIP(hipEventRecord(ev[0], stream));
-      hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args);
-      NM_HIP(hipGetLastError());
-      NM_HIP(hipEventRecord(ev[1], stream));
-      hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, c.tile_count, ntiles, d_nactive);
-      NM_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint8_t*)c.flag,
-                         (const uint32_t*)c.tile_count, c.total, c.active);
-      NM_HIP(hipGetLastError());
-      NM_HIP(hipEventRecord(ev[2], stream));
-      hipLaunchKernelGGL(iter, dim3(giter), dim3(kThreads), 0, stream, args);
-      NM_HIP(hipGetLastError());
-      NM_HIP(hipEventRecord(ev[3], stream));
-      hipLaunchKernelGGL(k_finalize, dim3(gtile), dim3(kThreads), 0, stream, c);
-      NM_HIP(hipGetLastError());
-      NM_HIP(hipEventRecord(ev[4], stream));
-      NM_HIP(hipEventSynchronize(ev[4]));
-      for (int k = 0; k < 4; k++) {
-        float t = 0;
-        NM_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-        ms[k] += t;
-      }
     }
+    if (evs) NM_HIP(hipEventRecord(evs[4], stream));
+    return NM_OK;
+  }
+
+  // The pass `repeats` times; then counters and output.  The first pass runs
+  // eagerly with per-stage events; with opt.graph the pass is captured once into
+  // a HIP graph and the other repeats replay it (timed as a whole).
+  template <class Args, class StartFn, class IterFn>
+  int run(Args& args, StartFn start, IterFn iter, int waves, const RenderOpts& opt, RenderStats* stats,
+          uint64_t nout, EscapeValue* host_out, EscapeValue* emit_to) {
+    const int repeats = std::max(1, opt.repeats);
+    double ms[4] = {};
+    double ms_graph = 0.0;
+    int st;
+    if ((st = launch_pass(args, start, iter, waves, emit_to, ev))) return st;
+    NM_HIP(hipEventSynchronize(ev[4]));
+    for (int k = 0; k < 4; k++) {
+      float t = 0;
+      NM_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      ms[k] = t;
+    }
+    if (repeats > 1 && opt.graph) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      NM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      st = launch_pass(args, start, iter, waves, emit_to, nullptr);
+      hipError_t ce = hipStreamEndCapture(stream, &graph);
+      if (st) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      if (ce != hipSuccess) return fail("hipStreamEndCapture", ce);
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail("hipGraphInstantiate", ie);
+      }
+      for (int rep = 1; rep < repeats && st == NM_OK; rep++) {
+        hipError_t e1 = hipEventRecord(ev[0], stream);
+        hipError_t e2 = e1 == hipSuccess ? hipGraphLaunch(exec, stream) : e1;
+        hipError_t e3 = e2 == hipSuccess ? hipEventRecord(ev[4], stream) : e2;
+        hipError_t e4 = e3 == hipSuccess ? hipEventSynchronize(ev[4]) : e3;
+        float t = 0;
+        hipError_t e5 = e4 == hipSuccess ? hipEventElapsedTime(&t, ev[0], ev[4]) : e4;
+        if (e5 != hipSuccess) st = fail("graph replay", e5);
+        ms_graph += t;
+      }
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      if (st) return st;
+      ms_graph /= (repeats - 1);
+    } else {
+      for (int rep = 1; rep < repeats; rep++) {
+        if ((st = launch_pass(args, start, iter, waves, emit_to, ev))) return st;
+        NM_HIP(hipEventSynchronize(ev[4]));
+        for (int k = 0; k < 4; k++) {
+          float t = 0;
+          NM_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+          ms[k] += t;
+        }
+      }
+      for (double& v : ms) v /= repeats;
+    }
+    const EscapeValue* result = emit_to ? emit_to : (const EscapeValue*)d_out;
     hipLaunchKernelGGL(k_sum_iterations, dim3(std::min(num_cu * 4, kPartSlots - kSlotSum)), dim3(kThreads), 0,
-                       stream, (const EscapeValue*)d_out, (unsigned long long)nout, d_part);
+                       stream, result, (unsigned long long)nout, d_part);
     NM_HIP(hipGetLastError());
     uint32_t nact = 0;
     NM_HIP(hipMemcpyAsync(h_part.data(), d_part, h_part.size() * 8, hipMemcpyDeviceToHost, stream));
     NM_HIP(hipMemcpyAsync(&nact, d_nactive, 4, hipMemcpyDeviceToHost, stream));
     if (host_out)
-      NM_HIP(hipMemcpyAsync(host_out, d_out, nout * sizeof(EscapeValue), hipMemcpyDeviceToHost, stream));
+      NM_HIP(hipMemcpyAsync(host_out, result, nout * sizeof(EscapeValue), hipMemcpyDeviceToHost, stream));
     NM_HIP(hipStreamSynchronize(stream));
     unsigned long long h[C_COUNT] = {};
     for (int s = 0; s < kPartSlots; s++)
@@ -1116,13 +1219,43 @@ class EngineImpl : public Engine {
       stats->skipped = h[C_SKIPPED];
       stats->n_rebased = h[C_REBASED];
       stats->credited = h[C_CREDITED] - h[C_CARDIOID] * (unsigned long long)opt.N;
-      stats->ms_start = ms[0] / repeats;
-      stats->ms_compact = ms[1] / repeats;
-      stats->ms_iterate = ms[2] / repeats;
-      stats->ms_finalize = ms[3] / repeats;
-      stats->ms_total = (ms[0] + ms[1] + ms[2] + ms[3]) / repeats;
+      stats->ms_start = ms[0];
+      stats->ms_compact = ms[1];
+      stats->ms_iterate = ms[2];
+      stats->ms_finalize = ms[3];
+      stats->ms_total = ms_graph > 0.0 ? ms_graph : ms[0] + ms[1] + ms[2] + ms[3];
+      stats->graph = ms_graph > 0.0 ? 1 : 0;
+      stats->tile = emit_to ? opt.tile : 0;
     }
     return NM_OK;
+  }
+
+  // DevFrame for frame k of a set: sample order, table offsets, bases
+  static DevFrame make_frame(uint64_t k, int nr, int nc, int tile, uint32_t frame_size) {
+    DevFrame d{};
+    d.sample_base = (uint32_t)(k * frame_size);
+    d.nr = (uint32_t)nr;
+    d.nc = (uint32_t)nc;
+    d.out_base = k * (uint64_t)nr * nc;
+    d.inv_nc = 1.0 / (double)nc;
+    d.tile = (uint32_t)tile;
+    if (tile) {
+      d.ntc = (uint32_t)((nc + tile - 1) / tile);
+      d.inv_ntc = 1.0 / (double)d.ntc;
+      int lt = 0;
+      while ((1 << lt) < tile) lt++;
+      d.tshift = (uint32_t)(2 * lt);
+      d.mshift = (uint32_t)(lt - 3);
+    }
+    return d;
+  }
+
+  // samples per frame in the sample order (tiles pad the frame); 0 = bad tile
+  static uint64_t frame_samples(int nr, int nc, int tile) {
+    if (tile == 0) return (uint64_t)nr * nc;
+    if (tile != 8 && tile != 16 && tile != 32 && tile != 64 && tile != 128 && tile != 256) return 0;
+    uint64_t ntr = (uint64_t)(nr + tile - 1) / tile, ntc = (uint64_t)(nc + tile - 1) / tile;
+    return ntr * ntc * (uint64_t)tile * tile;
   }
 
   int render_hw(const std::vector<const HwFrameHost*>& frames, const RenderOpts& opt, EscapeValue* host_out,
@@ -1139,21 +1272,20 @@ class EngineImpl : public Engine {
     }
     std::vector<double> cre, crelo, cim, cimlo;
     cre.reserve(ncol); crelo.reserve(ncol); cim.reserve(nrow); cimlo.reserve(nrow);
+    const uint64_t fs = frame_samples(frames[0]->nr, frames[0]->nc, opt.tile);
+    if (fs == 0) return NM_ERR_ARG;
+    uint64_t nout = 0;
     for (auto* f : frames) {
-      DevFrame d{};
-      d.sample_base = (uint32_t)total;
-      d.nr = f->nr;
-      d.nc = f->nc;
+      DevFrame d = make_frame(df.size(), f->nr, f->nc, opt.tile, (uint32_t)fs);
       d.tab_c = (uint32_t)cre.size();
       d.tab_r = (uint32_t)cim.size();
       d.card_mode = f->card_mode;
-      d.out_base = total;
-      d.mask_base = 0;
       cre.insert(cre.end(), f->cre.begin(), f->cre.end());
       crelo.insert(crelo.end(), f->crelo.begin(), f->crelo.end());
       cim.insert(cim.end(), f->cim.begin(), f->cim.end());
       cimlo.insert(cimlo.end(), f->cimlo.begin(), f->cimlo.end());
-      total += (uint64_t)f->nr * f->nc;
+      total += fs;
+      nout += (uint64_t)f->nr * f->nc;
       df.push_back(d);
     }
     int st;
@@ -1161,8 +1293,13 @@ class EngineImpl : public Engine {
     if ((st = grow(&d_tabs, &n_tabs_bytes, (2 * ncol + 2 * nrow) * sizeof(double)))) return st;
     PlainArgs a;
     if ((st = prepare_common(a.c, total, opt))) return st;
+    EscapeValue* emit_to = nullptr;
+    if (opt.tile) {
+      if ((st = grow(&d_emit, &n_emit_bytes, nout * sizeof(EscapeValue)))) return st;
+      emit_to = (EscapeValue*)d_emit;
+    }
     a.c.nframes = (int)df.size();
-    a.c.frame_size = (uint32_t)(total / df.size());
+    a.c.frame_size = (uint32_t)fs;
     a.c.inv_frame = 1.0 / (double)a.c.frame_size;
     double* t = (double*)d_tabs;
     NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
@@ -1170,9 +1307,6 @@ class EngineImpl : public Engine {
     NM_HIP(hipMemcpyAsync(t + ncol, crelo.data(), ncol * 8, hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(t + 2 * ncol, cim.data(), nrow * 8, hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(t + 2 * ncol + nrow, cimlo.data(), nrow * 8, hipMemcpyHostToDevice, stream));
-    a.nrows = (uint32_t)frames[0]->nr;
-    a.ncols = (uint32_t)frames[0]->nc;
-    a.inv_ncols = 1.0 / (double)a.ncols;
     a.cre = t;
     a.crelo = t + ncol;
     a.cim = t + 2 * ncol;
@@ -1185,10 +1319,10 @@ class EngineImpl : public Engine {
     }
     // 256-thread workgroups = 4 waves = one per SIMD: num_cu * waves workgroups
     switch (K) {
-      case 8: return run(a, k_plain_start, k_plain_iterate<8>, waves, opt, stats, total, host_out);
-      case 16: return run(a, k_plain_start, k_plain_iterate<16>, waves, opt, stats, total, host_out);
-      case 32: return run(a, k_plain_start, k_plain_iterate<32>, waves, opt, stats, total, host_out);
-      case 64: return run(a, k_plain_start, k_plain_iterate<64>, waves, opt, stats, total, host_out);
+      case 8: return run(a, k_plain_start, k_plain_iterate<8>, waves, opt, stats, nout, host_out, emit_to);
+      case 16: return run(a, k_plain_start, k_plain_iterate<16>, waves, opt, stats, nout, host_out, emit_to);
+      case 32: return run(a, k_plain_start, k_plain_iterate<32>, waves, opt, stats, nout, host_out, emit_to);
+      case 64: return run(a, k_plain_start, k_plain_iterate<64>, waves, opt, stats, nout, host_out, emit_to);
       default: return NM_ERR_ARG;
     }
   }
@@ -1208,16 +1342,14 @@ class EngineImpl : public Engine {
     std::vector<double> er, ei;
     std::vector<uint32_t> mask;
     er.reserve(ncol); ei.reserve(nrow); mask.reserve(nmask);
-    uint64_t total = 0;
+    uint64_t total = 0, nout = 0;
+    const uint64_t fs = frame_samples(frames[0]->nr, frames[0]->nc, opt.tile);
+    if (fs == 0) return NM_ERR_ARG;
     for (auto* f : frames) {
-      DevFrame d{};
-      d.sample_base = (uint32_t)total;
-      d.nr = f->nr;
-      d.nc = f->nc;
+      DevFrame d = make_frame(df.size(), f->nr, f->nc, opt.tile, (uint32_t)fs);
       d.tab_c = (uint32_t)er.size();
       d.tab_r = (uint32_t)ei.size();
       d.card_mode = f->card_mode;
-      d.out_base = total;
       d.mask_base = (uint64_t)mask.size() * 32;
       if (f->card_mode == CARD_MASK) {
         if (f->mask.size() * 32 < (size_t)f->nr * f->nc) return NM_ERR_ARG;
@@ -1225,7 +1357,8 @@ class EngineImpl : public Engine {
       }
       er.insert(er.end(), f->epsre.begin(), f->epsre.end());
       ei.insert(ei.end(), f->epsim.begin(), f->epsim.end());
-      total += (uint64_t)f->nr * f->nc;
+      total += fs;
+      nout += (uint64_t)f->nr * f->nc;
       df.push_back(d);
     }
     int st;
@@ -1234,8 +1367,13 @@ class EngineImpl : public Engine {
     if ((st = grow(&d_mask, &n_mask_bytes, std::max<size_t>(mask.size(), 1) * 4))) return st;
     SeriesArgs a;
     if ((st = prepare_common(a.c, total, opt))) return st;
+    EscapeValue* emit_to = nullptr;
+    if (opt.tile) {
+      if ((st = grow(&d_emit, &n_emit_bytes, nout * sizeof(EscapeValue)))) return st;
+      emit_to = (EscapeValue*)d_emit;
+    }
     a.c.nframes = (int)df.size();
-    a.c.frame_size = (uint32_t)(total / df.size());
+    a.c.frame_size = (uint32_t)fs;
     a.c.inv_frame = 1.0 / (double)a.c.frame_size;
     double* t = (double*)d_tabs;
     NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
@@ -1267,9 +1405,9 @@ class EngineImpl : public Engine {
       stats->waves_per_simd = waves;
     }
     switch (K) {
-      case 8: return run(a, k_series_start, k_series_iterate<8>, waves, opt, stats, total, host_out);
-      case 16: return run(a, k_series_start, k_series_iterate<16>, waves, opt, stats, total, host_out);
-      case 32: return run(a, k_series_start, k_series_iterate<32>, waves, opt, stats, total, host_out);
+      case 8: return run(a, k_series_start, k_series_iterate<8>, waves, opt, stats, nout, host_out, emit_to);
+      case 16: return run(a, k_series_start, k_series_iterate<16>, waves, opt, stats, nout, host_out, emit_to);
+      case 32: return run(a, k_series_start, k_series_iterate<32>, waves, opt, stats, nout, host_out, emit_to);
       default: return NM_ERR_ARG;
     }
   }

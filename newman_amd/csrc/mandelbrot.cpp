@@ -552,16 +552,25 @@ std::string Mandelbrot::snapshot() const {
   return s;
 }
 
+// the view's parameters and the launch settings into the engine options
+void Mandelbrot::applyLaunch(RenderOpts& o) const {
+  o.N = N;
+  o.bailout = bailout;
+  o.tol = error_tolerance;
+  o.delta_mode = delta_mode;
+  if (o.block_k == 0) o.block_k = launch.block_k;
+  if (o.waves_per_simd == 0) o.waves_per_simd = launch.waves_per_simd;
+  o.tile = launch.tile;
+  o.graph = launch.graph != 0;
+}
+
 int Mandelbrot::render(RenderOpts* opts, bool copy_back) {
   if (N <= 0 || rows() <= 0 || cols() <= 0) return NM_ERR_ARG;
   Engine* eng = nullptr;
   int st = Engine::instance(&eng);
   if (st != NM_OK) return st;
   RenderOpts o = opts ? *opts : RenderOpts();
-  o.N = N;
-  o.bailout = bailout;
-  o.tol = error_tolerance;
-  o.delta_mode = delta_mode;
+  applyLaunch(o);
   EscapeValue* out = copy_back ? grid.values.data() : nullptr;
   if (useHardware()) {
     HwFrameHost f;
@@ -597,16 +606,8 @@ int Mandelbrot::computeAll() { return render(nullptr, true); }
 // A zoom-video batch: every frame shares this object's centre, reference orbit
 // and grid size; frame k has sz = width_k / cols (square samples).  All frames go
 // to the device in one start launch and one persistent iterate launch.
-int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts* opts, EscapeValue* out) {
+int Mandelbrot::buildFrames(const std::vector<std::string>& widths, FrameSet& set) {
   if (widths.empty() || N <= 0) return NM_ERR_ARG;
-  Engine* eng = nullptr;
-  int st = Engine::instance(&eng);
-  if (st != NM_OK) return st;
-  RenderOpts o = opts ? *opts : RenderOpts();
-  o.N = N;
-  o.bailout = bailout;
-  o.tol = error_tolerance;
-  o.delta_mode = delta_mode;
   const int L = hp_limbs_for_bits(hp_default_bits());
   std::vector<HPComplex> sizes(widths.size());
   bool all_hw = true, any_hw = false;
@@ -621,25 +622,90 @@ int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts*
     any_hw = any_hw || hw;
   }
   if (any_hw && !all_hw) return NM_ERR_ARG;  // one path per batch
+  set.hw = all_hw;
   if (all_hw) {
-    std::vector<HwFrameHost> fs(widths.size());
-    std::vector<const HwFrameHost*> ptr;
-    parallel_for((long)fs.size(), [&](long k) { buildHwFrame(fs[k], center, sizes[k]); }, 1);
-    for (size_t k = 0; k < fs.size(); k++) ptr.push_back(&fs[k]);
-    return eng->render_hw(ptr, o, out, &stats_);
+    set.h.assign(widths.size(), HwFrameHost());
+    parallel_for((long)set.h.size(), [&](long k) { buildHwFrame(set.h[k], center, sizes[k]); }, 1);
+    return NM_OK;
   }
   if (!have_ref_) return NM_ERR_STATE;
+  set.s.assign(widths.size(), SaFrameHost());
+  std::vector<int> sts(set.s.size(), NM_OK);
+  parallel_for((long)set.s.size(), [&](long k) { sts[k] = buildSaFrame(set.s[k], center, sizes[k]); }, 1);
+  for (int v : sts)
+    if (v != NM_OK) return v;
+  return NM_OK;
+}
+
+int Mandelbrot::renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue* out, size_t first, size_t step) {
+  Engine* eng = nullptr;
+  int st = Engine::instance(&eng);
+  if (st != NM_OK) return st;
+  if (set.hw) {
+    std::vector<const HwFrameHost*> ptr;
+    for (size_t k = first; k < set.h.size(); k += step) ptr.push_back(&set.h[k]);
+    return eng->render_hw(ptr, o, out, &stats_);
+  }
   st = eng->set_reference(ref_, ref_serial_);
   if (st != NM_OK) return st;
-  std::vector<SaFrameHost> fs(widths.size());
   std::vector<const SaFrameHost*> ptr;
-  std::vector<int> sts(fs.size(), NM_OK);
-  parallel_for((long)fs.size(), [&](long k) { sts[k] = buildSaFrame(fs[k], center, sizes[k]); }, 1);
-  for (size_t k = 0; k < fs.size(); k++) {
-    if (sts[k] != NM_OK) return sts[k];
-    ptr.push_back(&fs[k]);
-  }
+  for (size_t k = first; k < set.s.size(); k += step) ptr.push_back(&set.s[k]);
   return eng->render_sa(ptr, o, out, &stats_);
+}
+
+int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts* opts, EscapeValue* out) {
+  RenderOpts o = opts ? *opts : RenderOpts();
+  applyLaunch(o);
+  FrameSet set;
+  int st = buildFrames(widths, set);
+  if (st != NM_OK) return st;
+  return renderSet(set, o, out, 0, 1);
+}
+
+// The project's launch autotuner.  Candidates: sample order (row-major, 32x32,
+// 64x64 and 128x128 tiles), persistent waves per SIMD (2, 4, 6, 8) and the block
+// size (8, 16 for the series kernel; 16, 32 for the plain one).  Each candidate
+// renders an evenly spaced subsample of at most 16 frames (so the set's depth
+// mix is kept) once untimed and once timed; the fastest total kernel time wins.
+int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vector<Launch>* table) {
+  FrameSet set;
+  int st = buildFrames(widths, set);
+  if (st != NM_OK) return st;
+  const size_t n = set.hw ? set.h.size() : set.s.size();
+  const size_t step = std::max<size_t>(1, (n + 15) / 16);
+  const int tiles[4] = {0, 32, 64, 128};
+  const int waves[4] = {2, 4, 6, 8};
+  const int ks_sa[2] = {8, 16}, ks_hw[2] = {16, 32};
+  Launch best;
+  best.ms = 1e300;
+  if (table) table->clear();
+  RenderStats keep = stats_;
+  const int* ks = set.hw ? ks_hw : ks_sa;
+  for (int tile : tiles)
+    for (int w : waves)
+      for (int ki = 0; ki < 2; ki++) {
+        const int k = ks[ki];
+        RenderOpts o;
+        o.block_k = k;
+        o.waves_per_simd = w;
+        applyLaunch(o);
+        o.tile = tile;
+        o.graph = false;
+        o.repeats = 1;
+        if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
+        if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
+        Launch l;
+        l.block_k = k;
+        l.waves_per_simd = w;
+        l.tile = tile;
+        l.graph = launch.graph;
+        l.ms = stats_.ms_total;
+        if (table) table->push_back(l);
+        if (l.ms < best.ms) best = l;
+      }
+  stats_ = keep;
+  launch = best;
+  return NM_OK;
 }
 
 // mandelbrot.cpp:285-290

@@ -38,6 +38,14 @@ class Mandelbrot {
   // reference's constant 1024, mandelbrot.cpp:58) and the delta arithmetic.
   double bailout = 1024.0;
   int delta_mode = DELTA_FP64;
+  // Launch settings for later renders (0 = the engine's defaults): iterations
+  // per block, persistent waves per SIMD, sample order (0 row-major, T = T x T
+  // tiles) and HIP-graph replay of repeated passes.  autotuneFrames sets them.
+  struct Launch {
+    int block_k = 0, waves_per_simd = 0, tile = 0, graph = 0;
+    double ms = 0.0;
+  };
+  Launch launch;
 
   Mandelbrot();
   Mandelbrot(int nr, int nc);
@@ -79,6 +87,18 @@ class Mandelbrot {
   // frames with this object's centre, reference and grid size, one per width
   // (width = cols * sz); outputs concatenated
   int renderFrames(const std::vector<std::string>& widths, RenderOpts* opts, EscapeValue* out);
+  // Sweeps block size, waves per SIMD and sample order over (a subsample of) the
+  // frame set, keeps the fastest in `launch` and returns the table.
+  int autotuneFrames(const std::vector<std::string>& widths, std::vector<Launch>* table);
+  struct FrameSet {
+    bool hw = false;
+    std::vector<HwFrameHost> h;
+    std::vector<SaFrameHost> s;
+  };
+  int buildFrames(const std::vector<std::string>& widths, FrameSet& set);
+  // frames first, first + step, ... of a built set, one launch set
+  int renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue* out, size_t first, size_t step);
+  void applyLaunch(RenderOpts& o) const;
   int buildHwFrame(HwFrameHost& f) { return buildHwFrame(f, center, sz); }
   int buildSaFrame(SaFrameHost& f) { return buildSaFrame(f, center, sz); }
   int buildHwFrame(HwFrameHost& f, const HPComplex& c, const HPComplex& s);
