@@ -15,7 +15,7 @@ iteration kernel (15 per executed perturbation step) over its HIP-event duration
 against the FP64 vector peak.  cpu_baseline = the CPU oracle (the reference method:
 series scan + HP continuation) on a row subsample of the same frame.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c4]
 """
 import argparse
 import json
@@ -38,6 +38,7 @@ def load_views():
 
 
 C1_BATCH = 512  # config 1 is ~2 us of work per frame: frames per launch set
+C4_FRAMES = 120
 
 
 def make_workload(name):
@@ -66,6 +67,22 @@ def make_workload(name):
         def step(k, w, reps=1):
             return m.render_frames(widths, k, w, reps, copy_back=False)[1]
         return m, desc, FLOP_PER_PLAIN_STEP, step
+    if name == "c4":
+        v = views["c4_seed1"]
+        widths = ["%.17e" % (1e-40 * 1.1 ** (C4_FRAMES - 1 - k)) for k in range(C4_FRAMES)]
+        m = nm.Mandelbrot(720, 1280)
+        m.set_view(v["N"], v["re"], v["im"], widths[-1])  # the deepest frame fixes precision and reference
+        desc = {"workload": "batched zoom video: %d frames of 1280x720, widths 1e-40 * 1.1^(119-k), one shared "
+                            "reference orbit at the deepest frame, N 60000, bailout 1024, tol 1e-10, centre = zoom "
+                            "walk seed 1; all frames in one start launch and one persistent iterate launch, "
+                            "repeated passes replayed from one HIP graph" % C4_FRAMES,
+                "samples": 1280 * 720 * C4_FRAMES, "N": 60000, "frames": C4_FRAMES,
+                "precision_bits": m.precision_bits()}
+
+        def step(k, w, reps=1):
+            return m.render_frames(widths, k, w, reps, copy_back=False)[1]
+        step.widths = widths
+        return m, desc, FLOP_PER_PERT_STEP, step
     raise SystemExit("unknown config " + name)
 
 
@@ -155,6 +172,29 @@ def cpu_baseline(desc, name, nthreads, rows_every):
                           "640x480 frame, repeated %d times (%.1f s); credited = iterations of non-cardioid samples"
                           % (reps, t_run)}
     views = load_views()
+    if name == "c4":
+        v = views["c4_seed1"]
+        nr, nc = 720, 1280
+        total_s = total_c = 0.0
+        pre = 0.0
+        nsamp = 0
+        frames = (0, 60, 119)
+        for k in frames:  # the reference method precomputes per keyframe (viewer.cpp:263)
+            o = O.OView(nr, nc)
+            o.set_view(v["N"], v["re"], v["im"], "%.17e" % (1e-40 * 1.1 ** (C4_FRAMES - 1 - k)))
+            t0 = time.perf_counter()
+            o.precompute(nthreads)
+            pre += time.perf_counter() - t0
+            rows = np.arange(rows_every // 4, nr, rows_every // 2, dtype=np.int32)
+            t0 = time.perf_counter()
+            out = o.render_rows(rows, nthreads)
+            total_s += time.perf_counter() - t0
+            total_c += float(out["iterations"].astype(np.int64).sum())
+            nsamp += out.size
+        return {"value": total_c / total_s, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
+                "sample": "oracle reference method (series scan + HP continuation) on rows %d::%d of frames %s "
+                          "(%d samples, %.3g credited px-it, %.1f s); per-frame precompute %.1f s, not included"
+                          % (rows_every // 4, rows_every // 2, frames, nsamp, total_c, total_s, pre)}
     v = views["c2_seed1"]
     nr, nc = 1080 * 4, 1920 * 4
     o = O.OView(nr, nc)
@@ -203,7 +243,16 @@ def main():
           file=sys.stderr, flush=True)
     k, w = args.block_k, args.waves
     tuned = None
-    if args.tune:
+    if args.config == "c4":
+        # the project's launch autotuner (sample order x waves x block size), then
+        # graph replay of the repeated passes
+        t0 = time.perf_counter()
+        best, table = m.autotune_frames(step.widths)
+        m.set_launch(best["block_k"], best["waves_per_simd"], best["tile"], graph=True)
+        k, w = best["block_k"], best["waves_per_simd"]
+        tuned = {"best": best, "sweep": table, "seconds": time.perf_counter() - t0}
+        print("# autotune: %s (%.1f s)" % (best, tuned["seconds"]), file=sys.stderr, flush=True)
+    elif args.tune:
         best, rows = tune(step, [8, 16, 32], [2, 4, 6, 8])
         k, w = best[0], best[1]
         tuned = {"best": {"block_k": k, "waves_per_simd": w, "ms": best[2]},
@@ -230,8 +279,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded zoom-walk view)" if args.config == "c2" else "synthetic (fixed view)",
-        "config": dict(desc, parallelism="single GPU", block_k=s["block_k"], waves_per_simd=s["waves_per_simd"]),
+        "data": "synthetic (fixed view)" if args.config == "c1" else "synthetic (seeded zoom-walk view)",
+        "config": dict(desc, parallelism="single GPU", block_k=s["block_k"], waves_per_simd=s["waves_per_simd"],
+                       tile=s["tile"], graph=s["graph"]),
         "roofline": roofline(args.config, flop, s),
         "detail": {"credited_px_it": credited, "executed_px_it": s["executed"],
                    "executed_in_start_kernel": s["executed_start"], "skipped": s["skipped"],
