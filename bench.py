@@ -223,6 +223,8 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--block-k", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--pre-iter", type=int, default=0, help="plain path: updates in the start kernel")
+    ap.add_argument("--tile", type=int, default=0, help="sample order: 0 row-major, T = T x T tiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true", help="sweep block size and waves per SIMD first")
     ap.add_argument("--cpu-rows-every", type=int, default=36)
@@ -242,6 +244,8 @@ def main():
     print("# precompute %.2f s (host HP: findProbe + series), len(X) = %s" % (t_pre, m.reference_length()),
           file=sys.stderr, flush=True)
     k, w = args.block_k, args.waves
+    if args.config != "c4":
+        m.set_launch(tile=args.tile, pre_iter=args.pre_iter)
     tuned = None
     if args.config == "c4":
         # the project's launch autotuner (sample order x waves x block size), then
@@ -281,7 +285,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (fixed view)" if args.config == "c1" else "synthetic (seeded zoom-walk view)",
         "config": dict(desc, parallelism="single GPU", block_k=s["block_k"], waves_per_simd=s["waves_per_simd"],
-                       tile=s["tile"], graph=s["graph"]),
+                       tile=s["tile"], graph=s["graph"], pre_iter=m.get_launch()["pre_iter"]),
         "roofline": roofline(args.config, flop, s),
         "detail": {"credited_px_it": credited, "executed_px_it": s["executed"],
                    "executed_in_start_kernel": s["executed_start"], "skipped": s["skipped"],

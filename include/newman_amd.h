@@ -138,7 +138,7 @@ int nm_render(nm_mandelbrot* h, int block_k, int waves_per_simd, int repeats, in
 int nm_render_frames(nm_mandelbrot* h, const char* const* widths, int nframes, int block_k, int waves_per_simd,
                      int repeats, nm_escape_value* out, nm_render_stats* stats);
 /* extension: launch settings of later renders on h.  0 keeps the engine's
- * default; tile = sample order (0 row-major, or T x T tiles of 8 x 8 blocks, T in
+ * default; pre_iter must be 0 or 1..1024; tile = sample order (0 row-major, or T x T tiles of 8 x 8 blocks, T in
  * 8..256, a power of two); graph = 1 replays repeated passes from one captured
  * HIP graph; ms = the kernel time the autotuner measured (output only). */
 typedef struct nm_launch {
@@ -146,6 +146,8 @@ typedef struct nm_launch {
   int32_t waves_per_simd;
   int32_t tile;
   int32_t graph;
+  int32_t pre_iter; /* plain path: updates tried in the start kernel (0 = default 32) */
+  int32_t reserved;
   double ms;
 } nm_launch;
 int nm_set_launch(nm_mandelbrot* h, const nm_launch* l);

@@ -69,6 +69,9 @@ struct RenderOpts {
   int repeats = 1;        // timing: launch the whole frame set this many times
   int tile = 0;           // sample order: 0 row-major, T = T x T tiles of 8 x 8 blocks (8..256)
   bool graph = false;     // replay repeats 2.. from one captured HIP graph
+  int pre_iter = 0;       // plain path: updates in the start kernel (0 = default)
+  uint64_t set_serial = 0; // identifies the frame set: the engine keeps its tables
+                           // on the device and skips the upload when it repeats (0 = always upload)
 };
 
 struct RenderStats {

@@ -101,8 +101,7 @@ struct Common {
 
 struct PlainArgs {
   Common c;
-  uint32_t nrows, ncols;  // frame size
-  double inv_ncols;
+  int pre_iter;  // updates tried in the start kernel before a sample is handed on
   const double* cre;
   const double* crelo;
   const double* cim;
@@ -237,6 +236,32 @@ __device__ bool dd_in_cardioid(double xh, double xl, double yh, double yl) {
   return dd_lt(lhs2, dd{0.0625, 0.0});
 }
 
+// The same test in double with a forward error bound: +1 / -1 when the sign of
+// each polynomial is certain (the remainders xl, yl of the coordinates enter
+// through the gradient), 0 when double-double must decide.  With u = 2^-53 and
+// at most 7 roundings per polynomial, |computed - exact| <= 8 u M, where M sums
+// the magnitudes of the terms; the factor 32 leaves a wide margin.
+__device__ __forceinline__ int cardioid_filter(double x, double xl, double y, double yl) {
+  const double u = 1.1102230246251565e-16;
+  const double xm = x - 0.25, y2 = y * y;
+  const double q = xm * xm + y2;
+  const double t = q + xm;
+  const double f = q * t - 0.25 * y2;
+  const double m1 = fabs(q) * (fabs(q) + fabs(xm)) + 0.25 * y2;
+  const double dx = 2.0 * fabs(xm) * (2.0 * fabs(q) + fabs(xm)) + fabs(q);
+  const double dy = 2.0 * fabs(y) * (2.0 * fabs(q) + fabs(xm)) + 0.5 * fabs(y);
+  const double b1 = 32.0 * u * m1 + 2.0 * (fabs(xl) * dx + fabs(yl) * dy);
+  if (f < -b1) return 1;
+  if (!(f > b1)) return 0;
+  const double xp = x + 1.0;
+  const double g = xp * xp + y2 - 0.0625;
+  const double m2 = xp * xp + y2 + 0.0625;
+  const double b2 = 32.0 * u * m2 + 2.0 * (fabs(xl) * 2.0 * fabs(xp) + fabs(yl) * 2.0 * fabs(y));
+  if (g < -b2) return 1;
+  if (g > b2) return -1;
+  return 0;
+}
+
 // descend(X[k] + d) with X[k] = hi + lo: round-to-nearest sum, then one step
 // toward zero when the exact residual has the opposite sign.
 __device__ __forceinline__ double trunc_sum3(double hi, double lo, double d) {
@@ -271,14 +296,14 @@ __device__ __forceinline__ void tile_count_store(uint32_t* tile_count, uint32_t 
 // test, then up to kPreIter updates per sample with the exact escape test (most
 // samples of a shallow view escape here, converged); the rest are flagged for
 // the persistent kernel with their state (z, c, updates done).
-constexpr int kPreIter = 64;
+constexpr int kPreIterDefault = 32;
 
 __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   const Common& c = a.c;
   unsigned long long ncard = 0, nres = 0, executed = 0;
   const uint32_t ntiles = (c.total + kTile - 1) / kTile;
   const int N = c.N;
-  const int pre = min(kPreIter, N);
+  const int pre = min(a.pre_iter, N);
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t mine = 0;
     for (int j = 0; j < kPerThread; j++) {
@@ -294,8 +319,11 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
       bool in = false;
       const double cx = a.cre[f.tab_c + col], cy = a.cim[f.tab_r + r];
       if (f.card_mode == CARD_ALL) in = true;
-      else if (f.card_mode == CARD_TEST_DD)
-        in = dd_in_cardioid(cx, a.crelo[f.tab_c + col], cy, a.cimlo[f.tab_r + r]);
+      else if (f.card_mode == CARD_TEST_DD) {
+        const double xl = a.crelo[f.tab_c + col], yl = a.cimlo[f.tab_r + r];
+        const int fc = cardioid_filter(cx, xl, cy, yl);
+        in = fc > 0 || (fc == 0 && dd_in_cardioid(cx, xl, cy, yl));
+      }
       bool want = false;
       if (in) {
         c.out[gid] = EscapeValue{N, 0.0f};
@@ -337,28 +365,42 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
 }
 
 // ============================================================ compaction
-// exclusive scan of tile_count (in place) by one workgroup; total -> *nactive
+// exclusive scan of tile_count (in place) by one workgroup; total -> *nactive.
+// Rounds of 4096 consecutive counts: 4 per thread, wave scans, a carry between
+// rounds (coalesced loads and stores).
 __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint32_t ntiles, uint32_t* nactive) {
-  __shared__ uint32_t sums[1024];
-  const uint32_t per = (ntiles + 1023) / 1024;
-  const uint32_t b = threadIdx.x * per, e = min(b + per, ntiles);
-  uint32_t s = 0;
-  for (uint32_t i = b; i < e; i++) s += tile_count[i];
-  sums[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-    uint32_t v = threadIdx.x >= (unsigned)off ? sums[threadIdx.x - off] : 0u;
+  __shared__ uint32_t wsum[16];
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < ntiles; base += 4096) {
+    const uint32_t i0 = base + 4 * threadIdx.x;
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = i0 + k < ntiles ? tile_count[i0 + k] : 0u;
+    const uint32_t mine = v[0] + v[1] + v[2] + v[3];
+    uint32_t x = mine;  // inclusive wave scan of the thread totals
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    sums[threadIdx.x] += v;
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < 16; w++) {
+      const uint32_t t = wsum[w];
+      before += w < wave ? t : 0u;
+      all += t;
+    }
+    uint32_t run = carry + before + x - mine;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (i0 + k < ntiles) tile_count[i0 + k] = run;
+      run += v[k];
+    }
+    carry += all;
     __syncthreads();
   }
-  uint32_t run = sums[threadIdx.x] - s;
-  for (uint32_t i = b; i < e; i++) {
-    uint32_t t = tile_count[i];
-    tile_count[i] = run;
-    run += t;
-  }
-  if (threadIdx.x == 1023) *nactive = sums[1023];
+  if (threadIdx.x == 0) *nactive = carry;
 }
 
 // flagged sample ids -> active[] in sample order
@@ -1067,9 +1109,22 @@ class EngineImpl : public Engine {
     return NM_OK;
   }
 
+  // frame tables on the device belong to (set serial, sample order); a render
+  // with the same pair and unmoved buffers reuses them
+  uint64_t tab_serial = 0;
+  int tab_tile = -1;
+  bool tables_stale(const RenderOpts& opt, bool same_buffers) const {
+    return opt.set_serial == 0 || opt.set_serial != tab_serial || opt.tile != tab_tile || !same_buffers;
+  }
+  void tables_uploaded(const RenderOpts& opt) {
+    tab_serial = opt.set_serial;
+    tab_tile = opt.tile;
+  }
+
   int grid_for_tiles(uint64_t total) const {
     uint64_t ntiles = (total + kTile - 1) / kTile;
-    return (int)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, (uint64_t)num_cu * 8));
+    const uint64_t cap = std::min<uint64_t>((uint64_t)num_cu * 16, (uint64_t)(kSlotIter - kSlotStart));
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, cap));
   }
 
   // buffers and the common kernel arguments for `total` samples
@@ -1289,8 +1344,10 @@ class EngineImpl : public Engine {
       df.push_back(d);
     }
     int st;
+    const void *pf = d_frames, *pt = d_tabs;
     if ((st = grow(&d_frames, &n_frames_bytes, df.size() * sizeof(DevFrame)))) return st;
     if ((st = grow(&d_tabs, &n_tabs_bytes, (2 * ncol + 2 * nrow) * sizeof(double)))) return st;
+    const bool upload = tables_stale(opt, pf == d_frames && pt == d_tabs);
     PlainArgs a;
     if ((st = prepare_common(a.c, total, opt))) return st;
     EscapeValue* emit_to = nullptr;
@@ -1302,17 +1359,21 @@ class EngineImpl : public Engine {
     a.c.frame_size = (uint32_t)fs;
     a.c.inv_frame = 1.0 / (double)a.c.frame_size;
     double* t = (double*)d_tabs;
-    NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
-    NM_HIP(hipMemcpyAsync(t, cre.data(), ncol * 8, hipMemcpyHostToDevice, stream));
-    NM_HIP(hipMemcpyAsync(t + ncol, crelo.data(), ncol * 8, hipMemcpyHostToDevice, stream));
-    NM_HIP(hipMemcpyAsync(t + 2 * ncol, cim.data(), nrow * 8, hipMemcpyHostToDevice, stream));
-    NM_HIP(hipMemcpyAsync(t + 2 * ncol + nrow, cimlo.data(), nrow * 8, hipMemcpyHostToDevice, stream));
+    if (upload) {
+      NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
+      NM_HIP(hipMemcpyAsync(t, cre.data(), ncol * 8, hipMemcpyHostToDevice, stream));
+      NM_HIP(hipMemcpyAsync(t + ncol, crelo.data(), ncol * 8, hipMemcpyHostToDevice, stream));
+      NM_HIP(hipMemcpyAsync(t + 2 * ncol, cim.data(), nrow * 8, hipMemcpyHostToDevice, stream));
+      NM_HIP(hipMemcpyAsync(t + 2 * ncol + nrow, cimlo.data(), nrow * 8, hipMemcpyHostToDevice, stream));
+      tables_uploaded(opt);
+    }
+    a.pre_iter = opt.pre_iter > 0 ? opt.pre_iter : kPreIterDefault;
     a.cre = t;
     a.crelo = t + ncol;
     a.cim = t + 2 * ncol;
     a.cimlo = t + 2 * ncol + nrow;
-    const int K = opt.block_k > 0 ? opt.block_k : 32;
-    const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 4;
+    const int K = opt.block_k > 0 ? opt.block_k : 16;
+    const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
     if (stats) {
       stats->block_k = K;
       stats->waves_per_simd = waves;
@@ -1362,9 +1423,11 @@ class EngineImpl : public Engine {
       df.push_back(d);
     }
     int st;
+    const void *pf = d_frames, *pt = d_tabs, *pm = d_mask;
     if ((st = grow(&d_frames, &n_frames_bytes, df.size() * sizeof(DevFrame)))) return st;
     if ((st = grow(&d_tabs, &n_tabs_bytes, (ncol + nrow) * sizeof(double)))) return st;
     if ((st = grow(&d_mask, &n_mask_bytes, std::max<size_t>(mask.size(), 1) * 4))) return st;
+    const bool upload = tables_stale(opt, pf == d_frames && pt == d_tabs && pm == d_mask);
     SeriesArgs a;
     if ((st = prepare_common(a.c, total, opt))) return st;
     EscapeValue* emit_to = nullptr;
@@ -1376,11 +1439,14 @@ class EngineImpl : public Engine {
     a.c.frame_size = (uint32_t)fs;
     a.c.inv_frame = 1.0 / (double)a.c.frame_size;
     double* t = (double*)d_tabs;
-    NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
-    NM_HIP(hipMemcpyAsync(t, er.data(), ncol * 8, hipMemcpyHostToDevice, stream));
-    NM_HIP(hipMemcpyAsync(t + ncol, ei.data(), nrow * 8, hipMemcpyHostToDevice, stream));
-    if (!mask.empty())
-      NM_HIP(hipMemcpyAsync(d_mask, mask.data(), mask.size() * 4, hipMemcpyHostToDevice, stream));
+    if (upload) {
+      NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
+      NM_HIP(hipMemcpyAsync(t, er.data(), ncol * 8, hipMemcpyHostToDevice, stream));
+      NM_HIP(hipMemcpyAsync(t + ncol, ei.data(), nrow * 8, hipMemcpyHostToDevice, stream));
+      if (!mask.empty())
+        NM_HIP(hipMemcpyAsync(d_mask, mask.data(), mask.size() * 4, hipMemcpyHostToDevice, stream));
+      tables_uploaded(opt);
+    }
     a.epsre = t;
     a.epsim = t + ncol;
     a.mask = (const uint32_t*)d_mask;

@@ -418,7 +418,7 @@ int Mandelbrot::uploadReference() {
   buildThresholds();
   have_ref_ = true;
   rendered_ = false;
-  ref_serial_ = g_ref_serial.fetch_add(1);
+  ref_serial_ = next_serial();
   return NM_OK;
 }
 
@@ -562,30 +562,38 @@ void Mandelbrot::applyLaunch(RenderOpts& o) const {
   if (o.waves_per_simd == 0) o.waves_per_simd = launch.waves_per_simd;
   o.tile = launch.tile;
   o.graph = launch.graph != 0;
+  if (o.pre_iter == 0) o.pre_iter = launch.pre_iter;
 }
 
 int Mandelbrot::render(RenderOpts* opts, bool copy_back) {
   if (N <= 0 || rows() <= 0 || cols() <= 0) return NM_ERR_ARG;
-  Engine* eng = nullptr;
-  int st = Engine::instance(&eng);
-  if (st != NM_OK) return st;
   RenderOpts o = opts ? *opts : RenderOpts();
   applyLaunch(o);
   EscapeValue* out = copy_back ? grid.values.data() : nullptr;
-  if (useHardware()) {
-    HwFrameHost f;
-    buildHwFrame(f);
-    st = eng->render_hw({&f}, o, out, &stats_);
-  } else {
+  const bool hw = useHardware();
+  if (!hw) {
     if (!have_ref_) return NM_ERR_STATE;  // precompute() first (mandelbrot.cpp:161)
     if (std::fabs(o.tol - error_tolerance) > 0) return NM_ERR_ARG;
-    st = eng->set_reference(ref_, ref_serial_);
-    if (st != NM_OK) return st;
-    SaFrameHost f;
-    st = buildSaFrame(f);
-    if (st != NM_OK) return st;
-    st = eng->render_sa({&f}, o, out, &stats_);
   }
+  const std::string key = snapshot() + "|view";
+  if (key != cache_key_) {
+    cache_key_.clear();
+    cache_ = FrameSet();
+    cache_.hw = hw;
+    int st = NM_OK;
+    if (hw) {
+      cache_.h.resize(1);
+      st = buildHwFrame(cache_.h[0]);
+    } else {
+      cache_.s.resize(1);
+      st = buildSaFrame(cache_.s[0]);
+    }
+    if (st != NM_OK) return st;
+    cache_key_ = key;
+    cache_serial_ = next_serial();
+  }
+  o.set_serial = cache_serial_;
+  int st = renderSet(cache_, o, out, 0, 1);
   if (st == NM_OK) {
     rendered_ = copy_back;
     snap_ = snapshot();
@@ -602,6 +610,9 @@ int Mandelbrot::computeRow(int r) {
 }
 
 int Mandelbrot::computeAll() { return render(nullptr, true); }
+
+// process-wide serial numbers of reference orbits and frame sets (device caches)
+uint64_t Mandelbrot::next_serial() { return g_ref_serial.fetch_add(1); }
 
 // A zoom-video batch: every frame shares this object's centre, reference orbit
 // and grid size; frame k has sz = width_k / cols (square samples).  All frames go
@@ -656,10 +667,18 @@ int Mandelbrot::renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue*
 int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts* opts, EscapeValue* out) {
   RenderOpts o = opts ? *opts : RenderOpts();
   applyLaunch(o);
-  FrameSet set;
-  int st = buildFrames(widths, set);
-  if (st != NM_OK) return st;
-  return renderSet(set, o, out, 0, 1);
+  std::string key = snapshot() + "|frames";
+  for (const auto& w : widths) key += "|" + w;
+  if (key != cache_key_) {
+    cache_key_.clear();
+    cache_ = FrameSet();
+    int st = buildFrames(widths, cache_);
+    if (st != NM_OK) return st;
+    cache_key_ = key;
+    cache_serial_ = next_serial();
+  }
+  o.set_serial = cache_serial_;
+  return renderSet(cache_, o, out, 0, 1);
 }
 
 // The project's launch autotuner.  Candidates: sample order (row-major, 32x32,
@@ -681,6 +700,7 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
   if (table) table->clear();
   RenderStats keep = stats_;
   const int* ks = set.hw ? ks_hw : ks_sa;
+  const uint64_t subset_serial = next_serial();  // the subsample's tables stay on the device
   for (int tile : tiles)
     for (int w : waves)
       for (int ki = 0; ki < 2; ki++) {
@@ -692,6 +712,7 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
         o.tile = tile;
         o.graph = false;
         o.repeats = 1;
+        o.set_serial = subset_serial;
         if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
         if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
         Launch l;
@@ -699,6 +720,7 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
         l.waves_per_simd = w;
         l.tile = tile;
         l.graph = launch.graph;
+        l.pre_iter = launch.pre_iter;
         l.ms = stats_.ms_total;
         if (table) table->push_back(l);
         if (l.ms < best.ms) best = l;

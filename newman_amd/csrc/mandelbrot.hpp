@@ -42,7 +42,7 @@ class Mandelbrot {
   // per block, persistent waves per SIMD, sample order (0 row-major, T = T x T
   // tiles) and HIP-graph replay of repeated passes.  autotuneFrames sets them.
   struct Launch {
-    int block_k = 0, waves_per_simd = 0, tile = 0, graph = 0;
+    int block_k = 0, waves_per_simd = 0, tile = 0, graph = 0, pre_iter = 0;
     double ms = 0.0;
   };
   Launch launch;
@@ -98,6 +98,11 @@ class Mandelbrot {
   int buildFrames(const std::vector<std::string>& widths, FrameSet& set);
   // frames first, first + step, ... of a built set, one launch set
   int renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue* out, size_t first, size_t step);
+  static uint64_t next_serial();
+  // the last frame set built, its key (view snapshot + widths) and serial
+  FrameSet cache_;
+  std::string cache_key_;
+  uint64_t cache_serial_ = 0;
   void applyLaunch(RenderOpts& o) const;
   int buildHwFrame(HwFrameHost& f) { return buildHwFrame(f, center, sz); }
   int buildSaFrame(SaFrameHost& f) { return buildSaFrame(f, center, sz); }

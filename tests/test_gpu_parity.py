@@ -71,6 +71,43 @@ def test_plain_path_symmetry(gpu):
         assert np.array_equal(g[r], g[198 - r])
 
 
+# points on the cardioid boundary c(t) = e^{it}/2 - e^{2it}/4 and the period-2
+# disk |c + 1| = 1/4, at widths where many samples sit within the double filter's
+# error bound of the boundary (the device then decides in double-double)
+_BOUNDARY = [("0.36552730612364814", "0.29715374784582538", "1e-3"),  # t = 1.2
+             ("-0.044662513057668218", "0.64384933723982285", "1e-9"),  # t = 2.0
+             ("-1.0", "0.25", "1e-13"), ("-1.1767766952966369", "0.1767766952966369", "1e-11")]
+
+
+@pytest.mark.parametrize("cre,cim,width", _BOUNDARY)
+def test_plain_path_cardioid_boundary_frames(gpu, cre, cim, width):
+    """Frames straddling the cardioid / period-2 disk boundary: the device's
+    double filter plus double-double test equals the HP test (mandelbrot.cpp:63-71)
+    on every sample."""
+    m, o = _pair(64, 96, 400, cre, cim, width)
+    assert m.useHardware()
+    st = m.render()
+    g = m.grid()
+    ref = o.render_rows(np.arange(64))
+    card = np.array([[o.in_cardioid(r, c) for c in range(96)] for r in range(64)])
+    assert 0 < card.sum() < card.size  # the boundary crosses the frame
+    assert st["n_cardioid"] == card.sum()
+    assert np.array_equal(g["iterations"], ref["iterations"])
+    assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
+
+
+@pytest.mark.parametrize("pre", [1, 16, 200])
+def test_plain_path_pre_iteration_depth(gpu, pre):
+    """The start kernel's pre-iteration depth only moves work between kernels."""
+    m, o = _pair(120, 160, 700, "-0.75", "0.05", "2.9")
+    m.set_launch(pre_iter=pre, block_k=16)
+    m.render()
+    ref = o.render_rows(np.arange(120))
+    g = m.grid()
+    assert np.array_equal(g["iterations"], ref["iterations"])
+    assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
+
+
 def test_plain_path_compute_row_dropin(gpu):
     """computeRow(r) for ascending rows (viewer.cpp:166-171) then at(r, c)."""
     import newman_amd as nm

@@ -84,3 +84,27 @@ def test_autotuner_picks_a_measured_setting(gpu):
     assert (st["block_k"], st["waves_per_simd"], st["tile"]) == (best["block_k"], best["waves_per_simd"],
                                                                   best["tile"])
     assert np.array_equal(got, ref)
+
+
+def test_device_frame_tables_follow_the_view(gpu):
+    """The engine keeps a frame set's tables on the device between renders of the
+    same set; any change of view, widths, sample order or object re-uploads."""
+    nm = gpu
+    m, widths = _video(nm)
+    a, _ = m.render_frames(widths)
+    b, _ = m.render_frames(widths[:4])
+    assert np.array_equal(m.render_frames(widths)[0], a)
+    m.set_launch(tile=64)
+    assert np.array_equal(m.render_frames(widths)[0], a)
+    m.set_launch(tile=0)
+    p = nm.Mandelbrot(45, 70)  # another object (its own frame set) in between
+    p.set_view(300, "-0.75", "0", "3.0")
+    p.render()
+    assert np.array_equal(m.render_frames(widths[:4])[0], b)
+    m.render()  # the single-view set of the same object
+    g1 = m.grid().copy()
+    m.N = m.N - 1  # changes the view snapshot
+    m.render()
+    m.N = m.N + 1
+    m.render()
+    assert np.array_equal(m.grid(), g1)
