@@ -972,6 +972,25 @@ __global__ void k_downsample(const EscapeValue* fine, int nr, int nc, int sc, Es
 }
 
 // sum of reported iterations over the output (credited work)
+// partial slots -> one total per counter (one workgroup)
+__global__ __launch_bounds__(kThreads) void k_reduce_partials(const unsigned long long* part, int nslots,
+                                                              unsigned long long* totals) {
+  __shared__ unsigned long long acc[kThreads / 64];
+  for (int k = 0; k < C_COUNT; k++) {
+    unsigned long long v = 0;
+    for (int sl = threadIdx.x; sl < nslots; sl += blockDim.x) v += part[(size_t)sl * C_COUNT + k];
+    v = wave_sum(v);
+    if (lane_id() == 0) acc[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int w = 0; w < kThreads / 64; w++) t += acc[w];
+      totals[k] = t;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_sum_iterations(const EscapeValue* out, unsigned long long n,
                                                              unsigned long long* part) {
   unsigned long long s = 0;
@@ -996,7 +1015,8 @@ __global__ void k_zero(unsigned long long* p, int n) {
 class EngineImpl : public Engine {
  public:
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {};  // stage boundaries of the eager pass
+  std::vector<hipEvent_t> evpool;  // stage boundaries of timed passes (grow-only)
+  unsigned long long* d_totals = nullptr;  // C_COUNT counter totals
   int device = 0;
   int num_cu = 256;
   std::string err;
@@ -1016,7 +1036,6 @@ class EngineImpl : public Engine {
   void* d_emit = nullptr; size_t n_emit_bytes = 0;  // row-major grids of a tiled sample order
   unsigned long long* d_part = nullptr;               // kPartSlots x C_COUNT
   uint32_t* d_nactive = nullptr;
-  std::vector<unsigned long long> h_part;
   uint64_t ref_serial = 0;
   // reference layout (offsets into d_ref)
   int ref_len = 0;
@@ -1041,10 +1060,9 @@ class EngineImpl : public Engine {
     NM_HIP(hipGetDeviceProperties(&prop, device));
     num_cu = prop.multiProcessorCount;
     NM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    for (auto& e : ev) NM_HIP(hipEventCreate(&e));
+    NM_HIP(hipMalloc((void**)&d_totals, sizeof(unsigned long long) * C_COUNT));
     NM_HIP(hipMalloc((void**)&d_part, sizeof(unsigned long long) * kPartSlots * C_COUNT));
     NM_HIP(hipMalloc((void**)&d_nactive, 256));
-    h_part.resize((size_t)kPartSlots * C_COUNT);
     return NM_OK;
   }
 
@@ -1192,26 +1210,33 @@ class EngineImpl : public Engine {
     return NM_OK;
   }
 
-  // The pass `repeats` times; then counters and output.  The first pass runs
-  // eagerly with per-stage events; with opt.graph the pass is captured once into
-  // a HIP graph and the other repeats replay it (timed as a whole).
+  int ensure_events(size_t n) {
+    while (evpool.size() < n) {
+      hipEvent_t e = nullptr;
+      NM_HIP(hipEventCreate(&e));
+      evpool.push_back(e);
+    }
+    return NM_OK;
+  }
+
+  // The pass `repeats` times back to back (one host synchronisation at the end);
+  // then counters and output.  Without opt.graph every pass runs eagerly with
+  // per-stage events; with it the first pass runs eagerly, the pass is captured
+  // once into a HIP graph and the other repeats replay it (timed as a whole).
   template <class Args, class StartFn, class IterFn>
   int run(Args& args, StartFn start, IterFn iter, int waves, const RenderOpts& opt, RenderStats* stats,
           uint64_t nout, EscapeValue* host_out, EscapeValue* emit_to) {
     const int repeats = std::max(1, opt.repeats);
-    double ms[4] = {};
-    double ms_graph = 0.0;
+    const bool use_graph = opt.graph && repeats > 1;
     int st;
-    if ((st = launch_pass(args, start, iter, waves, emit_to, ev))) return st;
-    NM_HIP(hipEventSynchronize(ev[4]));
-    for (int k = 0; k < 4; k++) {
-      float t = 0;
-      NM_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-      ms[k] = t;
-    }
-    if (repeats > 1 && opt.graph) {
-      hipGraph_t graph = nullptr;
-      hipGraphExec_t exec = nullptr;
+    if ((st = ensure_events(use_graph ? 5 + 2 * (size_t)repeats : 5 * (size_t)repeats))) return st;
+    hipEvent_t* ev = evpool.data();
+    const int eager = use_graph ? 1 : repeats;
+    for (int rep = 0; rep < eager; rep++)
+      if ((st = launch_pass(args, start, iter, waves, emit_to, ev + 5 * rep))) return st;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
       NM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       st = launch_pass(args, start, iter, waves, emit_to, nullptr);
       hipError_t ce = hipStreamEndCapture(stream, &graph);
@@ -1226,44 +1251,45 @@ class EngineImpl : public Engine {
         return fail("hipGraphInstantiate", ie);
       }
       for (int rep = 1; rep < repeats && st == NM_OK; rep++) {
-        hipError_t e1 = hipEventRecord(ev[0], stream);
+        hipEvent_t* e = ev + 5 + 2 * (rep - 1);
+        hipError_t e1 = hipEventRecord(e[0], stream);
         hipError_t e2 = e1 == hipSuccess ? hipGraphLaunch(exec, stream) : e1;
-        hipError_t e3 = e2 == hipSuccess ? hipEventRecord(ev[4], stream) : e2;
-        hipError_t e4 = e3 == hipSuccess ? hipEventSynchronize(ev[4]) : e3;
+        hipError_t e3 = e2 == hipSuccess ? hipEventRecord(e[1], stream) : e2;
+        if (e3 != hipSuccess) st = fail("graph replay", e3);
+      }
+    }
+    hipError_t se = hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (st) return st;
+    if (se != hipSuccess) return fail("hipStreamSynchronize", se);
+    double ms[4] = {};
+    for (int rep = 0; rep < eager; rep++)
+      for (int k = 0; k < 4; k++) {
         float t = 0;
-        hipError_t e5 = e4 == hipSuccess ? hipEventElapsedTime(&t, ev[0], ev[4]) : e4;
-        if (e5 != hipSuccess) st = fail("graph replay", e5);
-        ms_graph += t;
+        NM_HIP(hipEventElapsedTime(&t, ev[5 * rep + k], ev[5 * rep + k + 1]));
+        ms[k] += t / eager;
       }
-      (void)hipGraphExecDestroy(exec);
-      (void)hipGraphDestroy(graph);
-      if (st) return st;
-      ms_graph /= (repeats - 1);
-    } else {
-      for (int rep = 1; rep < repeats; rep++) {
-        if ((st = launch_pass(args, start, iter, waves, emit_to, ev))) return st;
-        NM_HIP(hipEventSynchronize(ev[4]));
-        for (int k = 0; k < 4; k++) {
-          float t = 0;
-          NM_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-          ms[k] += t;
-        }
-      }
-      for (double& v : ms) v /= repeats;
+    double ms_graph = 0.0;
+    for (int rep = 1; use_graph && rep < repeats; rep++) {
+      float t = 0;
+      NM_HIP(hipEventElapsedTime(&t, ev[5 + 2 * (rep - 1)], ev[5 + 2 * (rep - 1) + 1]));
+      ms_graph += t / (repeats - 1);
     }
     const EscapeValue* result = emit_to ? emit_to : (const EscapeValue*)d_out;
     hipLaunchKernelGGL(k_sum_iterations, dim3(std::min(num_cu * 4, kPartSlots - kSlotSum)), dim3(kThreads), 0,
                        stream, result, (unsigned long long)nout, d_part);
     NM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, stream, (const unsigned long long*)d_part,
+                       kPartSlots, d_totals);
+    NM_HIP(hipGetLastError());
     uint32_t nact = 0;
-    NM_HIP(hipMemcpyAsync(h_part.data(), d_part, h_part.size() * 8, hipMemcpyDeviceToHost, stream));
+    unsigned long long h[C_COUNT] = {};
+    NM_HIP(hipMemcpyAsync(h, d_totals, sizeof h, hipMemcpyDeviceToHost, stream));
     NM_HIP(hipMemcpyAsync(&nact, d_nactive, 4, hipMemcpyDeviceToHost, stream));
     if (host_out)
       NM_HIP(hipMemcpyAsync(host_out, result, nout * sizeof(EscapeValue), hipMemcpyDeviceToHost, stream));
     NM_HIP(hipStreamSynchronize(stream));
-    unsigned long long h[C_COUNT] = {};
-    for (int s = 0; s < kPartSlots; s++)
-      for (int k = 0; k < C_COUNT; k++) h[k] += h_part[(size_t)s * C_COUNT + k];
     if (stats) {
       stats->samples = nout;
       stats->n_cardioid = h[C_CARDIOID];
@@ -1278,8 +1304,8 @@ class EngineImpl : public Engine {
       stats->ms_compact = ms[1];
       stats->ms_iterate = ms[2];
       stats->ms_finalize = ms[3];
-      stats->ms_total = ms_graph > 0.0 ? ms_graph : ms[0] + ms[1] + ms[2] + ms[3];
-      stats->graph = ms_graph > 0.0 ? 1 : 0;
+      stats->ms_total = use_graph ? ms_graph : ms[0] + ms[1] + ms[2] + ms[3];
+      stats->graph = use_graph ? 1 : 0;
       stats->tile = emit_to ? opt.tile : 0;
     }
     return NM_OK;
