@@ -1236,7 +1236,9 @@ int orc_zoom_walk(uint64_t seed, int N, const char* target_width, int res, int n
   ohp_set_str(&tgt, target_width);
   uint64_t state = seed;
   int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)res * res);
-  int k = 0;
+  int k = 0, retries = 0;
+  ohp pre, pim, pw; /* the previous step's centre and width (back-off) */
+  pre.n = pim.n = pw.n = nmax;
   for (;;) {
     ohp w10;
     w10.n = nmax;
@@ -1262,7 +1264,17 @@ int orc_zoom_walk(uint64_t seed, int N, const char* target_width, int res, int n
       if (counts[i] < N && counts[i] > maxesc) maxesc = counts[i];
     }
     if (ncand == 0) {
-      if (maxesc < 0) break; /* nothing escaped: keep the centre */
+      if (maxesc < 0) {
+        /* nothing escaped at this depth: back off one step and draw another
+         * candidate there (the draws continue the same splitmix64 stream) */
+        if (k == 0 || retries >= 32) break;
+        ohp_copy(&cre, &pre);
+        ohp_copy(&cim, &pim);
+        ohp_copy(&w, &pw);
+        k--;
+        retries++;
+        continue;
+      }
       thr = (int)ceil(0.9 * maxesc);
     }
     /* candidates on a band edge (a 4-neighbour has another count) keep the walk
@@ -1301,6 +1313,9 @@ int orc_zoom_walk(uint64_t seed, int N, const char* target_width, int res, int n
       }
     free(edge);
     int r = (int)(sel / res), c = (int)(sel % res);
+    ohp_copy(&pre, &cre);
+    ohp_copy(&pim, &cim);
+    ohp_copy(&pw, &w);
     ohp_mul_si(&t, &sz, (long)(c - res / 2));
     ohp_add(&cre, &cre, &t);
     ohp_mul_si(&t, &sz, (long)(res / 2 - r - 1));

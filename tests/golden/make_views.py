@@ -5,7 +5,9 @@ The walk (oracle/oracle.c, orc_zoom_walk) starts from the reference's full view
 current width, and moves the centre to a uniformly random escaped sample with
 count >= 0.9 N (splitmix64(seed)), preferring samples next to a non-escaped sample
 (the set's boundary), then samples on a band edge; the width is divided by 10
-until the next width would fall below the target, which is then used as is.
+until the next width would fall below the target, which is then used as is.  A
+depth whose render has no escaped sample backs the walk off one step, where another
+candidate is drawn from the same splitmix64 stream.
 
 Usage: python tests/golden/make_views.py [name ...]   (default: all)
 """
@@ -31,7 +33,8 @@ SPECS = {
 ABOUT = ("Deep views fixed by the seeded zoom walk of tests/golden/make_views.py: from the full view "
          "(centre -0.5+0i, width 4) render res x res, move to a uniformly random escaped sample with "
          "count >= 0.9 N (splitmix64(seed)), preferring samples next to a non-escaped one, then band edges; "
-         "divide the width by 10 until the next width is below the target; use the target width.")
+         "divide the width by 10 until the next width is below the target; use the target width. A depth "
+         "with no escaped sample backs the walk off one step to draw another candidate.")
 
 
 def main(names):
