@@ -159,6 +159,58 @@ int nm_get_launch(const nm_mandelbrot* h, nm_launch* l);
  * to *n_table. */
 int nm_autotune_frames(nm_mandelbrot* h, const char* const* widths, int nframes, nm_launch* best,
                        nm_launch* table, int cap, int* n_table);
+/* ---- learned series skip (extension) ---------------------------------------
+ * A per-tile alternative to the reference's per-pixel series scan
+ * (mandelbrot.cpp:165-181): every sample of a T x T tile starts at the skip of the
+ * tile's probe, its sample farthest from the reference point (the one whose scan
+ * stops first).  The probe's first unstable index comes from the reference's scan,
+ * from the threshold tables, or from a small network (nm_skipnet: 132 inputs, two
+ * hidden layers of 64, ReLU, FP32) whose every prediction is validated by the
+ * reference's stability test, with the scan as fallback. */
+#define NM_SKIP_PIXEL 0
+#define NM_SKIP_TILE_SCAN 1
+#define NM_SKIP_TILE_TABLE 2
+#define NM_SKIP_TILE_LEARNED 3
+#define NM_SKIPNET_INPUTS 132
+#define NM_SKIPNET_PARAMS 12737
+typedef struct nm_skipnet nm_skipnet;
+typedef struct nm_tile_stats {
+  int32_t ntiles;
+  int32_t accepted;   /* predictions the check accepted */
+  int32_t rejected;   /* predictions rejected: the tile was scanned */
+  int32_t reserved;
+  uint64_t evals;     /* exact stability tests evaluated at the probes */
+  double ms_search;   /* tile kernel, ms */
+  double ms_predict;  /* features, network and snapping, ms (host wall time) */
+} nm_tile_stats;
+int nm_skipnet_create(nm_skipnet** out);
+void nm_skipnet_destroy(nm_skipnet* net);
+/* params: NM_SKIPNET_PARAMS floats (W1[64][132], b1, W2[64][64], b2, W3[64], b3);
+ * mean, scale: the input standardisation (NM_SKIPNET_INPUTS each) */
+int nm_skipnet_get(const nm_skipnet* net, float* params, float* mean, float* scale);
+int nm_skipnet_set(nm_skipnet* net, const float* params, const float* mean, const float* scale);
+/* trains on the GPU (HIP forward, backward and Adam kernels): standardises the
+ * inputs on this data, initialises from seed, minimises the mean squared error;
+ * curve (epochs floats, NULL ok) gets each epoch's first-batch loss */
+int nm_skipnet_train(nm_skipnet* net, const float* x, const float* t, int n, int epochs, int batch, float lr,
+                     uint64_t seed, float* curve);
+int nm_skipnet_predict(nm_skipnet* net, const float* x, int n, float* y);
+/* loss and gradient of a batch of standardised inputs, on the GPU */
+int nm_skipnet_gradient(nm_skipnet* net, const float* xn, const float* t, int n, float* grad, float* loss);
+/* the view's tile features (NM_SKIPNET_INPUTS per tile, row-major tiles) */
+int nm_tile_features(nm_mandelbrot* h, int tile, float* x, int cap_tiles, int* ntiles);
+/* first unstable index of every tile probe (0 = none): mode 0 scan, 1 tables,
+ * 2 learned (net required); how (NULL ok): 0 scan, 1 tables, 2 accepted, 3 rejected */
+int nm_tile_skips(nm_mandelbrot* h, int tile, int mode, nm_skipnet* net, int* ua, int* how, int cap_tiles,
+                  nm_tile_stats* stats);
+/* the network's training target of each first unstable index for this view: the
+ * rank of the index among the indices where the threshold prefix minimum drops,
+ * t = (rank + 1) / (R + 1), and 1 for 0 ("never unstable") */
+int nm_tile_targets(nm_mandelbrot* h, const int* ua, int n, float* t);
+/* skip granularity of later renders on the series path (NM_SKIP_*); the net is
+ * not owned and must outlive its use */
+int nm_set_skip_mode(nm_mandelbrot* h, int mode, int tile, nm_skipnet* net);
+int nm_get_tile_stats(const nm_mandelbrot* h, nm_tile_stats* stats);
 /* the text of the last engine error */
 const char* nm_last_error(void);
 

@@ -61,6 +61,69 @@ class Launch(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TileStats(C.Structure):
+    _fields_ = [("ntiles", C.c_int32), ("accepted", C.c_int32), ("rejected", C.c_int32), ("reserved", C.c_int32),
+                ("evals", C.c_uint64), ("ms_search", C.c_double), ("ms_predict", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+SKIP_PIXEL, SKIP_TILE_SCAN, SKIP_TILE_TABLE, SKIP_TILE_LEARNED = 0, 1, 2, 3
+SKIPNET_INPUTS, SKIPNET_PARAMS = 132, 12737
+
+
+class SkipNet:
+    """The learned series-skip predictor (132-64-64-1 ReLU MLP, trained on the GPU)."""
+
+    def __init__(self):
+        h = _H()
+        _check(lib().nm_skipnet_create(C.byref(h)), "nm_skipnet_create")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.nm_skipnet_destroy(self._h)
+            self._h = None
+
+    def get(self):
+        p = np.zeros(SKIPNET_PARAMS, dtype=np.float32)
+        mu = np.zeros(SKIPNET_INPUTS, dtype=np.float32)
+        sc = np.zeros(SKIPNET_INPUTS, dtype=np.float32)
+        _check(lib().nm_skipnet_get(self._h, p.ctypes.data, mu.ctypes.data, sc.ctypes.data), "nm_skipnet_get")
+        return p, mu, sc
+
+    def set(self, params, mean=None, scale=None):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        mu = None if mean is None else np.ascontiguousarray(mean, dtype=np.float32)
+        sc = None if scale is None else np.ascontiguousarray(scale, dtype=np.float32)
+        _check(lib().nm_skipnet_set(self._h, p.ctypes.data, None if mu is None else mu.ctypes.data,
+                                    None if sc is None else sc.ctypes.data), "nm_skipnet_set")
+
+    def train(self, x, t, epochs=200, batch=256, lr=3e-3, seed=1):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        curve = np.zeros(epochs, dtype=np.float32)
+        _check(lib().nm_skipnet_train(self._h, x.ctypes.data, t.ctypes.data, len(t), epochs, batch, lr, seed,
+                                      curve.ctypes.data), "nm_skipnet_train")
+        return curve
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(len(x), dtype=np.float32)
+        _check(lib().nm_skipnet_predict(self._h, x.ctypes.data, len(x), y.ctypes.data), "nm_skipnet_predict")
+        return y
+
+    def gradient(self, xn, t):
+        xn = np.ascontiguousarray(xn, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(SKIPNET_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().nm_skipnet_gradient(self._h, xn.ctypes.data, t.ctypes.data, len(t), g.ctypes.data,
+                                         C.byref(loss)), "nm_skipnet_gradient")
+        return loss.value, g
+
+
 _lib = None
 
 # (name, restype, argtypes) for every declaration of include/newman_amd.h
@@ -111,6 +174,19 @@ _SIGS = [
     ("nm_get_launch", C.c_int, [_H, C.c_void_p]),
     ("nm_autotune_frames", C.c_int, [_H, C.POINTER(C.c_char_p), C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                      C.POINTER(C.c_int)]),
+    ("nm_skipnet_create", C.c_int, [C.POINTER(_H)]),
+    ("nm_skipnet_destroy", None, [_H]),
+    ("nm_skipnet_get", C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("nm_skipnet_set", C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("nm_skipnet_train", C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                   C.c_void_p]),
+    ("nm_skipnet_predict", C.c_int, [_H, C.c_void_p, C.c_int, C.c_void_p]),
+    ("nm_skipnet_gradient", C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    ("nm_tile_features", C.c_int, [_H, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    ("nm_tile_skips", C.c_int, [_H, C.c_int, C.c_int, _H, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    ("nm_tile_targets", C.c_int, [_H, C.c_void_p, C.c_int, C.c_void_p]),
+    ("nm_set_skip_mode", C.c_int, [_H, C.c_int, C.c_int, _H]),
+    ("nm_get_tile_stats", C.c_int, [_H, C.c_void_p]),
     ("nm_last_error", C.c_char_p, []),
     ("nm_hp_op", C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_long, C.c_double, C.c_long,
                            C.POINTER(C.c_int), C.POINTER(C.c_long), C.c_void_p]),
@@ -325,6 +401,38 @@ class Mandelbrot:
         st = RenderStats()
         _check(lib().nm_render(self._h, block_k, waves_per_simd, repeats, 1 if copy_back else 0, C.byref(st)),
                "nm_render")
+        return st.as_dict()
+
+    def tile_features(self, tile=64):
+        n = C.c_int(0)
+        _check(lib().nm_tile_features(self._h, tile, None, 0, C.byref(n)), "nm_tile_features")
+        x = np.zeros((n.value, SKIPNET_INPUTS), dtype=np.float32)
+        _check(lib().nm_tile_features(self._h, tile, x.ctypes.data, n.value, C.byref(n)), "nm_tile_features")
+        return x
+
+    def tile_skips(self, tile=64, mode=0, net=None):
+        """First unstable index per tile probe: mode 0 scan, 1 tables, 2 learned."""
+        nt = ((self.rows() + tile - 1) // tile) * ((self.cols() + tile - 1) // tile)
+        ua = np.zeros(nt, dtype=np.int32)
+        how = np.zeros(nt, dtype=np.int32)
+        st = TileStats()
+        _check(lib().nm_tile_skips(self._h, tile, mode, net._h if net else None, ua.ctypes.data, how.ctypes.data, nt,
+                                   C.byref(st)), "nm_tile_skips")
+        return ua, how, st.as_dict()
+
+    def tile_targets(self, ua):
+        ua = np.ascontiguousarray(ua, dtype=np.int32)
+        t = np.zeros(len(ua), dtype=np.float32)
+        _check(lib().nm_tile_targets(self._h, ua.ctypes.data, len(ua), t.ctypes.data), "nm_tile_targets")
+        return t
+
+    def set_skip_mode(self, mode, tile=64, net=None):
+        self._skipnet = net  # keeps the network alive while this view uses it
+        _check(lib().nm_set_skip_mode(self._h, mode, tile, net._h if net else None), "nm_set_skip_mode")
+
+    def tile_stats(self):
+        st = TileStats()
+        _check(lib().nm_get_tile_stats(self._h, C.byref(st)), "nm_get_tile_stats")
         return st.as_dict()
 
     def set_launch(self, block_k=0, waves_per_simd=0, tile=0, graph=False, pre_iter=0):

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libnewman_amd.so")
 ARCH = os.environ.get("NEWMAN_AMD_ARCH", "gfx950")
 
 HOST_SRCS = ["hpfloat.cpp", "mandelbrot.cpp", "capi.cpp"]
-HIP_SRCS = ["kernels.hip"]
+HIP_SRCS = ["kernels.hip", "skipnet.hip"]
 
 CXXFLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-function",
             "-pthread"]
@@ -32,16 +32,21 @@ def _run(cmd):
 
 
 def build_library(verbose=True):
+    """Every source compiled in parallel, then linked."""
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HOST_SRCS:
-        obj = os.path.join(BUILD, src + ".o")
-        _run(["g++"] + CXXFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append((["g++"] + CXXFLAGS + ["-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD, src + ".o")]))
     for src in HIP_SRCS:
-        obj = os.path.join(BUILD, src + ".o")
-        _run(["hipcc"] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append((["hipcc"] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD, src + ".o")]))
+    procs = []
+    for cmd in jobs:
+        print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    failed = [cmd for cmd, p in zip(jobs, procs) if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+    objs = [cmd[-1] for cmd in jobs]
     tmp = LIB + ".tmp"
     _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)

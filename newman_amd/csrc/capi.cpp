@@ -1,9 +1,11 @@
 // capi.cpp - extern "C" boundary (include/newman_amd.h) over nm::Mandelbrot.
 // No exception crosses this boundary; every entry point returns a status.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/newman_amd.h"
 #include "mandelbrot.hpp"
@@ -358,6 +360,132 @@ int nm_autotune_frames(nm_mandelbrot* h, const char* const* widths, int nframes,
     if (n_table) *n_table = std::min(n, cap);
     return (int)NM_OK;
   });
+}
+
+struct nm_skipnet {
+  nm::SkipNet net;
+};
+
+int nm_skipnet_create(nm_skipnet** out) {
+  if (!out) return NM_ERR_ARG;
+  return guard([&] {
+    *out = new nm_skipnet();
+    return (int)NM_OK;
+  });
+}
+
+void nm_skipnet_destroy(nm_skipnet* net) { delete net; }
+
+int nm_skipnet_get(const nm_skipnet* net, float* params, float* mean, float* scale) {
+  if (!net) return NM_ERR_ARG;
+  if (params) std::copy(net->net.params.begin(), net->net.params.end(), params);
+  if (mean) std::copy(net->net.mean.begin(), net->net.mean.end(), mean);
+  if (scale) std::copy(net->net.scale.begin(), net->net.scale.end(), scale);
+  return NM_OK;
+}
+
+int nm_skipnet_set(nm_skipnet* net, const float* params, const float* mean, const float* scale) {
+  if (!net || !params) return NM_ERR_ARG;
+  std::copy(params, params + nm::SkipNet::kParams, net->net.params.begin());
+  if (mean) std::copy(mean, mean + nm::SkipNet::kIn, net->net.mean.begin());
+  if (scale) std::copy(scale, scale + nm::SkipNet::kIn, net->net.scale.begin());
+  net->net.trained = true;
+  return NM_OK;
+}
+
+static int skipnet_status(nm_skipnet* net, int st) {
+  if (st == NM_ERR_HIP) g_err = net->net.last_error();
+  return st;
+}
+
+int nm_skipnet_train(nm_skipnet* net, const float* x, const float* t, int n, int epochs, int batch, float lr,
+                     uint64_t seed, float* curve) {
+  if (!net || !x || !t || n <= 0 || epochs <= 0 || batch <= 0 || !(lr > 0.0f)) return NM_ERR_ARG;
+  return guard([&] {
+    nm::SkipNet::TrainOpts o;
+    o.epochs = epochs;
+    o.batch = batch;
+    o.lr = lr;
+    o.seed = seed;
+    std::vector<float> c;
+    int st = net->net.train(x, t, (size_t)n, o, curve ? &c : nullptr);
+    if (st == NM_OK && curve) std::copy(c.begin(), c.end(), curve);
+    return skipnet_status(net, st);
+  });
+}
+
+int nm_skipnet_predict(nm_skipnet* net, const float* x, int n, float* y) {
+  if (!net || !x || !y || n <= 0) return NM_ERR_ARG;
+  return guard([&] { return skipnet_status(net, net->net.predict(x, (size_t)n, y)); });
+}
+
+int nm_skipnet_gradient(nm_skipnet* net, const float* xn, const float* t, int n, float* grad, float* loss) {
+  if (!net || !xn || !t || !grad || n <= 0) return NM_ERR_ARG;
+  return guard([&] { return skipnet_status(net, net->net.gradient(xn, t, (size_t)n, grad, loss)); });
+}
+
+int nm_tile_features(nm_mandelbrot* h, int tile, float* x, int cap_tiles, int* ntiles) {
+  if (!h || tile < 8 || cap_tiles < 0) return NM_ERR_ARG;
+  return guard([&] {
+    std::vector<float> f;
+    int st = h->m.tileFeatures(tile, f);
+    if (st != NM_OK) return engine_status(st);
+    const int nt = (int)(f.size() / nm::SkipNet::kIn);
+    if (ntiles) *ntiles = nt;
+    if (x) std::copy(f.begin(), f.begin() + (size_t)std::min(nt, cap_tiles) * nm::SkipNet::kIn, x);
+    return (int)NM_OK;
+  });
+}
+
+static void copy_tile_stats(const nm::Mandelbrot::TileStats& s, nm_tile_stats* o) {
+  if (!o) return;
+  o->ntiles = s.ntiles;
+  o->accepted = s.accepted;
+  o->rejected = s.rejected;
+  o->reserved = 0;
+  o->evals = s.evals;
+  o->ms_search = s.ms_search;
+  o->ms_predict = s.ms_predict;
+}
+
+int nm_tile_skips(nm_mandelbrot* h, int tile, int mode, nm_skipnet* net, int* ua, int* how, int cap_tiles,
+                  nm_tile_stats* stats) {
+  if (!h || tile < 8 || mode < 0 || mode > 2 || (mode == 2 && !net) || cap_tiles < 0) return NM_ERR_ARG;
+  return guard([&] {
+    std::vector<int> u, w;
+    int st = h->m.tileSkips(tile, mode, net ? &net->net : nullptr, u, &w);
+    if (st != NM_OK) return engine_status(st);
+    const size_t n = std::min(u.size(), (size_t)cap_tiles);
+    if (ua) std::copy(u.begin(), u.begin() + n, ua);
+    if (how) std::copy(w.begin(), w.begin() + n, how);
+    copy_tile_stats(h->m.tile_stats, stats);
+    return (int)NM_OK;
+  });
+}
+
+int nm_tile_targets(nm_mandelbrot* h, const int* ua, int n, float* t) {
+  if (!h || !ua || !t || n < 0) return NM_ERR_ARG;
+  return guard([&] {
+    if (!h->m.hasReference()) return (int)NM_ERR_STATE;
+    h->m.ensureRecords();
+    for (int i = 0; i < n; i++) t[i] = (float)h->m.targetOf(ua[i]);
+    return (int)NM_OK;
+  });
+}
+
+int nm_set_skip_mode(nm_mandelbrot* h, int mode, int tile, nm_skipnet* net) {
+  if (!h || mode < NM_SKIP_PIXEL || mode > NM_SKIP_TILE_LEARNED || tile < 8) return NM_ERR_ARG;
+  if (mode == NM_SKIP_TILE_LEARNED && !net) return NM_ERR_ARG;
+  h->m.skip_mode = mode;
+  h->m.skip_tile = tile;
+  h->m.skipnet = net ? &net->net : nullptr;
+  return NM_OK;
+}
+
+int nm_get_tile_stats(const nm_mandelbrot* h, nm_tile_stats* stats) {
+  if (!h || !stats) return NM_ERR_ARG;
+  copy_tile_stats(h->m.tile_stats, stats);
+  return NM_OK;
 }
 
 const char* nm_last_error(void) { return g_err.c_str(); }

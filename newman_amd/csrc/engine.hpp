@@ -70,6 +70,8 @@ struct RenderOpts {
   int tile = 0;           // sample order: 0 row-major, T = T x T tiles of 8 x 8 blocks (8..256)
   bool graph = false;     // replay repeats 2.. from one captured HIP graph
   int pre_iter = 0;       // plain path: updates in the start kernel (0 = default)
+  const int* tile_ua = nullptr;  // series path: per-tile first unstable index (null: per sample)
+  int skip_T = 0;                // its tile side
   uint64_t set_serial = 0; // identifies the frame set: the engine keeps its tables
                            // on the device and skips the upload when it repeats (0 = always upload)
 };
@@ -106,6 +108,15 @@ class Engine {
                         EscapeValue* host_out, RenderStats* stats) = 0;
   virtual int render_sa(const std::vector<const SaFrameHost*>& frames, const RenderOpts& opt,
                         EscapeValue* host_out, RenderStats* stats) = 0;
+  // per-tile series skip at each tile's probe (see k_tile_skip)
+  struct TileSearch {
+    int T = 64;             // tile side
+    int mode = 0;           // 0 the reference's scan, 1 threshold tables, 2 validate pred
+    std::vector<int> pred;  // mode 2: one per tile (frame-major, row-major tiles)
+    std::vector<int> ua, how, evals;
+    double ms = 0.0;
+  };
+  virtual int tile_skip(const std::vector<const SaFrameHost*>& frames, const RenderOpts& opt, TileSearch& ts) = 0;
   // multisample average (mandelbrot.cpp:320-332) of a fine grid on the device
   virtual int downsample(const EscapeValue* fine, int nr, int nc, int sc, EscapeValue* out) = 0;
   virtual const char* last_error() const = 0;

@@ -128,6 +128,10 @@ struct SeriesArgs {
   int len;
   int mend;  // len + 1
   double tol;
+  // per-tile series skip (the first unstable index of each tile's probe); null:
+  // every sample searches its own (the reference's per-pixel scan)
+  const int* tile_ua;
+  uint32_t skip_T, skip_ntc, skip_tpf;  // tile side, tiles per frame row, tiles per frame
 };
 
 // ------------------------------------------------------------ device helpers
@@ -667,6 +671,74 @@ __device__ int sa_stop_index(const SeriesArgs& a, double L, int2 br, double2 e2,
   return 0;
 }
 
+// Per-tile series skip: the first index at which the reference's stability test
+// fails at each tile's probe (its sample farthest from the reference point).
+// One wave per tile.  mode 0: the reference's scan over i = 1, 2, ... (64 indices
+// per round, ballot for the first unstable one); mode 1: the threshold-table
+// search of the per-sample path; mode 2: validate pred[t] and fall back to the
+// scan.  A prediction p is accepted only when it is exactly the first unstable
+// index: every i < p is certainly stable by the prefix minimum of the
+// thresholds (PM[p-1] > log2|e|^2 + margin, with the magnitude checks that make
+// the exact test safe) and the exact test fails at p (p = 0: PM[last] clears).
+// how[t]: 0 scanned, 1 table, 2 prediction accepted, 3 prediction rejected
+// (scanned); evals[t]: exact tests evaluated.
+__global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* probe, const int* pred, int mode,
+                                                  int ntiles, int* ua_out, int* how, int* evals) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int lane = lane_id();
+  const int2 pp = probe[t];
+  const double2 e = make_double2(a.epsre[pp.x], a.epsim[pp.y]);
+  const double2 e2 = make_double2(e.x * e.x - e.y * e.y, 2.0 * e.x * e.y);
+  const double2 e3 = cmul(e, e2);
+  const double L = log2_sqmag(e);
+  const int last = a.len - 1;
+  int ua = -1, h = 0, ev = 0;
+  if (last < 1 || L == -INFINITY) {  // e = 0: the test never fires
+    ua = 0;
+    h = 1;
+  } else if (mode == 1) {
+    ua = sa_stop_index(a, L, make_int2(1, last), e2, e3);
+    h = 1;
+  } else if (mode == 2) {
+    const int p = pred[t];
+    const int below = p > 0 ? p - 1 : last;
+    bool ok = p >= 0 && p <= last && below >= 1 && a.PM[below] > L + kMarg && a.SP[below] == 0 &&
+              L > -600.0 && L < 600.0 && a.BX[below] + 2.0 * L < 960.0 && a.BN[below] + 2.0 * L > -900.0 &&
+              a.CX[below] + 3.0 * L < 960.0 && a.CN[below] + 3.0 * L > -960.0;
+    if (ok && p > 0) {
+      ok = unstable_at(a, p, e2, e3);
+      ev = 1;
+    }
+    if (ok) {
+      ua = p;
+      h = 2;
+    } else {
+      h = 3;
+    }
+  }
+  if (ua < 0) {  // the reference's scan
+    ua = 0;
+    for (int base = 1; base <= last; base += 64) {
+      const int i = base + lane;
+      const bool u = i <= last && unstable_at(a, i, e2, e3);
+      const unsigned long long m = __ballot(u);
+      if (m) {
+        const int k = __ffsll((unsigned long long)m);  // 1 + the first unstable lane
+        ua = base + k - 1;
+        ev += k;
+        break;
+      }
+      ev += min(64, last - base + 1);
+    }
+  }
+  if (lane == 0) {
+    ua_out[t] = ua;
+    how[t] = h;
+    evals[t] = ev;
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
   const Common& c = a.c;
   unsigned long long ncard = 0, nres = 0, skipped = 0;
@@ -696,7 +768,14 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
         }
       }
       // phase 2 (all lanes): the wave's bracket of the stop-index search
-      const int2 br = wave_stop_bracket(a, L, valid && !in && L > -INFINITY);
+      const int2 br = a.tile_ua ? make_int2(1, a.len - 1) : wave_stop_bracket(a, L, valid && !in && L > -INFINITY);
+      uint32_t skip_tile = 0;
+      if (a.tile_ua && valid) {
+        const uint32_t fi = frame_of(gid, c.inv_frame);
+        const DevFrame& f = c.frames[fi];
+        const Loc lc = locate(f, gid - f.sample_base);
+        skip_tile = fi * a.skip_tpf + (lc.r / a.skip_T) * a.skip_ntc + lc.c / a.skip_T;
+      }
       // phase 3: the reference's series start for this sample
       if (!valid) {
         if (gid < c.total) c.flag[gid] = 0;
@@ -710,7 +789,7 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
         // eps2 = sq(eps), eps3 = eps * eps2 (complex.h:19-21, mandelbrot.cpp:160-161)
         double2 e2 = make_double2(e.x * e.x - e.y * e.y, 2.0 * e.x * e.y);
         double2 e3 = cmul(e, e2);
-        int ua = sa_stop_index(a, L, br, e2, e3);
+        int ua = a.tile_ua ? a.tile_ua[skip_tile] : sa_stop_index(a, L, br, e2, e3);
         int dsize = ua ? max(ua - 3, 1) : a.len;
         int found = dsize - 1;
         double2 d = sa_term(a, found, e, e2, e3);
@@ -1034,6 +1113,9 @@ class EngineImpl : public Engine {
   void* d_ref = nullptr; size_t n_ref_bytes = 0;
   void* d_grid = nullptr; size_t n_grid_bytes = 0;  // downsample staging
   void* d_emit = nullptr; size_t n_emit_bytes = 0;  // row-major grids of a tiled sample order
+  void* d_ttabs = nullptr; size_t n_ttabs_bytes = 0;  // tile search: epsilon tables
+  void* d_tio = nullptr; size_t n_tio_bytes = 0;      // tile search: probes, predictions, results
+  void* d_tile_ua = nullptr; size_t n_tile_ua_bytes = 0;  // per-tile skips of a render
   unsigned long long* d_part = nullptr;               // kPartSlots x C_COUNT
   uint32_t* d_nactive = nullptr;
   uint64_t ref_serial = 0;
@@ -1476,6 +1558,35 @@ class EngineImpl : public Engine {
     a.epsre = t;
     a.epsim = t + ncol;
     a.mask = (const uint32_t*)d_mask;
+    fill_reference(a, opt);
+    a.tile_ua = nullptr;
+    a.skip_T = a.skip_ntc = a.skip_tpf = 0;
+    if (opt.tile_ua) {
+      if (opt.skip_T < 8) return NM_ERR_ARG;
+      const uint32_t T = (uint32_t)opt.skip_T, ntc = (frames[0]->nc + T - 1) / T, ntr = (frames[0]->nr + T - 1) / T;
+      const size_t nt = (size_t)ntc * ntr * frames.size();
+      if ((st = grow(&d_tile_ua, &n_tile_ua_bytes, nt * 4))) return st;
+      NM_HIP(hipMemcpyAsync(d_tile_ua, opt.tile_ua, nt * 4, hipMemcpyHostToDevice, stream));
+      a.tile_ua = (const int*)d_tile_ua;
+      a.skip_T = T;
+      a.skip_ntc = ntc;
+      a.skip_tpf = ntc * ntr;
+    }
+    const int K = opt.block_k > 0 ? opt.block_k : 8;
+    const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
+    if (stats) {
+      stats->block_k = K;
+      stats->waves_per_simd = waves;
+    }
+    switch (K) {
+      case 8: return run(a, k_series_start, k_series_iterate<8>, waves, opt, stats, nout, host_out, emit_to);
+      case 16: return run(a, k_series_start, k_series_iterate<16>, waves, opt, stats, nout, host_out, emit_to);
+      case 32: return run(a, k_series_start, k_series_iterate<32>, waves, opt, stats, nout, host_out, emit_to);
+      default: return NM_ERR_ARG;
+    }
+  }
+
+  void fill_reference(SeriesArgs& a, const RenderOpts& opt) const {
     a.Zd = rZd;
     a.Xlo = rXlo;
     a.A = rA;
@@ -1490,18 +1601,74 @@ class EngineImpl : public Engine {
     a.len = ref_len;
     a.mend = ref_len + 1;
     a.tol = opt.tol;
-    const int K = opt.block_k > 0 ? opt.block_k : 8;
-    const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
-    if (stats) {
-      stats->block_k = K;
-      stats->waves_per_simd = waves;
+  }
+
+  int tile_skip(const std::vector<const SaFrameHost*>& frames, const RenderOpts& opt, TileSearch& ts) override {
+    if (frames.empty() || ref_len < 1) return NM_ERR_STATE;
+    if (ts.T < 8 || ts.mode < 0 || ts.mode > 2) return NM_ERR_ARG;
+    const int nr = frames[0]->nr, nc = frames[0]->nc, T = ts.T;
+    const int ntc = (nc + T - 1) / T, ntr = (nr + T - 1) / T;
+    const size_t nt = (size_t)ntc * ntr * frames.size();
+    if (ts.mode == 2 && ts.pred.size() != nt) return NM_ERR_ARG;
+    std::vector<double> eps;
+    std::vector<int2> probe(nt);
+    size_t ncol = 0, nrow = 0;
+    for (auto* f : frames) {
+      if (!f || f->nr != nr || f->nc != nc) return NM_ERR_ARG;
+      ncol += nc;
+      nrow += nr;
     }
-    switch (K) {
-      case 8: return run(a, k_series_start, k_series_iterate<8>, waves, opt, stats, nout, host_out, emit_to);
-      case 16: return run(a, k_series_start, k_series_iterate<16>, waves, opt, stats, nout, host_out, emit_to);
-      case 32: return run(a, k_series_start, k_series_iterate<32>, waves, opt, stats, nout, host_out, emit_to);
-      default: return NM_ERR_ARG;
+    eps.reserve(ncol + nrow);
+    for (auto* f : frames) eps.insert(eps.end(), f->epsre.begin(), f->epsre.end());
+    for (auto* f : frames) eps.insert(eps.end(), f->epsim.begin(), f->epsim.end());
+    for (size_t k = 0; k < frames.size(); k++) {
+      const SaFrameHost& f = *frames[k];
+      for (int tr = 0; tr < ntr; tr++)
+        for (int tc = 0; tc < ntc; tc++) {
+          // the probe: the tile's sample farthest from the reference point; the
+          // epsilon tables are separable and monotone, so it is a corner
+          const int c0 = tc * T, c1 = std::min(nc, c0 + T) - 1, r0 = tr * T, r1 = std::min(nr, r0 + T) - 1;
+          const int pc = std::fabs(f.epsre[c1]) > std::fabs(f.epsre[c0]) ? c1 : c0;
+          const int pr = std::fabs(f.epsim[r1]) > std::fabs(f.epsim[r0]) ? r1 : r0;
+          probe[k * (size_t)ntc * ntr + (size_t)tr * ntc + tc] = make_int2((int)(k * nc + pc), (int)(k * nr + pr));
+        }
     }
+    int st;
+    if ((st = grow(&d_ttabs, &n_ttabs_bytes, eps.size() * 8))) return st;
+    if ((st = grow(&d_tio, &n_tio_bytes, nt * 20))) return st;
+    char* io = (char*)d_tio;
+    int2* dprobe = (int2*)io;
+    int* dpred = (int*)(io + nt * 8);
+    int* dua = (int*)(io + nt * 12);
+    int* dhow = (int*)(io + nt * 16);
+    std::vector<int> evals_host(nt);
+    int* devals = nullptr;
+    if ((st = grow(&d_tile_ua, &n_tile_ua_bytes, nt * 4))) return st;
+    devals = (int*)d_tile_ua;
+    NM_HIP(hipMemcpyAsync(d_ttabs, eps.data(), eps.size() * 8, hipMemcpyHostToDevice, stream));
+    NM_HIP(hipMemcpyAsync(dprobe, probe.data(), nt * 8, hipMemcpyHostToDevice, stream));
+    if (ts.mode == 2) NM_HIP(hipMemcpyAsync(dpred, ts.pred.data(), nt * 4, hipMemcpyHostToDevice, stream));
+    SeriesArgs a{};
+    a.epsre = (const double*)d_ttabs;
+    a.epsim = (const double*)d_ttabs + ncol;
+    fill_reference(a, opt);
+    if ((st = ensure_events(2))) return st;
+    NM_HIP(hipEventRecord(evpool[0], stream));
+    hipLaunchKernelGGL(k_tile_skip, dim3((unsigned)nt), dim3(64), 0, stream, a, (const int2*)dprobe,
+                       (const int*)dpred, ts.mode, (int)nt, dua, dhow, devals);
+    NM_HIP(hipGetLastError());
+    NM_HIP(hipEventRecord(evpool[1], stream));
+    ts.ua.resize(nt);
+    ts.how.resize(nt);
+    ts.evals.resize(nt);
+    NM_HIP(hipMemcpyAsync(ts.ua.data(), dua, nt * 4, hipMemcpyDeviceToHost, stream));
+    NM_HIP(hipMemcpyAsync(ts.how.data(), dhow, nt * 4, hipMemcpyDeviceToHost, stream));
+    NM_HIP(hipMemcpyAsync(ts.evals.data(), devals, nt * 4, hipMemcpyDeviceToHost, stream));
+    NM_HIP(hipStreamSynchronize(stream));
+    float ms = 0;
+    NM_HIP(hipEventElapsedTime(&ms, evpool[0], evpool[1]));
+    ts.ms = ms;
+    return NM_OK;
   }
 
   int downsample(const EscapeValue* fine, int nr, int nc, int sc, EscapeValue* out) override {

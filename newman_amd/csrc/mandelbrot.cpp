@@ -2,7 +2,9 @@
 // Each method cites the reference method it replaces.
 #include "mandelbrot.hpp"
 
+#include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -546,6 +548,8 @@ std::string Mandelbrot::snapshot() const {
   add(&error_tolerance, sizeof error_tolerance);
   add(&bailout, sizeof bailout);
   add(&delta_mode, sizeof delta_mode);
+  add(&skip_mode, sizeof skip_mode);
+  add(&skip_tile, sizeof skip_tile);
   add(&grid.nr, sizeof grid.nr);
   add(&grid.nc, sizeof grid.nc);
   add(&ref_serial_, sizeof ref_serial_);
@@ -661,7 +665,200 @@ int Mandelbrot::renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue*
   if (st != NM_OK) return st;
   std::vector<const SaFrameHost*> ptr;
   for (size_t k = first; k < set.s.size(); k += step) ptr.push_back(&set.s[k]);
-  return eng->render_sa(ptr, o, out, &stats_);
+  if (skip_mode == SKIP_PIXEL) return eng->render_sa(ptr, o, out, &stats_);
+  // one series skip per tile, from the tile probes
+  std::vector<int> ua;
+  if ((st = tileSkipsFor(eng, ptr, o, ua)) != NM_OK) return st;
+  RenderOpts ot = o;
+  ot.tile_ua = ua.data();
+  ot.skip_T = skip_tile;
+  return eng->render_sa(ptr, ot, out, &stats_);
+}
+
+// ------------------------------------------------------------ tile skips
+namespace {
+// complex floatexp: (re + i im) * 2^e with max(|re|, |im|) in [1/2, 1), or zero
+struct FeC {
+  double re = 0.0, im = 0.0;
+  int64_t e = 0;
+};
+FeC fe_norm(double re, double im, int64_t e) {
+  const double m = std::max(std::fabs(re), std::fabs(im));
+  if (m == 0.0 || !std::isfinite(m)) return FeC{};
+  int k = 0;
+  std::frexp(m, &k);
+  return FeC{std::ldexp(re, -k), std::ldexp(im, -k), e + k};
+}
+FeC fe_mul(const FeC& a, const FeC& b) {
+  return fe_norm(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re, a.e + b.e);
+}
+FeC fe_add(const FeC& a, const FeC& b) {
+  if (a.re == 0.0 && a.im == 0.0) return b;
+  if (b.re == 0.0 && b.im == 0.0) return a;
+  const int64_t e = std::max(a.e, b.e);
+  const int64_t da = a.e - e, db = b.e - e;
+  const double sa = da < -1100 ? 0.0 : std::ldexp(1.0, (int)da), sb = db < -1100 ? 0.0 : std::ldexp(1.0, (int)db);
+  return fe_norm(a.re * sa + b.re * sb, a.im * sa + b.im * sb, e);
+}
+double fe_log2(const FeC& a) {
+  const double m = std::hypot(a.re, a.im);
+  return m > 0.0 ? (double)a.e + std::log2(m) : -4096.0;
+}
+}  // namespace
+
+// log2 |a_k(i)| of the Taylor series delta_i = sum_k a_k(i) eps^k of the
+// perturbation recurrence delta_{i+1} = 2 X_i delta_i + delta_i^2 + eps with
+// delta_0 = eps (a_1(0) = 1), for k = 1..8 at the orbit indices
+// i_j = (j + 1) len / 17, j = 0..15 (the Taylor series, not the reference's
+// three-term one: features only)
+static void taylor_features(const std::vector<HPComplex>& X, int len, std::vector<double>& out) {
+  constexpr int K = 8, J = 16;
+  out.assign(K * J, -4096.0);
+  FeC a[K + 1];
+  a[1] = fe_norm(1.0, 0.0, 0);
+  int j = 0;
+  for (int i = 1; i < len && j < J; i++) {
+    const FeC z2 = fe_norm(2.0 * X[i - 1].re.get_d(), 2.0 * X[i - 1].im.get_d(), 0);
+    FeC nx[K + 1];
+    for (int k = 1; k <= K; k++) {
+      FeC v = fe_mul(z2, a[k]);
+      for (int p = 1; p < k; p++) v = fe_add(v, fe_mul(a[p], a[k - p]));
+      if (k == 1) v = fe_add(v, fe_norm(1.0, 0.0, 0));
+      nx[k] = v;
+    }
+    for (int k = 1; k <= K; k++) a[k] = nx[k];
+    while (j < J && i == (int)(((int64_t)(j + 1) * len) / 17)) {
+      for (int k = 1; k <= K; k++) out[(size_t)(k - 1) * J + j] = fe_log2(a[k]);
+      j++;
+    }
+  }
+}
+
+static double hp_log10(const hpf& v) {
+  long e = 0;
+  const double m = v.get_d_2exp(&e);
+  return m != 0.0 ? std::log10(std::fabs(m)) + (double)e * std::log10(2.0) : -4096.0;
+}
+
+int Mandelbrot::tileFeatures(int T, std::vector<float>& feats) {
+  if (!have_ref_ || useHardware()) return NM_ERR_STATE;
+  if (T < 8) return NM_ERR_ARG;
+  if (taylor_serial_ != ref_serial_) {
+    taylor_features(X, ref_.len, taylor_feats_);
+    taylor_serial_ = ref_serial_;
+  }
+  SaFrameHost f;
+  int st = buildSaFrame(f);
+  if (st != NM_OK) return st;
+  hpf w(sz.re.L);
+  hp_mul_si(w, sz.re, (long)cols());
+  const double lw = hp_log10(w);
+  const double wd = std::pow(10.0, lw);  // width as a double exponent-safe scale below
+  const int ntc = (cols() + T - 1) / T, ntr = (rows() + T - 1) / T;
+  feats.assign((size_t)ntc * ntr * SkipNet::kIn, 0.0f);
+  for (int tr = 0; tr < ntr; tr++)
+    for (int tc = 0; tc < ntc; tc++) {
+      const int c0 = tc * T, c1 = std::min(cols(), c0 + T) - 1, r0 = tr * T, r1 = std::min(rows(), r0 + T) - 1;
+      const int pc = std::fabs(f.epsre[c1]) > std::fabs(f.epsre[c0]) ? c1 : c0;
+      const int pr = std::fabs(f.epsim[r1]) > std::fabs(f.epsim[r0]) ? r1 : r0;
+      float* x = &feats[((size_t)tr * ntc + tc) * SkipNet::kIn];
+      x[0] = (float)lw;
+      x[1] = wd > 0.0 ? (float)(f.epsre[pc] / wd) : 0.0f;
+      x[2] = wd > 0.0 ? (float)(f.epsim[pr] / wd) : 0.0f;
+      for (int k = 0; k < 128; k++) x[3 + k] = (float)taylor_feats_[k];
+      x[131] = (float)std::log10((double)N);
+    }
+  return NM_OK;
+}
+
+// Indices where the prefix minimum of the log2 thresholds drops ("records"): a
+// probe's first unstable index is one of them unless its |e|^2 lies within
+// rounding of a threshold.  The network's target is the record's rank, scaled:
+// t = (rank + 1) / (R + 1) for R records, and t = 1 for "never unstable"; its
+// output snaps to the nearest rank.
+void Mandelbrot::ensureRecords() {
+  if (records_serial_ == ref_serial_) return;
+  records_.clear();
+  records_serial_ = ref_serial_;
+  for (int i = 1; i < ref_.len; i++)
+    if (ref_.PM[i] < (i > 1 ? ref_.PM[i - 1] : INFINITY)) records_.push_back(i);
+}
+
+int Mandelbrot::snapIndex(double frac) const {
+  const int R = (int)records_.size();
+  const long rank = std::lround(frac * (R + 1)) - 1;
+  if (rank >= R) return 0;  // no unstable index
+  return records_[(size_t)std::max(0L, rank)];
+}
+
+double Mandelbrot::targetOf(int ua) const {
+  const int R = (int)records_.size();
+  if (ua <= 0) return 1.0;
+  auto it = std::lower_bound(records_.begin(), records_.end(), ua);
+  const long rank = it - records_.begin();  // a non-record index takes the next record's rank
+  return (double)(rank + 1) / (double)(R + 1);
+}
+
+int Mandelbrot::tileSkips(int T, int mode, SkipNet* net, std::vector<int>& ua, std::vector<int>* how) {
+  if (!have_ref_ || useHardware()) return NM_ERR_STATE;
+  Engine* eng = nullptr;
+  int st = Engine::instance(&eng);
+  if (st != NM_OK) return st;
+  if ((st = eng->set_reference(ref_, ref_serial_)) != NM_OK) return st;
+  SaFrameHost f;
+  if ((st = buildSaFrame(f)) != NM_OK) return st;
+  RenderOpts o;
+  applyLaunch(o);
+  const int keep_mode = skip_mode, keep_T = skip_tile;
+  SkipNet* keep_net = skipnet;
+  skip_mode = mode == 0 ? SKIP_TILE_SCAN : (mode == 1 ? SKIP_TILE_TABLE : SKIP_TILE_LEARNED);
+  skip_tile = T;
+  skipnet = net;
+  std::vector<const SaFrameHost*> ptr{&f};
+  Engine::TileSearch ts;
+  st = tileSkipsFor(eng, ptr, o, ua, &ts);
+  skip_mode = keep_mode;
+  skip_tile = keep_T;
+  skipnet = keep_net;
+  if (st == NM_OK && how) *how = ts.how;
+  return st;
+}
+
+int Mandelbrot::tileSkipsFor(Engine* eng, const std::vector<const SaFrameHost*>& frames, const RenderOpts& o,
+                             std::vector<int>& ua, Engine::TileSearch* out) {
+  Engine::TileSearch ts;
+  ts.T = skip_tile;
+  tile_stats = TileStats();
+  if (skip_mode == SKIP_TILE_LEARNED) {
+    if (!skipnet || !skipnet->trained) return NM_ERR_STATE;
+    if (frames.size() != 1) return NM_ERR_ARG;  // the features describe this object's view
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<float> feats;
+    int st = tileFeatures(skip_tile, feats);
+    if (st != NM_OK) return st;
+    const size_t nt = feats.size() / SkipNet::kIn;
+    std::vector<float> y(nt);
+    if ((st = skipnet->predict(feats.data(), nt, y.data())) != NM_OK) return NM_ERR_HIP;
+    ensureRecords();
+    ts.pred.resize(nt);
+    for (size_t t = 0; t < nt; t++) ts.pred[t] = snapIndex((double)y[t]);
+    tile_stats.ms_predict = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ts.mode = 2;
+  } else {
+    ts.mode = skip_mode == SKIP_TILE_TABLE ? 1 : 0;
+  }
+  int st = eng->tile_skip(frames, o, ts);
+  if (st != NM_OK) return st;
+  ua = ts.ua;
+  tile_stats.ntiles = (int)ua.size();
+  for (size_t t = 0; t < ua.size(); t++) {
+    tile_stats.accepted += ts.how[t] == 2;
+    tile_stats.rejected += ts.how[t] == 3;
+    tile_stats.evals += (uint64_t)ts.evals[t];
+  }
+  tile_stats.ms_search = ts.ms;
+  if (out) *out = ts;
+  return NM_OK;
 }
 
 int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts* opts, EscapeValue* out) {

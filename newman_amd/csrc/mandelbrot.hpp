@@ -13,6 +13,7 @@
 
 #include "engine.hpp"
 #include "hpfloat.hpp"
+#include "skipnet.hpp"
 
 namespace nm {
 
@@ -47,6 +48,23 @@ class Mandelbrot {
   };
   Launch launch;
 
+  // Series skip granularity on the series path: per sample (the reference's
+  // per-pixel scan, default) or one skip per T x T tile taken at the tile's probe
+  // (its sample farthest from the reference point), found by the reference's scan
+  // (SKIP_TILE_SCAN), the threshold tables (SKIP_TILE_TABLE) or predicted by a
+  // SkipNet and validated, with the scan as fallback (SKIP_TILE_LEARNED).
+  enum SkipMode { SKIP_PIXEL = 0, SKIP_TILE_SCAN = 1, SKIP_TILE_TABLE = 2, SKIP_TILE_LEARNED = 3 };
+  int skip_mode = SKIP_PIXEL;
+  int skip_tile = 64;
+  SkipNet* skipnet = nullptr;  // not owned
+  struct TileStats {
+    int ntiles = 0, accepted = 0, rejected = 0;
+    uint64_t evals = 0;         // exact stability tests evaluated at the probes
+    double ms_search = 0.0;     // the tile kernel
+    double ms_predict = 0.0;    // features + network + snapping (host wall time)
+  };
+  TileStats tile_stats;
+
   Mandelbrot();
   Mandelbrot(int nr, int nc);
 
@@ -80,6 +98,7 @@ class Mandelbrot {
   int precomputeAt(const char* re, const char* im);  // explicit reference point
   int referenceLength() const { return (int)ref_.len; }
   int probeWinner() const { return winner_; }
+  bool hasReference() const { return have_ref_; }
   const SaReferenceHost& reference() const { return ref_; }
   RenderGrid& grid_ref() { return grid; }
   const RenderStats& lastStats() const { return stats_; }
@@ -99,6 +118,24 @@ class Mandelbrot {
   // frames first, first + step, ... of a built set, one launch set
   int renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue* out, size_t first, size_t step);
   static uint64_t next_serial();
+  // per-tile features of the learned skip predictor for this view (kIn floats per
+  // tile, tiles row-major): log10 width; the probe's offset from the reference
+  // point in widths (re, im); log2 |a_k(i)| of the 8-term Taylor series of the
+  // delta in epsilon (k = 1..8) at 16 orbit indices; log10 N
+  int tileFeatures(int T, std::vector<float>& feats);
+  // per-tile first unstable index at the probes; mode 0 scan, 1 tables, 2 learned
+  int tileSkips(int T, int mode, SkipNet* net, std::vector<int>& ua, std::vector<int>* how);
+  // the learned network's target for a first unstable index, and back
+  void ensureRecords();
+  double targetOf(int ua) const;
+  int snapIndex(double frac) const;
+  int tileSkipsFor(Engine* eng, const std::vector<const SaFrameHost*>& frames, const RenderOpts& o,
+                   std::vector<int>& ua, Engine::TileSearch* out = nullptr);
+  std::vector<double> taylor_feats_;  // 128 per reference, cached by serial
+  uint64_t taylor_serial_ = 0;
+  std::vector<int> records_;          // indices where the threshold prefix minimum drops
+  uint64_t records_serial_ = 0;
+
   // the last frame set built, its key (view snapshot + widths) and serial
   FrameSet cache_;
   std::string cache_key_;

@@ -837,8 +837,9 @@ static double* make_zd(const orc_view* v) {
 }
 
 /* the GPU method for one sample: reference SA start (scan), SA-escape binary
- * search with double-double truncation, then pert_from */
-static orc_escape pert_sample(const orc_view* v, const pert_ctx* P, const ohp* y0r, const ohp* y0i) {
+ * search with double-double truncation, then pert_from.  ua_given >= 0 replaces
+ * the sample's own scan by a given first unstable index (a tile's, 0 = none). */
+static orc_escape pert_sample(const orc_view* v, const pert_ctx* P, const ohp* y0r, const ohp* y0i, int ua_given) {
   orc_escape esc;
   const int N = v->N;
   const double b2 = v->bailout * v->bailout;
@@ -850,7 +851,13 @@ static orc_escape pert_sample(const orc_view* v, const pert_ctx* P, const ohp* y
   lpc eps = pixel_eps(v, y0r, y0i);
   lpc eps2 = lp_sq(eps), eps3 = lp_mul(eps, eps2);
   int ua;
-  int dsize = sa_scan(v, eps, &ua);
+  int dsize;
+  if (ua_given < 0) {
+    dsize = sa_scan(v, eps, &ua);
+  } else {
+    ua = ua_given;
+    dsize = ua ? (ua - 3 < 1 ? 1 : ua - 3) : v->len;
+  }
   int found = dsize - 1;
   lpc d = sa_term(v, found, eps, eps2, eps3);
   lpc y = {trunc_sum3(v->Xn[2 * found], v->Xlo[2 * found], d.re),
@@ -888,27 +895,28 @@ typedef struct {
   pert_ctx P;
   int ncols; /* for full rows */
   const int* rows;
+  const int* ua; /* per-sample first unstable index (tile skips), or NULL */
 } render_ctx;
 
-static orc_escape sample_value(render_ctx* R, int r, int c) {
+static orc_escape sample_value(render_ctx* R, int r, int c, long i) {
   orc_view* v = R->v;
   ohp re, im;
   pixel_coord(v, r, c, &re, &im);
   if (orc_use_hardware(v)) return hw_iterations(&re, &im, v->n, v->N, v->bailout);
   if (R->mode == 0) return sa_iterations(v, &re, &im);
-  return pert_sample(v, &R->P, &re, &im);
+  return pert_sample(v, &R->P, &re, &im, R->ua ? R->ua[i] : -1);
 }
 
 static void samples_fn(void* cp, long i) {
   render_ctx* R = (render_ctx*)cp;
-  R->out[i] = sample_value(R, R->rc[2 * i], R->rc[2 * i + 1]);
+  R->out[i] = sample_value(R, R->rc[2 * i], R->rc[2 * i + 1], i);
 }
 
 static void rows_fn(void* cp, long i) {
   render_ctx* R = (render_ctx*)cp;
   long k = i / R->ncols;
   int c = (int)(i % R->ncols);
-  R->out[i] = sample_value(R, R->rows[k], c);
+  R->out[i] = sample_value(R, R->rows[k], c, i);
 }
 
 static void setup_pert(orc_view* v, render_ctx* R) {
@@ -925,7 +933,7 @@ static void setup_pert(orc_view* v, render_ctx* R) {
 }
 
 void orc_render_rows(orc_view* v, const int* rows, int nrows, orc_escape* out, int nthreads) {
-  render_ctx R = {v, NULL, out, 0};
+  render_ctx R = {v, NULL, out, 0, {0}, 0, NULL, NULL};
   R.ncols = v->nc;
   R.rows = rows;
   setup_pert(v, &R);
@@ -934,17 +942,57 @@ void orc_render_rows(orc_view* v, const int* rows, int nrows, orc_escape* out, i
 }
 
 void orc_render_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nthreads) {
-  render_ctx R = {v, rc, out, 0};
+  render_ctx R = {v, rc, out, 0, {0}, 0, NULL, NULL};
   setup_pert(v, &R);
   parallel_for(n, nthreads, 4, samples_fn, &R);
   free((void*)R.P.Zd);
 }
 
 void orc_pert_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nthreads) {
-  render_ctx R = {v, rc, out, 1};
+  render_ctx R = {v, rc, out, 1, {0}, 0, NULL, NULL};
   setup_pert(v, &R);
   parallel_for(n, nthreads, 4, samples_fn, &R);
   free((void*)R.P.Zd);
+}
+
+/* the GPU method with a given first unstable index per sample (tile skips) */
+void orc_pert_samples_ua(orc_view* v, const int* rc, const int* ua, int n, orc_escape* out, int nthreads) {
+  render_ctx R = {v, rc, out, 1, {0}, 0, NULL, ua};
+  setup_pert(v, &R);
+  parallel_for(n, nthreads, 4, samples_fn, &R);
+  free((void*)R.P.Zd);
+}
+
+/* Per T x T tile (row-major tiles): the probe is the tile's sample farthest from
+ * the reference point, a corner (epsilon is separable and monotone along rows and
+ * columns; a tie keeps the first column / row); its first unstable index by the
+ * reference's scan (mandelbrot.cpp:165-181), 0 = none.  probe_rc gets (r, c). */
+int orc_tile_probe_ua(orc_view* v, int T, int* ua, int* probe_rc) {
+  if (T < 1 || v->len < 1) return -1;
+  int ntc = (v->nc + T - 1) / T, ntr = (v->nr + T - 1) / T;
+  for (int tr = 0; tr < ntr; tr++)
+    for (int tc = 0; tc < ntc; tc++) {
+      int c0 = tc * T, c1 = (c0 + T < v->nc ? c0 + T : v->nc) - 1;
+      int r0 = tr * T, r1 = (r0 + T < v->nr ? r0 + T : v->nr) - 1;
+      double e0[2], e1[2], f0[2], f1[2];
+      orc_pixel_eps(v, r0, c0, e0);
+      orc_pixel_eps(v, r0, c1, e1);
+      orc_pixel_eps(v, r0, c0, f0);
+      orc_pixel_eps(v, r1, c0, f1);
+      int pc = fabs(e1[0]) > fabs(e0[0]) ? c1 : c0;
+      int pr = fabs(f1[1]) > fabs(f0[1]) ? r1 : r0;
+      double e[2];
+      orc_pixel_eps(v, pr, pc, e);
+      lpc eps = {e[0], e[1]};
+      int u = 0;
+      sa_scan(v, eps, &u);
+      ua[tr * ntc + tc] = u;
+      if (probe_rc) {
+        probe_rc[2 * (tr * ntc + tc)] = pr;
+        probe_rc[2 * (tr * ntc + tc) + 1] = pc;
+      }
+    }
+  return ntc * ntr;
 }
 
 /* ------------------------------------------------------- multisampling */

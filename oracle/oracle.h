@@ -70,6 +70,17 @@ void orc_render_rows(orc_view* v, const int* rows, int nrows, orc_escape* out, i
 void orc_render_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nthreads);
 /* GPU-method restatement (pert64) on selected samples */
 void orc_pert_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nthreads);
+/* the same with a given first unstable index per sample (a tile's skip; 0 = none) */
+void orc_pert_samples_ua(orc_view* v, const int* rc, const int* ua, int n, orc_escape* out, int nthreads);
+/* per T x T tile (row-major): first unstable index of the reference's scan at the
+ * tile's probe (its sample farthest from the reference point); returns the tile
+ * count; probe_rc (NULL ok) gets (row, column) per tile */
+int orc_tile_probe_ua(orc_view* v, int T, int* ua, int* probe_rc);
+
+/* the learned skip predictor's network in double (mlp.c) */
+int orc_mlp_nparams(void);
+void orc_mlp_forward(const float* p, const float* x, int n, double* y);
+double orc_mlp_grad(const float* p, const float* x, const float* t, int n, double* g);
 /* per-sample SA start of the reference (mandelbrot.cpp:155-207): returns the first
  * unstable index (0 if none), `found`, d[found], and the SA-escape result if any. */
 int orc_sa_start(orc_view* v, int r, int c, int* found, double* d, orc_escape* esc);

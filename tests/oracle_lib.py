@@ -44,6 +44,11 @@ _SIGS = [
     ("orc_render_rows", None, [_V, _V, C.c_int, _V, C.c_int]),
     ("orc_render_samples", None, [_V, _V, C.c_int, _V, C.c_int]),
     ("orc_pert_samples", None, [_V, _V, C.c_int, _V, C.c_int]),
+    ("orc_pert_samples_ua", None, [_V, _V, _V, C.c_int, _V, C.c_int]),
+    ("orc_tile_probe_ua", C.c_int, [_V, C.c_int, _V, _V]),
+    ("orc_mlp_nparams", C.c_int, []),
+    ("orc_mlp_forward", None, [_V, _V, C.c_int, _V]),
+    ("orc_mlp_grad", C.c_double, [_V, _V, _V, C.c_int, _V]),
     ("orc_sa_start", C.c_int, [_V, C.c_int, C.c_int, C.POINTER(C.c_int), _V, C.POINTER(OEscape)]),
     ("orc_pixel_eps", None, [_V, C.c_int, C.c_int, _V]),
     ("orc_pixel_coord_d", None, [_V, C.c_int, C.c_int, _V]),
@@ -169,6 +174,22 @@ class OView:
         fn(self.v, rc.ctypes.data, len(rc), out.ctypes.data, nthreads)
         return out
 
+    def pert_samples_ua(self, rc, ua, nthreads=0):
+        """The GPU method with a given first unstable index per sample."""
+        rc = np.ascontiguousarray(rc, dtype=np.int32).reshape(-1, 2)
+        ua = np.ascontiguousarray(ua, dtype=np.int32)
+        out = np.zeros(len(rc), dtype=ESCAPE_DTYPE)
+        self.L.orc_pert_samples_ua(self.v, rc.ctypes.data, ua.ctypes.data, len(rc), out.ctypes.data, nthreads)
+        return out
+
+    def tile_probe_ua(self, T):
+        ntc, ntr = (self.nc + T - 1) // T, (self.nr + T - 1) // T
+        ua = np.zeros(ntc * ntr, dtype=np.int32)
+        rc = np.zeros(2 * ntc * ntr, dtype=np.int32)
+        n = self.L.orc_tile_probe_ua(self.v, T, ua.ctypes.data, rc.ctypes.data)
+        assert n == ntc * ntr
+        return ua.reshape(ntr, ntc), rc.reshape(ntr, ntc, 2)
+
     def sa_start(self, r, c):
         found = C.c_int()
         d = np.zeros(2)
@@ -189,6 +210,23 @@ class OView:
         im = C.create_string_buffer(4096)
         self.L.orc_get_center(self.v, re, im, 4096)
         return re.value.decode(), im.value.decode()
+
+
+def mlp_forward(params, x):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros(len(x))
+    lib().orc_mlp_forward(params.ctypes.data, x.ctypes.data, len(x), y.ctypes.data)
+    return y
+
+
+def mlp_grad(params, x, t):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(lib().orc_mlp_nparams())
+    loss = lib().orc_mlp_grad(params.ctypes.data, x.ctypes.data, t.ctypes.data, len(x), g.ctypes.data)
+    return loss, g
 
 
 def point_hw(re, im, N=1024, bits=64):

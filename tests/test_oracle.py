@@ -226,3 +226,78 @@ def test_method_restatement_matches_reference_method():
     a = o.render_samples(rc)
     b = o.render_samples(rc, pert=True)
     assert np.array_equal(a["iterations"], b["iterations"])
+
+
+# ---- the learned skip predictor's oracle (oracle/mlp.c) and the tile probes ----
+
+def _mlp_offsets():
+    IN, H = 132, 64
+    oW1 = 0
+    oB1 = oW1 + H * IN
+    oW2 = oB1 + H
+    oB2 = oW2 + H * H
+    oW3 = oB2 + H
+    oB3 = oW3 + H
+    return IN, H, oW1, oB1, oW2, oB2, oW3, oB3
+
+
+def test_mlp_oracle_forward_closed_form():
+    """W1 routes input j to unit j, W2 = I, W3 = 1, b3 = 0.5: y = 0.5 + sum_j relu(x_j)."""
+    IN, H, oW1, oB1, oW2, oB2, oW3, oB3 = _mlp_offsets()
+    p = np.zeros(O.lib().orc_mlp_nparams(), dtype=np.float32)
+    for j in range(H):
+        p[oW1 + j * IN + j] = 1.0
+        p[oW2 + j * H + j] = 1.0
+        p[oW3 + j] = 1.0
+    p[oB3] = 0.5
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(7, IN)).astype(np.float32)
+    y = O.mlp_forward(p, x)
+    want = 0.5 + np.maximum(x[:, :H].astype(np.float64), 0.0).sum(axis=1)
+    assert np.allclose(y, want, rtol=0, atol=1e-12)
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The oracle's reverse-mode gradient against central differences of its own
+    forward pass (double), on every layer's weights and biases."""
+    IN, H, oW1, oB1, oW2, oB2, oW3, oB3 = _mlp_offsets()
+    rng = np.random.default_rng(7)
+    p = (rng.normal(size=O.lib().orc_mlp_nparams()) * 0.2).astype(np.float32)
+    x = rng.normal(size=(9, IN)).astype(np.float32)
+    t = rng.normal(size=9).astype(np.float32)
+    loss, g = O.mlp_grad(p, x, t)
+
+    def L(q):
+        y = O.mlp_forward(q, x)
+        return np.mean((y - t.astype(np.float64)) ** 2)
+
+    assert abs(loss - L(p)) < 1e-12 * max(1.0, loss)
+    idx = list(rng.choice(oB1, 12, replace=False)) + [oB1 + 3, oW2 + 5 * H + 9, oB2 + 7, oW3 + 11, oB3]
+    for i in idx:
+        h = np.float32(1e-3)
+        qp = p.copy()
+        qm = p.copy()
+        qp[i] += h
+        qm[i] -= h
+        fd = (L(qp) - L(qm)) / (np.float64(qp[i]) - np.float64(qm[i]))
+        assert abs(fd - g[i]) <= 2e-4 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_tile_probe_is_the_worst_sample_of_its_tile():
+    """The probe (the corner farthest from the reference point) stops the
+    reference's scan first: every sample of the tile has a first unstable index
+    >= the probe's (0 = never), by brute force over the tile."""
+    v = json.load(open(os.path.join(GOLD, "views.json")))["t13"]
+    o = O.OView(40, 56)
+    o.set_view(v["N"], v["re"], v["im"], "3e-24")
+    o.precompute()
+    ua, rc = o.tile_probe_ua(16)
+    for tr in range(ua.shape[0]):
+        for tc in range(ua.shape[1]):
+            pr, pc = rc[tr, tc]
+            u_probe = o.sa_start(int(pr), int(pc))[0]
+            assert u_probe == ua[tr, tc]
+            for r in range(tr * 16, min(40, tr * 16 + 16), 5):
+                for c in range(tc * 16, min(56, tc * 16 + 16), 5):
+                    u = o.sa_start(r, c)[0]
+                    assert u == 0 or (ua[tr, tc] != 0 and u >= ua[tr, tc])
