@@ -54,9 +54,6 @@ struct SaReferenceHost {
   std::vector<double> BX, BN;  // max / min log2 |B_j|^2
   std::vector<double> CX, CN;  // max / min log2 |C_j|^2
   std::vector<int32_t> SP;     // count of special j (zero or non-finite B_j or C_j)
-  // floatexp copies for the FP32X kernel: mantissa pairs and shared exponent
-  std::vector<float> Zf;       // 2*(len+2) Zd as float with per-entry exponent in Ze
-  std::vector<int32_t> Ze;     // len+2
 };
 
 struct RenderOpts {

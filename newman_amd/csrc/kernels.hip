@@ -904,10 +904,75 @@ __device__ __forceinline__ void pert_block(double& dr, double& di, const double 
   rb += moved ? 1u : 0u;
 }
 
-// FP64 perturbation with rebasing (replaces the HP continuation of
-// mandelbrot.cpp:209-224).  Persistent waves, lanes refilled from 64-sample
-// chunks; blocks of K steps with the block's orbit entries loaded up front.
-template <int K>
+// The FP32X form of pert_block: the delta is w * 2^ex with an FP32 mantissa
+// pair w and an int32 exponent ex shared by re and im; es = e * 2^-ex.  Per step,
+// in FP32: d = w * 2^ex (0 below the FP32 range), Y = Z + d, the rebase test
+// |Y|^2 < |d|^2, W = Z + Y and w' = w W + es.  In FP64: the escape test when the
+// FP32 |Y|^2 comes within 2^-10 of the bailout (Y = Zd + w * 2^ex, the reference's
+// sqMag), and every rebase (d = Y from the FP64 Zd and delta, renormalised).
+__device__ __forceinline__ void fx_normalise(double yr, double yi, double er, double ei, float& wr, float& wi,
+                                             int& ex, float& esr, float& esi) {
+  const double mx = fmax(fabs(yr), fabs(yi));
+  if (mx > 0.0) ex = ilogb(mx);
+  wr = (float)ldexp(yr, -ex);
+  wi = (float)ldexp(yi, -ex);
+  esr = (float)ldexp(er, -ex);
+  esi = (float)ldexp(ei, -ex);
+}
+
+template <int K, bool END>
+__device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, float& esr, float& esi, const double er,
+                                              const double ei, int& m, const int it, const double2* zb,
+                                              const double2* zhead, const int endj, const int mend, const double b2,
+                                              const float b2g, int& esc, double& esc_r2, unsigned int& rb) {
+  const float qnanf = __int_as_float(0x7fc00000);
+  bool moved = false;
+  unsigned int hoff = 0;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const double2 zh = *(const double2*)((const char*)zhead + hoff);
+    const double zrd = moved ? zh.x : zb[j].x;
+    const double zid = moved ? zh.y : zb[j].y;
+    const float zr = (float)zrd, zi = (float)zid;
+    const float dr = ldexpf(wr, ex), di = ldexpf(wi, ex);
+    const float yr = zr + dr, yi = zi + di;
+    const float r2 = fmaf(yr, yr, yi * yi);
+    if (r2 > b2g) {  // near the bailout: the reference's test in FP64
+      const double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
+      const double r2d = ydr * ydr + ydi * ydi;
+      if (r2d > b2) {  // first escape: NaN in wr reaches every later step
+        esc = it + j;
+        esc_r2 = r2d;
+        wr = qnanf;
+      }
+    }
+    const float dmag = fmaf(dr, dr, di * di);
+    bool reb = r2 < dmag;
+    if (END) reb = reb || (moved ? (int)hoff == 16 * mend : j == endj);
+    float Wr = zr + yr, Wi = zi + yi;
+    if (reb) {  // rebase in FP64: d = Y against Zd[0] = 0, so W = Y
+      const double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
+      fx_normalise(ydr, ydi, er, ei, wr, wi, ex, esr, esi);
+      Wr = (float)ydr;
+      Wi = (float)ydi;
+      hoff = 0;
+      moved = true;
+    }
+    const float nwr = fmaf(wr, Wr, fmaf(-wi, Wi, esr));
+    const float nwi = fmaf(wr, Wi, fmaf(wi, Wr, esi));
+    wr = nwr;
+    wi = nwi;
+    hoff += 16;
+  }
+  m = moved ? (int)(hoff >> 4) : m + K;
+  rb += moved ? 1u : 0u;
+}
+
+// FP64 (FX = false) or FP32X (FX = true) perturbation with rebasing (replaces the
+// HP continuation of mandelbrot.cpp:209-224).  Persistent waves, lanes refilled
+// from 64-sample chunks; blocks of K steps with the block's orbit entries loaded
+// up front.
+template <int K, bool FX>
 __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
   const Common& c = a.c;
   const uint32_t total = *c.nactive;
@@ -936,6 +1001,8 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
   bool act = false;
   uint32_t pix = 0;
   double dr = 0.0, di = 0.0, er = 0.0, ei = 0.0;
+  float wr = 0.0f, wi = 0.0f, esr = 0.0f, esi = 0.0f;  // FP32X delta and scaled epsilon
+  int ex = 0;
   int it = 0, m = 0;
   for (;;) {
     unsigned long long need = __ballot(!act);
@@ -956,6 +1023,13 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
         it = g4;
         m = g5;
         act = true;
+        if (FX) {
+          ex = 0;
+          const double mx = fmax(fabs(dr), fabs(di));
+          const double me = fmax(fabs(er), fabs(ei));
+          if (mx == 0.0 && me > 0.0) ex = ilogb(me);
+          fx_normalise(dr, di, er, ei, wr, wi, ex, esr, esi);
+        }
       }
       pos += take;
       if (pos == ncur) {  // rotate: NEXT becomes CUR, the far chunk is decoded
@@ -978,6 +1052,8 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
     if (__ballot(act) == 0) break;
     if (!act) {  // idle lanes run the block on a harmless state
       dr = di = er = ei = 0.0;
+      wr = wi = esr = esi = 0.0f;
+      ex = 0;
       m = 0;
       it = 0;
     }
@@ -990,10 +1066,31 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
     double esc_r2 = 0.0;
     unsigned int rb = 0;
     const int it0 = it;
-    if (end_zone)
+    if (FX) {
+      const float b2g = (float)(c.b2 * (1.0 - 1.0 / 1024.0));
+      if (end_zone)
+        pert_block_fx<K, true>(wr, wi, ex, esr, esi, er, ei, m, it, zb, zhead, mend - m, mend, c.b2, b2g, esc,
+                               esc_r2, rb);
+      else
+        pert_block_fx<K, false>(wr, wi, ex, esr, esi, er, ei, m, it, zb, zhead, 0, mend, c.b2, b2g, esc, esc_r2,
+                                rb);
+      // keep the mantissa near 1: |w| grows by at most |W| per step
+      const float mx = fmaxf(fabsf(wr), fabsf(wi));
+      if (mx > 0.0f && mx < INFINITY) {
+        const int k = ilogbf(mx);
+        if (k > 8 || k < -8) {
+          wr = ldexpf(wr, -k);
+          wi = ldexpf(wi, -k);
+          ex += k;
+          esr = (float)ldexp(er, -ex);
+          esi = (float)ldexp(ei, -ex);
+        }
+      }
+    } else if (end_zone) {
       pert_block<K, true>(dr, di, er, ei, m, it, zb, zhead, mend - m, mend, c.b2, esc, esc_r2, rb);
-    else
+    } else {
       pert_block<K, false>(dr, di, er, ei, m, it, zb, zhead, 0, mend, c.b2, esc, esc_r2, rb);
+    }
     if (act) {
       it = it0 + K;
       const bool escaped = esc >= 0 && esc <= N - 1;
@@ -1574,14 +1671,21 @@ class EngineImpl : public Engine {
     }
     const int K = opt.block_k > 0 ? opt.block_k : 8;
     const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
+    const bool fx = opt.delta_mode == DELTA_FP32X;
     if (stats) {
       stats->block_k = K;
       stats->waves_per_simd = waves;
     }
     switch (K) {
-      case 8: return run(a, k_series_start, k_series_iterate<8>, waves, opt, stats, nout, host_out, emit_to);
-      case 16: return run(a, k_series_start, k_series_iterate<16>, waves, opt, stats, nout, host_out, emit_to);
-      case 32: return run(a, k_series_start, k_series_iterate<32>, waves, opt, stats, nout, host_out, emit_to);
+      case 8:
+        return fx ? run(a, k_series_start, k_series_iterate<8, true>, waves, opt, stats, nout, host_out, emit_to)
+                  : run(a, k_series_start, k_series_iterate<8, false>, waves, opt, stats, nout, host_out, emit_to);
+      case 16:
+        return fx ? run(a, k_series_start, k_series_iterate<16, true>, waves, opt, stats, nout, host_out, emit_to)
+                  : run(a, k_series_start, k_series_iterate<16, false>, waves, opt, stats, nout, host_out, emit_to);
+      case 32:
+        return fx ? run(a, k_series_start, k_series_iterate<32, true>, waves, opt, stats, nout, host_out, emit_to)
+                  : run(a, k_series_start, k_series_iterate<32, false>, waves, opt, stats, nout, host_out, emit_to);
       default: return NM_ERR_ARG;
     }
   }

@@ -392,8 +392,6 @@ int Mandelbrot::uploadReference() {
   const int len = ref_.len;
   ref_.Zd.assign(2 * (size_t)(len + 2), 0.0);
   ref_.Xlo.assign(2 * (size_t)len, 0.0);
-  ref_.Zf.assign(2 * (size_t)(len + 2), 0.0f);
-  ref_.Ze.assign((size_t)(len + 2), 0);
   for (int k = 0; k <= len; k++) {
     for (int p = 0; p < 2; p++) {
       const hpf& x = p ? X[k].im : X[k].re;
@@ -406,16 +404,6 @@ int Mandelbrot::uploadReference() {
         ref_.Xlo[2 * k + p] = get_d_nearest(rem);
       }
     }
-  }
-  // floatexp orbit: Z = (fr, fi) * 2^e with max(|fr|, |fi|) in [0.5, 1)
-  for (int k = 0; k < len + 2; k++) {
-    double zr = ref_.Zd[2 * k], zi = ref_.Zd[2 * k + 1];
-    double m = std::max(std::fabs(zr), std::fabs(zi));
-    int e = 0;
-    if (m > 0) std::frexp(m, &e);
-    ref_.Ze[k] = e;
-    ref_.Zf[2 * k] = (float)std::ldexp(zr, -e);
-    ref_.Zf[2 * k + 1] = (float)std::ldexp(zi, -e);
   }
   buildThresholds();
   have_ref_ = true;

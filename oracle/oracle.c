@@ -825,6 +825,108 @@ static orc_escape pert_from(const pert_ctx* P, double dr, double di, double er, 
   return esc;
 }
 
+/* The FP32X form (mantissa pair w in float, shared int exponent ex; es = e * 2^-ex)
+ * of pert_from, as the HIP kernel runs it: the first step (at it0) in FP64 like
+ * pert_from, then per step in float d = w 2^ex, Y = Z + d, the rebase test, W and
+ * w' = w W + es; the escape test in double when the float |Y|^2 exceeds
+ * b2 (1 - 2^-10) (Y = Zd + w 2^ex, sqMag); every rebase in double (d = Y,
+ * renormalised); and after every K steps of a lane's blocks the mantissa is
+ * renormalised when its exponent leaves [-8, 8].  The K-step schedule starts at
+ * it0 + 1, where the kernel picks the sample up. */
+static void fx_normalise(double yr, double yi, double er, double ei, float* wr, float* wi, int* ex, float* esr,
+                         float* esi) {
+  double mx = fmax(fabs(yr), fabs(yi));
+  if (mx > 0.0) *ex = ilogb(mx);
+  *wr = (float)ldexp(yr, -*ex);
+  *wi = (float)ldexp(yi, -*ex);
+  *esr = (float)ldexp(er, -*ex);
+  *esi = (float)ldexp(ei, -*ex);
+}
+
+static orc_escape pert_from_fx(const pert_ctx* P, double dr, double di, double er, double ei, int it0, int K) {
+  orc_escape esc;
+  int N = P->N;
+  if (it0 + 1 >= N) {
+    esc.iterations = N;
+    esc.smoothing = 0.0f;
+    return esc;
+  }
+  /* the first step, FP64 (the start kernel) */
+  int m = it0 + 1;
+  {
+    double zr = P->Zd[2 * m], zi = P->Zd[2 * m + 1];
+    double yr = zr + dr, yi = zi + di;
+    double r2 = yr * yr + yi * yi;
+    double dmag = fma(dr, dr, di * di);
+    double wr = zr + yr, wi = zi + yi;
+    if (r2 < dmag || m == P->mend) {
+      dr = yr;
+      di = yi;
+      wr = yr;
+      wi = yi;
+      m = 0;
+    }
+    double ndr = fma(dr, wr, fma(-di, wi, er));
+    double ndi = fma(dr, wi, fma(di, wr, ei));
+    dr = ndr;
+    di = ndi;
+    m++;
+  }
+  float wr, wi, esr, esi;
+  int ex = 0;
+  if (fmax(fabs(dr), fabs(di)) == 0.0 && fmax(fabs(er), fabs(ei)) > 0.0) ex = ilogb(fmax(fabs(er), fabs(ei)));
+  fx_normalise(dr, di, er, ei, &wr, &wi, &ex, &esr, &esi);
+  const double b2 = P->b2;
+  const float b2g = (float)(b2 * (1.0 - 1.0 / 1024.0));
+  for (int it = it0 + 1;; it++) {
+    double zrd = P->Zd[2 * m], zid = P->Zd[2 * m + 1];
+    float zr = (float)zrd, zi = (float)zid;
+    float fdr = ldexpf(wr, ex), fdi = ldexpf(wi, ex);
+    float yr = zr + fdr, yi = zi + fdi;
+    float r2 = fmaf(yr, yr, yi * yi);
+    if (r2 > b2g) {
+      double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
+      double r2d = ydr * ydr + ydi * ydi;
+      if (r2d > b2) {
+        esc.iterations = it;
+        esc.smoothing = (float)smoothing_of(r2d, P->bailout);
+        return esc;
+      }
+    }
+    if (it == N - 1) break;
+    float dmag = fmaf(fdr, fdr, fdi * fdi);
+    float Wr = zr + yr, Wi = zi + yi;
+    if (r2 < dmag || m == P->mend) {
+      double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
+      fx_normalise(ydr, ydi, er, ei, &wr, &wi, &ex, &esr, &esi);
+      Wr = (float)ydr;
+      Wi = (float)ydi;
+      m = 0;
+    }
+    float nwr = fmaf(wr, Wr, fmaf(-wi, Wi, esr));
+    float nwi = fmaf(wr, Wi, fmaf(wi, Wr, esi));
+    wr = nwr;
+    wi = nwi;
+    m++;
+    if ((it - it0) % K == 0) { /* a block ended: renormalise the mantissa */
+      float mx = fmaxf(fabsf(wr), fabsf(wi));
+      if (mx > 0.0f && mx < INFINITY) {
+        int k = ilogbf(mx);
+        if (k > 8 || k < -8) {
+          wr = ldexpf(wr, -k);
+          wi = ldexpf(wi, -k);
+          ex += k;
+          esr = (float)ldexp(er, -ex);
+          esi = (float)ldexp(ei, -ex);
+        }
+      }
+    }
+  }
+  esc.iterations = N;
+  esc.smoothing = 0.0f;
+  return esc;
+}
+
 static double* make_zd(const orc_view* v) {
   int len = v->len;
   double* Zd = (double*)malloc(sizeof(double) * 2 * (size_t)(len + 2));
@@ -839,7 +941,8 @@ static double* make_zd(const orc_view* v) {
 /* the GPU method for one sample: reference SA start (scan), SA-escape binary
  * search with double-double truncation, then pert_from.  ua_given >= 0 replaces
  * the sample's own scan by a given first unstable index (a tile's, 0 = none). */
-static orc_escape pert_sample(const orc_view* v, const pert_ctx* P, const ohp* y0r, const ohp* y0i, int ua_given) {
+static orc_escape pert_sample(const orc_view* v, const pert_ctx* P, const ohp* y0r, const ohp* y0i, int ua_given,
+                              int fx_k) {
   orc_escape esc;
   const int N = v->N;
   const double b2 = v->bailout * v->bailout;
@@ -883,6 +986,7 @@ static orc_escape pert_sample(const orc_view* v, const pert_ctx* P, const ohp* y
     esc.smoothing = (float)smoothing_of(lp_sqmag(y), v->bailout);
     return esc;
   }
+  if (fx_k > 0) return pert_from_fx(P, d.re, d.im, eps.re, eps.im, found, fx_k);
   return pert_from(P, d.re, d.im, eps.re, eps.im, found);
 }
 
@@ -891,11 +995,12 @@ typedef struct {
   orc_view* v;
   const int* rc;
   orc_escape* out;
-  int mode; /* 0 reference, 1 pert64 */
+  int mode; /* 0 reference, 1 pert64, 2 pert FP32X */
   pert_ctx P;
   int ncols; /* for full rows */
   const int* rows;
   const int* ua; /* per-sample first unstable index (tile skips), or NULL */
+  int fx_k;      /* mode 2: the kernel's block size */
 } render_ctx;
 
 static orc_escape sample_value(render_ctx* R, int r, int c, long i) {
@@ -904,7 +1009,7 @@ static orc_escape sample_value(render_ctx* R, int r, int c, long i) {
   pixel_coord(v, r, c, &re, &im);
   if (orc_use_hardware(v)) return hw_iterations(&re, &im, v->n, v->N, v->bailout);
   if (R->mode == 0) return sa_iterations(v, &re, &im);
-  return pert_sample(v, &R->P, &re, &im, R->ua ? R->ua[i] : -1);
+  return pert_sample(v, &R->P, &re, &im, R->ua ? R->ua[i] : -1, R->mode == 2 ? R->fx_k : 0);
 }
 
 static void samples_fn(void* cp, long i) {
@@ -933,7 +1038,7 @@ static void setup_pert(orc_view* v, render_ctx* R) {
 }
 
 void orc_render_rows(orc_view* v, const int* rows, int nrows, orc_escape* out, int nthreads) {
-  render_ctx R = {v, NULL, out, 0, {0}, 0, NULL, NULL};
+  render_ctx R = {v, NULL, out, 0, {0}, 0, NULL, NULL, 0};
   R.ncols = v->nc;
   R.rows = rows;
   setup_pert(v, &R);
@@ -942,14 +1047,22 @@ void orc_render_rows(orc_view* v, const int* rows, int nrows, orc_escape* out, i
 }
 
 void orc_render_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nthreads) {
-  render_ctx R = {v, rc, out, 0, {0}, 0, NULL, NULL};
+  render_ctx R = {v, rc, out, 0, {0}, 0, NULL, NULL, 0};
   setup_pert(v, &R);
   parallel_for(n, nthreads, 4, samples_fn, &R);
   free((void*)R.P.Zd);
 }
 
 void orc_pert_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nthreads) {
-  render_ctx R = {v, rc, out, 1, {0}, 0, NULL, NULL};
+  render_ctx R = {v, rc, out, 1, {0}, 0, NULL, NULL, 0};
+  setup_pert(v, &R);
+  parallel_for(n, nthreads, 4, samples_fn, &R);
+  free((void*)R.P.Zd);
+}
+
+/* the GPU method with FP32X deltas (block size K of the kernel) */
+void orc_pert_samples_fx(orc_view* v, const int* rc, int n, int K, orc_escape* out, int nthreads) {
+  render_ctx R = {v, rc, out, 2, {0}, 0, NULL, NULL, K};
   setup_pert(v, &R);
   parallel_for(n, nthreads, 4, samples_fn, &R);
   free((void*)R.P.Zd);
@@ -957,7 +1070,7 @@ void orc_pert_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nt
 
 /* the GPU method with a given first unstable index per sample (tile skips) */
 void orc_pert_samples_ua(orc_view* v, const int* rc, const int* ua, int n, orc_escape* out, int nthreads) {
-  render_ctx R = {v, rc, out, 1, {0}, 0, NULL, ua};
+  render_ctx R = {v, rc, out, 1, {0}, 0, NULL, ua, 0};
   setup_pert(v, &R);
   parallel_for(n, nthreads, 4, samples_fn, &R);
   free((void*)R.P.Zd);
