@@ -15,13 +15,15 @@ iteration kernel (15 per executed perturbation step) over its HIP-event duration
 against the FP64 vector peak.  cpu_baseline = the CPU oracle (the reference method:
 series scan + HP continuation) on a row subsample of the same frame.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c4]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c4|c5]
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -67,6 +69,18 @@ def make_workload(name):
         def step(k, w, reps=1):
             return m.render_frames(widths, k, w, reps, copy_back=False)[1]
         return m, desc, FLOP_PER_PLAIN_STEP, step
+    if name == "c5":
+        v = views["c5_seed1"]
+        m = nm.Mandelbrot(2160, 3840)
+        m.set_view(v["N"], v["re"], v["im"], v["width"])
+        desc = {"workload": "learned-skip configuration: 3840x2160, width 1e-100, N 100000, bailout 1024, tol 1e-10, "
+                            "centre = zoom walk seed 1; the FP64 per-sample path (variants: FP32X deltas, 64x64 "
+                            "tile skips from the probe search and from the learned predictor)",
+                "samples": 3840 * 2160, "N": 100000, "width": "1e-100", "precision_bits": m.precision_bits()}
+
+        def step(k, w, reps=1):
+            return m.render(k, w, reps, copy_back=False)
+        return m, desc, FLOP_PER_PERT_STEP, step
     if name == "c4":
         v = views["c4_seed1"]
         widths = ["%.17e" % (1e-40 * 1.1 ** (C4_FRAMES - 1 - k)) for k in range(C4_FRAMES)]
@@ -131,6 +145,64 @@ def roofline(config, flop, s):
             "ms_per_launch": dom["ms"], "kernels": list(kernels.values())}
 
 
+def train_skipnet(nm, m, T, factors, epochs=120):
+    """The learned tile skip's network, trained on the GPU on the probe searches
+    (the reference's scan at each tile's probe) of the same centre at other widths,
+    all against this view's reference point."""
+    re, im = m.reference_point()
+    cre, cim = m.center
+    width = m.sz_d()[0] * m.cols()
+    xs, ts = [], []
+    for f in factors:
+        t = nm.Mandelbrot(m.rows(), m.cols())
+        t.set_view(m.N, cre, cim, "%.17e" % (width * f))
+        t.precompute_at(re, im)
+        ua, _, _ = t.tile_skips(T, mode=0)
+        xs.append(t.tile_features(T))
+        ts.append(t.tile_targets(ua))
+    x, t = np.concatenate(xs), np.concatenate(ts)
+    net = nm.SkipNet()
+    t0 = time.perf_counter()
+    curve = net.train(x, t, epochs=epochs, batch=256, lr=2e-3, seed=1)
+    return net, {"samples": int(len(t)), "views": len(factors), "epochs": epochs, "loss_first": float(curve[0]),
+                 "loss_last": float(curve[-1]), "train_s": time.perf_counter() - t0}
+
+
+def variants(m, step, args, value64, wall64):
+    """The same frame with FP32X deltas, with per-tile skips (64 x 64 tiles) from
+    the plain probe search (the reference's scan at each tile's probe) and from
+    the learned predictor, and with FP32X + learned skips: credited px-it/s, its
+    ratio to the FP64 per-sample path, and count agreement with it."""
+    import newman_amd as nm
+    T = 64
+    k, w = args.block_k, args.waves
+    m.render(k, w, 1, copy_back=True)
+    base = m.grid()["iterations"].copy()
+    net, train = train_skipnet(nm, m, T, [0.3, 0.5, 0.7, 1.5, 2.2, 3.3])
+    out = {"training": train}
+    for name, delta, skip in (("fp32x", 1, 0), ("fp64_tile_scan", 0, 1), ("fp64_learned_skip", 0, 3),
+                              ("fp32x_learned_skip", 1, 3)):
+        m.set_delta_mode(delta)
+        m.set_skip_mode(skip, T, net if skip else None)
+        step(k, w, 1)  # warm-up
+        t0 = time.perf_counter()
+        st = step(k, w, args.steps)
+        wl = (time.perf_counter() - t0) / args.steps
+        m.render(k, w, 1, copy_back=True)
+        same = float((m.grid()["iterations"] == base).mean())
+        v = {"value": st["credited"] / wl, "ms_per_step": wl * 1e3, "vs_fp64": (st["credited"] / wl) / value64,
+             "ms_iterate": st["ms_iterate"], "ms_start": st["ms_start"], "executed": st["executed"],
+             "counts_equal_fp64": same}
+        if skip:
+            v["tiles"] = m.tile_stats()
+        out[name] = v
+        print("# variant %s: %.3e px-it/s (%.2fx FP64), counts = FP64 on %.4f%%" % (
+            name, v["value"], v["vs_fp64"], 100 * same), file=sys.stderr, flush=True)
+    m.set_delta_mode(0)
+    m.set_skip_mode(0, T, None)
+    return out
+
+
 def tune(step, ks, waves):
     """Sweep block size and persistent waves per SIMD; kernel time by HIP events."""
     best = None
@@ -172,6 +244,18 @@ def cpu_baseline(desc, name, nthreads, rows_every):
                           "640x480 frame, repeated %d times (%.1f s); credited = iterations of non-cardioid samples"
                           % (reps, t_run)}
     views = load_views()
+    if name == "c5":
+        v = views["c5_seed1"]
+        m = nm.Mandelbrot(2160, 3840)
+        m.set_view(v["N"], v["re"], v["im"], v["width"])
+        desc = {"workload": "learned-skip configuration: 3840x2160, width 1e-100, N 100000, bailout 1024, tol 1e-10, "
+                            "centre = zoom walk seed 1; the FP64 per-sample path (variants: FP32X deltas, 64x64 "
+                            "tile skips from the probe search and from the learned predictor)",
+                "samples": 3840 * 2160, "N": 100000, "width": "1e-100", "precision_bits": m.precision_bits()}
+
+        def step(k, w, reps=1):
+            return m.render(k, w, reps, copy_back=False)
+        return m, desc, FLOP_PER_PERT_STEP, step
     if name == "c4":
         v = views["c4_seed1"]
         nr, nc = 720, 1280
@@ -226,6 +310,8 @@ def main():
     ap.add_argument("--pre-iter", type=int, default=0, help="plain path: updates in the start kernel")
     ap.add_argument("--tile", type=int, default=0, help="sample order: 0 row-major, T = T x T tiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variants", type=int, default=1,
+                    help="also time FP32X deltas and the learned tile skip on the same frame (c2, c5)")
     ap.add_argument("--tune", action="store_true", help="sweep block size and waves per SIMD first")
     ap.add_argument("--cpu-rows-every", type=int, default=36)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -298,6 +384,8 @@ def main():
     }
     if tuned:
         line["autotune"] = tuned
+    if args.variants and args.config in ("c2", "c5"):
+        line["variants"] = variants(m, step, args, value, wall)
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(desc, args.config, args.cpu_threads, args.cpu_rows_every)

@@ -123,6 +123,8 @@ int nm_set_view(nm_mandelbrot* h, int N, const char* cre, const char* cim, const
 /* precompute around an explicit reference point instead of findProbe */
 int nm_precompute_at(nm_mandelbrot* h, const char* re, const char* im);
 int nm_reference_length(const nm_mandelbrot* h);  /* len(X) */
+/* the reference point X[0] as decimal strings (precompute first) */
+int nm_reference_point(const nm_mandelbrot* h, char* re, char* im, size_t cap);
 int nm_probe_winner(const nm_mandelbrot* h);      /* index of findProbe's winner, -1 if none */
 /* reference series as the device sees it: X (nearest), A, B, C (truncated),
  * 2*len doubles each (re, im interleaved); any pointer may be NULL */

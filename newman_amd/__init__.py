@@ -185,6 +185,7 @@ _SIGS = [
     ("nm_tile_features", C.c_int, [_H, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     ("nm_tile_skips", C.c_int, [_H, C.c_int, C.c_int, _H, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("nm_tile_targets", C.c_int, [_H, C.c_void_p, C.c_int, C.c_void_p]),
+    ("nm_reference_point", C.c_int, [_H, C.c_char_p, C.c_char_p, C.c_size_t]),
     ("nm_set_skip_mode", C.c_int, [_H, C.c_int, C.c_int, _H]),
     ("nm_get_tile_stats", C.c_int, [_H, C.c_void_p]),
     ("nm_last_error", C.c_char_p, []),
@@ -419,6 +420,12 @@ class Mandelbrot:
         _check(lib().nm_tile_skips(self._h, tile, mode, net._h if net else None, ua.ctypes.data, how.ctypes.data, nt,
                                    C.byref(st)), "nm_tile_skips")
         return ua, how, st.as_dict()
+
+    def reference_point(self):
+        re = C.create_string_buffer(8192)
+        im = C.create_string_buffer(8192)
+        _check(lib().nm_reference_point(self._h, re, im, 8192), "nm_reference_point")
+        return re.value.decode(), im.value.decode()
 
     def tile_targets(self, ua):
         ua = np.ascontiguousarray(ua, dtype=np.int32)

@@ -274,6 +274,19 @@ int nm_precompute_at(nm_mandelbrot* h, const char* re, const char* im) {
 }
 
 int nm_reference_length(const nm_mandelbrot* h) { return h ? h->m.referenceLength() : NM_ERR_ARG; }
+
+int nm_reference_point(const nm_mandelbrot* h, char* re, char* im, size_t cap) {
+  if (!h || !re || !im || cap < 2) return NM_ERR_ARG;
+  return guard([&] {
+    if (!h->m.hasReference()) return (int)NM_ERR_STATE;
+    const nm::HPComplex& x0 = h->m.referencePoint();
+    std::string a = x0.re.to_string(), b = x0.im.to_string();
+    if (a.size() + 1 > cap || b.size() + 1 > cap) return (int)NM_ERR_RANGE;
+    std::memcpy(re, a.c_str(), a.size() + 1);
+    std::memcpy(im, b.c_str(), b.size() + 1);
+    return (int)NM_OK;
+  });
+}
 int nm_probe_winner(const nm_mandelbrot* h) { return h ? h->m.probeWinner() : NM_ERR_ARG; }
 
 int nm_get_reference(const nm_mandelbrot* h, double* X, double* A, double* B, double* C) {
@@ -390,6 +403,7 @@ int nm_skipnet_set(nm_skipnet* net, const float* params, const float* mean, cons
   if (mean) std::copy(mean, mean + nm::SkipNet::kIn, net->net.mean.begin());
   if (scale) std::copy(scale, scale + nm::SkipNet::kIn, net->net.scale.begin());
   net->net.trained = true;
+  net->net.version++;
   return NM_OK;
 }
 

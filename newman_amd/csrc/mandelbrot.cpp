@@ -728,16 +728,20 @@ static double hp_log10(const hpf& v) {
   return m != 0.0 ? std::log10(std::fabs(m)) + (double)e * std::log10(2.0) : -4096.0;
 }
 
-int Mandelbrot::tileFeatures(int T, std::vector<float>& feats) {
+int Mandelbrot::tileFeatures(int T, std::vector<float>& feats, const SaFrameHost* frame) {
   if (!have_ref_ || useHardware()) return NM_ERR_STATE;
   if (T < 8) return NM_ERR_ARG;
   if (taylor_serial_ != ref_serial_) {
     taylor_features(X, ref_.len, taylor_feats_);
     taylor_serial_ = ref_serial_;
   }
-  SaFrameHost f;
-  int st = buildSaFrame(f);
-  if (st != NM_OK) return st;
+  SaFrameHost own;
+  if (!frame) {
+    int st = buildSaFrame(own);
+    if (st != NM_OK) return st;
+    frame = &own;
+  }
+  const SaFrameHost& f = *frame;
   hpf w(sz.re.L);
   hp_mul_si(w, sz.re, (long)cols());
   const double lw = hp_log10(w);
@@ -821,15 +825,23 @@ int Mandelbrot::tileSkipsFor(Engine* eng, const std::vector<const SaFrameHost*>&
     if (!skipnet || !skipnet->trained) return NM_ERR_STATE;
     if (frames.size() != 1) return NM_ERR_ARG;  // the features describe this object's view
     auto t0 = std::chrono::steady_clock::now();
-    std::vector<float> feats;
-    int st = tileFeatures(skip_tile, feats);
-    if (st != NM_OK) return st;
-    const size_t nt = feats.size() / SkipNet::kIn;
-    std::vector<float> y(nt);
-    if ((st = skipnet->predict(feats.data(), nt, y.data())) != NM_OK) return NM_ERR_HIP;
-    ensureRecords();
-    ts.pred.resize(nt);
-    for (size_t t = 0; t < nt; t++) ts.pred[t] = snapIndex((double)y[t]);
+    // the predictions are a function of the view and the network: kept until either changes
+    const std::string key = snapshot() + "|" + std::to_string((uintptr_t)skipnet) + "|" +
+                            std::to_string(skipnet->version) + "|" + std::to_string(skip_tile);
+    if (key != pred_key_) {
+      pred_key_.clear();
+      std::vector<float> feats;
+      int st = tileFeatures(skip_tile, feats, frames[0]);
+      if (st != NM_OK) return st;
+      const size_t nt = feats.size() / SkipNet::kIn;
+      std::vector<float> y(nt);
+      if ((st = skipnet->predict(feats.data(), nt, y.data())) != NM_OK) return NM_ERR_HIP;
+      ensureRecords();
+      pred_.resize(nt);
+      for (size_t t = 0; t < nt; t++) pred_[t] = snapIndex((double)y[t]);
+      pred_key_ = key;
+    }
+    ts.pred = pred_;
     tile_stats.ms_predict = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ts.mode = 2;
   } else {

@@ -99,6 +99,7 @@ class Mandelbrot {
   int referenceLength() const { return (int)ref_.len; }
   int probeWinner() const { return winner_; }
   bool hasReference() const { return have_ref_; }
+  const HPComplex& referencePoint() const { return X[0]; }
   const SaReferenceHost& reference() const { return ref_; }
   RenderGrid& grid_ref() { return grid; }
   const RenderStats& lastStats() const { return stats_; }
@@ -122,7 +123,9 @@ class Mandelbrot {
   // tile, tiles row-major): log10 width; the probe's offset from the reference
   // point in widths (re, im); log2 |a_k(i)| of the 8-term Taylor series of the
   // delta in epsilon (k = 1..8) at 16 orbit indices; log10 N
-  int tileFeatures(int T, std::vector<float>& feats);
+  int tileFeatures(int T, std::vector<float>& feats, const SaFrameHost* frame = nullptr);
+  std::vector<int> pred_;  // the last snapped predictions and their key
+  std::string pred_key_;
   // per-tile first unstable index at the probes; mode 0 scan, 1 tables, 2 learned
   int tileSkips(int T, int mode, SkipNet* net, std::vector<int>& ua, std::vector<int>* how);
   // the learned network's target for a first unstable index, and back

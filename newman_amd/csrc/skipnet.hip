@@ -1,9 +1,10 @@
 // skipnet.hip - the learned series-skip predictor: a 132-64-64-1 MLP (ReLU, FP32)
 // with hand-written HIP forward, backward and Adam kernels, and its training
-// loop.  It predicts, per 64 x 64 tile, the first index at which the series
-// approximation becomes unstable at the tile's probe point (as a fraction of the
-// reference orbit's length); the render side validates every prediction with the
-// reference's stability test and falls back to the plain probe search.
+// loop.  It predicts, per tile, the first index at which the series approximation
+// becomes unstable at the tile's probe point (as the scaled rank of that index
+// among the candidate indices of the reference's thresholds); the render side
+// validates every prediction with the reference's stability test and falls back
+// to the plain probe search.
 //
 // Parameters, one flat FP32 array (kParams = 12737):
 //   W1[64][132], b1[64], W2[64][64], b2[64], W3[64], b3
@@ -341,6 +342,7 @@ int SkipNet::train(const float* raw, const float* t, size_t n, const TrainOpts& 
   }
   SN_HIP(hipMemcpy(params.data(), dP, kParams * 4, hipMemcpyDeviceToHost));
   trained = true;
+  version++;
   return NM_OK;
 }
 

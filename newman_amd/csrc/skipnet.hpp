@@ -42,6 +42,7 @@ class SkipNet {
   std::vector<float> params;        // W1, b1, W2, b2, W3, b3
   std::vector<float> mean, scale;   // feature standardisation
   bool trained = false;
+  uint64_t version = 0;  // bumped whenever the parameters change
 
  private:
   int reserve(size_t nx, size_t nb);

@@ -22,6 +22,8 @@ CASES = [("t13", 48, 64, "1e-25"), ("t13", 48, 64, "3e-23"), ("t11", 48, 64, "1e
 # error grows with the length of the orbit after a sample leaves the reference
 # (N 3000 for t11, 8000 for t13, 60000 for c4_seed1)
 MIN_AGREE = {"t13": 0.999, "t11": 0.995, "c4_seed1": 0.97}
+# the same for smoothing within 1e-3 where the counts agree
+MIN_SMOOTH = {"t13": 0.98, "t11": 0.97, "c4_seed1": 0.90}
 
 
 def _pair(name, nr, nc, width):
@@ -56,8 +58,8 @@ def test_fp32x_error_against_fp64_and_reference(gpu, name, nr, nc, width):
     """The per-sample error of FP32X deltas, reported and bounded: counts equal to
     the FP64-delta kernel and to the reference method (SA + HP continuation) on
     at least MIN_AGREE of the samples; where counts agree the smoothing differs by
-    < 1e-3 on at least 97% of them (long orbits after the reference shift the
-    escape radius more than the count)."""
+    < 1e-3 on at least MIN_SMOOTH of them (long orbits after the reference shift
+    the escape radius more than the count)."""
     m, o = _pair(name, nr, nc, width)
     m.render()
     g64 = m.grid().copy()
@@ -76,4 +78,4 @@ def test_fp32x_error_against_fp64_and_reference(gpu, name, nr, nc, width):
     bar = MIN_AGREE[name]
     assert same64.mean() >= bar
     assert sameref.mean() >= bar
-    assert (ds < 1e-3).mean() >= min(bar, 0.97)
+    assert (ds < 1e-3).mean() >= MIN_SMOOTH[name]
