@@ -54,6 +54,12 @@ struct SaReferenceHost {
   std::vector<double> BX, BN;  // max / min log2 |B_j|^2
   std::vector<double> CX, CN;  // max / min log2 |C_j|^2
   std::vector<int32_t> SP;     // count of special j (zero or non-finite B_j or C_j)
+  // For views whose series terms under/overflow in the exact test (widths below
+  // about 1e-90): the thresholds T_j themselves, their sparse table of range
+  // minima (level k at [k * len], minima over [i, i + 2^k)), and suffix minima of
+  // log2 |B_j|^2 and log2 |C_j|^2 (over j >= i).
+  std::vector<double> T, ST, BNs, CNs;
+  int st_levels = 0;
 };
 
 struct RenderOpts {

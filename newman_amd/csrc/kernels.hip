@@ -125,6 +125,10 @@ struct SeriesArgs {
   const double* CX;
   const double* CN;
   const int* SP;
+  const double* ST;   // range-minimum sparse table of the thresholds (st_levels x len)
+  const double* BNs;  // suffix minima of log2 |B_j|^2, |C_j|^2
+  const double* CNs;
+  int st_levels;
   int len;
   int mend;  // len + 1
   double tol;
@@ -653,6 +657,48 @@ __device__ __forceinline__ int2 wave_stop_bracket(const SeriesArgs& a, double L,
   return make_int2(lo, min(hi, last));
 }
 
+// last i in [1, last] with the non-decreasing CX[i] < v, or 0
+__device__ __forceinline__ int last_lt(const double* CX, int last, double v) {
+  int l = 1, h = last + 1;  // first index with CX >= v
+  while (l < h) {
+    const int mid = (l + h) >> 1;
+    if (CX[mid] >= v) h = mid;
+    else l = mid + 1;
+  }
+  return l - 1;
+}
+
+// first i >= lo with T_i <= v (last + 1 if none), by the sparse table
+__device__ __forceinline__ int first_le_from(const SeriesArgs& a, int lo, double v) {
+  const int len = a.len;
+  int pos = lo;
+  for (int k = a.st_levels - 1; k >= 0; k--)
+    if (pos + (1 << k) <= len && a.ST[(size_t)k * len + pos] > v) pos += 1 << k;
+  return pos;
+}
+
+// Where the reference's exact scan must start when the prefix certificate of
+// sa_stop_index fails (its terms under- or overflow somewhere before the stop).
+// Indices whose |C_j e^3|^2 lies below 2^-1080 are stable whatever their
+// threshold says: cmag rounds to 0 and bmag tol < 0 is false; that holds for a
+// whole prefix [1, ic] (CX is a prefix maximum, no special index inside).  From
+// ic + 1 on, the threshold table applies where every exact-test intermediate of
+// the range [ic + 1, ia) stays normal with margin (suffix minima from ic + 1,
+// prefix maxima up to ia - 1; e^3 no more than 30 bits into the subnormals).
+__device__ int deep_start(const SeriesArgs& a, double L) {
+  const int last = a.len - 1;
+  int ic = last_lt(a.CX, last, -1080.0 - 3.0 * L);
+  if (ic > 0 && a.SP[ic] != 0) ic = 0;
+  const int lo = ic + 1;
+  if (lo > last || !(L > -696.0 && L < 600.0)) return lo;
+  const int ia = first_le_from(a, lo, L + kMarg);
+  const int sidx = min(ia, last + 1) - 1;  // the last index the table certifies
+  if (sidx < lo) return lo;
+  const bool ok = a.SP[sidx] - a.SP[lo - 1] == 0 && a.BNs[lo] + 2.0 * L > -900.0 && a.CNs[lo] + 3.0 * L > -960.0 &&
+                  a.BX[sidx] + 2.0 * L < 960.0 && a.CX[sidx] + 3.0 * L < 960.0;
+  return ok ? ia : lo;
+}
+
 __device__ int sa_stop_index(const SeriesArgs& a, double L, int2 br, double2 e2, double2 e3) {
   const int last = a.len - 1;
   if (last < 1) return 0;
@@ -665,7 +711,7 @@ __device__ int sa_stop_index(const SeriesArgs& a, double L, int2 br, double2 e2,
     safe = a.SP[safe_idx] == 0 && a.BX[safe_idx] + 2.0 * L < 960.0 && a.BN[safe_idx] + 2.0 * L > -900.0 &&
            a.CX[safe_idx] + 3.0 * L < 960.0 && a.CN[safe_idx] + 3.0 * L > -960.0;
   }
-  int from = safe ? ia : 1;
+  int from = safe ? ia : deep_start(a, L);
   for (int i = from; i <= last; i++)
     if (unstable_at(a, i, e2, e3)) return i;
   return 0;
@@ -1221,6 +1267,8 @@ class EngineImpl : public Engine {
   const double2 *rZd = nullptr, *rXlo = nullptr, *rA = nullptr, *rB = nullptr, *rC = nullptr;
   const double *rPM = nullptr, *rBX = nullptr, *rBN = nullptr, *rCX = nullptr, *rCN = nullptr;
   const int* rSP = nullptr;
+  const double *rST = nullptr, *rBNs = nullptr, *rCNs = nullptr;
+  int rst_levels = 0;
 
   int fail(const char* what, hipError_t e) {
     err = std::string(what) + ": " + hipGetErrorString(e);
@@ -1272,7 +1320,8 @@ class EngineImpl : public Engine {
     size_t oZ = place(nz * 16), oXlo = place((size_t)len * 16), oA = place((size_t)len * 16),
            oB = place((size_t)len * 16), oC = place((size_t)len * 16), oPM = place((size_t)len * 8),
            oBX = place((size_t)len * 8), oBN = place((size_t)len * 8), oCX = place((size_t)len * 8),
-           oCN = place((size_t)len * 8), oSP = place((size_t)len * 4);
+           oCN = place((size_t)len * 8), oSP = place((size_t)len * 4), oST = place(ref.ST.size() * 8),
+           oBNs = place(ref.BNs.size() * 8), oCNs = place(ref.CNs.size() * 8);
     std::vector<char> h(off, 0);
     std::memcpy(h.data() + oZ, ref.Zd.data(), ref.Zd.size() * 8);
     std::memcpy(h.data() + oXlo, ref.Xlo.data(), ref.Xlo.size() * 8);
@@ -1285,6 +1334,9 @@ class EngineImpl : public Engine {
     std::memcpy(h.data() + oCX, ref.CX.data(), ref.CX.size() * 8);
     std::memcpy(h.data() + oCN, ref.CN.data(), ref.CN.size() * 8);
     std::memcpy(h.data() + oSP, ref.SP.data(), ref.SP.size() * 4);
+    std::memcpy(h.data() + oST, ref.ST.data(), ref.ST.size() * 8);
+    std::memcpy(h.data() + oBNs, ref.BNs.data(), ref.BNs.size() * 8);
+    std::memcpy(h.data() + oCNs, ref.CNs.data(), ref.CNs.size() * 8);
     int st = grow(&d_ref, &n_ref_bytes, off);
     if (st) return st;
     NM_HIP(hipMemcpyAsync(d_ref, h.data(), off, hipMemcpyHostToDevice, stream));
@@ -1301,6 +1353,10 @@ class EngineImpl : public Engine {
     rCX = (const double*)(b + oCX);
     rCN = (const double*)(b + oCN);
     rSP = (const int*)(b + oSP);
+    rST = (const double*)(b + oST);
+    rBNs = (const double*)(b + oBNs);
+    rCNs = (const double*)(b + oCNs);
+    rst_levels = ref.st_levels;
     ref_len = len;
     ref_serial = serial;
     return NM_OK;
@@ -1702,6 +1758,10 @@ class EngineImpl : public Engine {
     a.CX = rCX;
     a.CN = rCN;
     a.SP = rSP;
+    a.ST = rST;
+    a.BNs = rBNs;
+    a.CNs = rCNs;
+    a.st_levels = rst_levels;
     a.len = ref_len;
     a.mend = ref_len + 1;
     a.tol = opt.tol;

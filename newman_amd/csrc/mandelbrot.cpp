@@ -357,6 +357,32 @@ void Mandelbrot::buildThresholds() {
     }
     ref_.PM[i] = pm; ref_.BX[i] = bx; ref_.BN[i] = bn; ref_.CX[i] = cx; ref_.CN[i] = cn; ref_.SP[i] = sp;
   }
+  // thresholds, their range-minimum sparse table and the magnitude suffix minima
+  ref_.T.assign(len, INFINITY);
+  ref_.BNs.assign(len + 1, INFINITY);
+  ref_.CNs.assign(len + 1, INFINITY);
+  for (int i = 1; i < len; i++) {
+    double br = ref_.B[2 * i], bi = ref_.B[2 * i + 1], cr = ref_.C[2 * i], ci = ref_.C[2 * i + 1];
+    bool special = !std::isfinite(br) || !std::isfinite(bi) || !std::isfinite(cr) || !std::isfinite(ci) ||
+                   (br == 0.0 && bi == 0.0) || (cr == 0.0 && ci == 0.0);
+    if (!special) ref_.T[i] = ltol + log2_sqmag(br, bi) - log2_sqmag(cr, ci);
+  }
+  for (int i = len - 1; i >= 1; i--) {
+    double br = ref_.B[2 * i], bi = ref_.B[2 * i + 1], cr = ref_.C[2 * i], ci = ref_.C[2 * i + 1];
+    bool special = !std::isfinite(br) || !std::isfinite(bi) || !std::isfinite(cr) || !std::isfinite(ci) ||
+                   (br == 0.0 && bi == 0.0) || (cr == 0.0 && ci == 0.0);
+    ref_.BNs[i] = std::min(ref_.BNs[i + 1], special ? INFINITY : log2_sqmag(br, bi));
+    ref_.CNs[i] = std::min(ref_.CNs[i + 1], special ? INFINITY : log2_sqmag(cr, ci));
+  }
+  int K = 1;
+  while ((1 << K) <= len) K++;
+  ref_.st_levels = K;
+  ref_.ST.assign((size_t)K * len, INFINITY);
+  for (int i = 0; i < len; i++) ref_.ST[i] = ref_.T[i];
+  for (int k = 1; k < K; k++)
+    for (int i = 0; i + (1 << k) <= len; i++)
+      ref_.ST[(size_t)k * len + i] =
+          std::min(ref_.ST[(size_t)(k - 1) * len + i], ref_.ST[(size_t)(k - 1) * len + i + (1 << (k - 1))]);
 }
 
 // mandelbrot.cpp:261-267

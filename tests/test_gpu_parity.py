@@ -130,7 +130,9 @@ import os
 _VIEWS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "views.json")))
 # (view, rows, cols, width): series path (sample spacing < 1.5e-16)
 DEEP_CASES = [("t11", 96, 128, "1e-16"), ("t11", 96, 128, "1e-14"), ("t13", 96, 128, "1e-25"),
-              ("t13", 96, 128, "1e-22"), ("t13", 45, 61, "3e-24")]
+              ("t13", 96, 128, "1e-22"), ("t13", 45, 61, "3e-24"),
+              # series terms under/overflow in the exact test: the deep start of the stop search
+              ("c5_seed1", 36, 48, "1e-100"), ("c5_seed1", 36, 48, "3e-97"), ("c4_seed1", 36, 48, "1e-40")]
 
 
 def _deep(name, nr, nc, width):
