@@ -246,16 +246,21 @@ def cpu_baseline(desc, name, nthreads, rows_every):
     views = load_views()
     if name == "c5":
         v = views["c5_seed1"]
-        m = nm.Mandelbrot(2160, 3840)
-        m.set_view(v["N"], v["re"], v["im"], v["width"])
-        desc = {"workload": "learned-skip configuration: 3840x2160, width 1e-100, N 100000, bailout 1024, tol 1e-10, "
-                            "centre = zoom walk seed 1; the FP64 per-sample path (variants: FP32X deltas, 64x64 "
-                            "tile skips from the probe search and from the learned predictor)",
-                "samples": 3840 * 2160, "N": 100000, "width": "1e-100", "precision_bits": m.precision_bits()}
-
-        def step(k, w, reps=1):
-            return m.render(k, w, reps, copy_back=False)
-        return m, desc, FLOP_PER_PERT_STEP, step
+        nr, nc = 2160, 3840
+        o = O.OView(nr, nc)
+        o.set_view(v["N"], v["re"], v["im"], v["width"])
+        t0 = time.perf_counter()
+        o.precompute(nthreads)
+        t_pre = time.perf_counter() - t0
+        rows = np.arange(rows_every // 2, nr, rows_every * 4, dtype=np.int32)
+        t0 = time.perf_counter()
+        out = o.render_rows(rows, nthreads)
+        t_run = time.perf_counter() - t0
+        credited = int(out["iterations"].astype(np.int64).sum())
+        return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
+                "sample": "oracle reference method (series scan + HP continuation, %d-bit) on rows %d::%d of the "
+                          "3840x2160 frame (%d samples, %.3g credited px-it, %.1f s); precompute %.1f s, not "
+                          "included" % (o.bits(), rows_every // 2, rows_every * 4, out.size, credited, t_run, t_pre)}
     if name == "c4":
         v = views["c4_seed1"]
         nr, nc = 720, 1280
