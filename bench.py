@@ -118,9 +118,10 @@ def pmc_traffic(config, kernel):
     with open(path) as f:
         for ln in f:
             p = ln.split()
-            if len(p) >= 3 and p[0] == kernel and p[1] == "FETCH_SIZE_KiB":
+            name = p[0].split("<")[0] if p else ""
+            if len(p) >= 3 and name == kernel and p[1] == "FETCH_SIZE_KiB":
                 fetch = float(p[2])
-            if len(p) >= 3 and p[0] == kernel and p[1] == "WRITE_SIZE_KiB":
+            if len(p) >= 3 and name == kernel and p[1] == "WRITE_SIZE_KiB":
                 write = float(p[2])
     if fetch is None or write is None:
         return None
