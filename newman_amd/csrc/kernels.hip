@@ -908,24 +908,25 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
 // and |Y|^2 and poisons the lane with NaN so later steps cannot escape again.
 template <int K, bool END>
 __device__ __forceinline__ void pert_block(double& dr, double& di, const double er, const double ei, int& m,
-                                           const int it, const double2* zb, const double2* zhead, const int endj,
+                                           const int it, double2* zb, const double2* zhead, const int endj,
                                            const int mend, const double b2, int& esc, double& esc_r2,
                                            unsigned int& rb) {
-  const double qnan = __longlong_as_double(0x7ff8000000000000ll);
   bool moved = false;
   unsigned int hoff = 0;
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    const double2 zh = *(const double2*)((const char*)zhead + hoff);
-    const double zr = moved ? zh.x : zb[j].x;
-    const double zi = moved ? zh.y : zb[j].y;
+    // zb[j] is dead after this step: a lane that rebased overwrites it in place
+    if (moved) zb[j] = *(const double2*)((const char*)zhead + hoff);
+    const double zr = zb[j].x;
+    const double zi = zb[j].y;
     const double yr = zr + dr, yi = zi + di;
     const double r2 = yr * yr + yi * yi;  // sqMag (mandelbrot.cpp:60)
-    if (r2 > b2) {  // first escape: NaN in dr reaches every later step
-      esc = it + j;
-      esc_r2 = r2;
-      dr = qnan;
-    }
+    // the first escape records its step and |Y|^2; a NaN high word in dr reaches
+    // every later step, so no later step escapes (selects, no branch)
+    const bool e = r2 > b2;
+    esc = e ? it + j : esc;
+    esc_r2 = e ? r2 : esc_r2;
+    dr = __hiloint2double(e ? 0x7ff80000 : __double2hiint(dr), __double2loint(dr));
     const double dmag = fma(dr, dr, di * di);
     double wr = zr + yr, wi = zi + yi;
     bool reb = r2 < dmag;
@@ -968,7 +969,7 @@ __device__ __forceinline__ void fx_normalise(double yr, double yi, double er, do
 
 template <int K, bool END>
 __device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, float& esr, float& esi, const double er,
-                                              const double ei, int& m, const int it, const double2* zb,
+                                              const double ei, int& m, const int it, double2* zb,
                                               const double2* zhead, const int endj, const int mend, const double b2,
                                               const float b2g, int& esc, double& esc_r2, unsigned int& rb) {
   const float qnanf = __int_as_float(0x7fc00000);
@@ -976,9 +977,9 @@ __device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, flo
   unsigned int hoff = 0;
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    const double2 zh = *(const double2*)((const char*)zhead + hoff);
-    const double zrd = moved ? zh.x : zb[j].x;
-    const double zid = moved ? zh.y : zb[j].y;
+    if (moved) zb[j] = *(const double2*)((const char*)zhead + hoff);
+    const double zrd = zb[j].x;
+    const double zid = zb[j].y;
     const float zr = (float)zrd, zi = (float)zid;
     const float dr = ldexpf(wr, ex), di = ldexpf(wi, ex);
     const float yr = zr + dr, yi = zi + di;
@@ -1023,7 +1024,7 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
   const Common& c = a.c;
   const uint32_t total = *c.nactive;
   const uint32_t W = gridDim.x * (blockDim.x >> 6);
-  unsigned long long executed = 0, rebased = 0;
+  uint32_t executed = 0, rebased = 0;  // per lane (< 2^32 steps per lane per launch)
   const int N = c.N, mend = a.mend;
   // orbit head for lanes that rebase inside a block (Zd is padded past it)
   __shared__ double2 zhead[kHead + 1];
