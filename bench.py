@@ -15,7 +15,7 @@ iteration kernel (15 per executed perturbation step) over its HIP-event duration
 against the FP64 vector peak.  cpu_baseline = the CPU oracle (the reference method:
 series scan + HP continuation) on a row subsample of the same frame.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c4|c5]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5]
 """
 import argparse
 import json
@@ -69,6 +69,19 @@ def make_workload(name):
         def step(k, w, reps=1):
             return m.render_frames(widths, k, w, reps, copy_back=False)[1]
         return m, desc, FLOP_PER_PLAIN_STEP, step
+    if name == "c3":
+        v = views["c3_seed1"]
+        m = nm.Mandelbrot(1440, 2560)
+        m.set_view(v["N"], v["re"], v["im"], v["width"])
+        m.set_skip_mode(nm.SKIP_NONE, 64)
+        desc = {"workload": "precision variants: 2560x1440, width 1e-200, N 50000, bailout 1024, centre = zoom walk "
+                            "seed 1; no series (the reference's double-precision series test degenerates at this "
+                            "depth), FP64 deltas (variant: FP32X deltas)",
+                "samples": 2560 * 1440, "N": 50000, "width": "1e-200", "precision_bits": m.precision_bits()}
+
+        def step(k, w, reps=1):
+            return m.render(k, w, reps, copy_back=False)
+        return m, desc, FLOP_PER_PERT_STEP, step
     if name == "c5":
         v = views["c5_seed1"]
         m = nm.Mandelbrot(2160, 3840)
@@ -204,6 +217,28 @@ def variants(m, step, args, value64, wall64):
     return out
 
 
+def fp32x_variant(m, step, args, value64):
+    """FP32X deltas on the same frame: value, ratio and per-sample error against
+    the FP64 deltas (bit-exact with the oracle's FP64 method)."""
+    k, w = args.block_k, args.waves
+    m.render(k, w, 1, copy_back=True)
+    g64 = m.grid().copy()
+    m.set_delta_mode(1)
+    step(k, w, 1)
+    t0 = time.perf_counter()
+    st = step(k, w, args.steps)
+    wl = (time.perf_counter() - t0) / args.steps
+    m.render(k, w, 1, copy_back=True)
+    g32 = m.grid().copy()
+    m.set_delta_mode(0)
+    same = g32["iterations"] == g64["iterations"]
+    ds = np.abs(g32["smoothing"].astype(np.float64) - g64["smoothing"])[same]
+    return {"fp32x": {"value": st["credited"] / wl, "ms_per_step": wl * 1e3, "vs_fp64": (st["credited"] / wl) / value64,
+                      "ms_iterate": st["ms_iterate"], "counts_equal_fp64": float(same.mean()),
+                      "max_count_diff": int(np.abs(g32["iterations"].astype(np.int64) - g64["iterations"]).max()),
+                      "smoothing_absdiff_p99": float(np.percentile(ds, 99)) if ds.size else 0.0}}
+
+
 def tune(step, ks, waves):
     """Sweep block size and persistent waves per SIMD; kernel time by HIP events."""
     best = None
@@ -245,6 +280,24 @@ def cpu_baseline(desc, name, nthreads, rows_every):
                           "640x480 frame, repeated %d times (%.1f s); credited = iterations of non-cardioid samples"
                           % (reps, t_run)}
     views = load_views()
+    if name == "c3":
+        v = views["c3_seed1"]
+        nr, nc = 1440, 2560
+        o = O.OView(nr, nc)
+        o.set_view(v["N"], v["re"], v["im"], v["width"])
+        t0 = time.perf_counter()
+        o.precompute(nthreads)
+        t_pre = time.perf_counter() - t0
+        rows = np.arange(rows_every // 2, nr, rows_every * 5, dtype=np.int32)
+        rc = np.array([(r, c) for r in rows for c in range(nc)], dtype=np.int32)
+        t0 = time.perf_counter()
+        out = o.pert_samples_ua(rc, np.ones(len(rc), dtype=np.int32), nthreads)
+        t_run = time.perf_counter() - t0
+        credited = int(out["iterations"].astype(np.int64).sum())
+        return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
+                "sample": "oracle FP64 perturbation without series (the method the GPU runs here) on rows %d::%d of "
+                          "the 2560x1440 frame (%d samples, %.3g px-it, %.1f s); precompute %.1f s, not included"
+                          % (rows_every // 2, rows_every * 5, out.size, credited, t_run, t_pre)}
     if name == "c5":
         v = views["c5_seed1"]
         nr, nc = 2160, 3840
@@ -392,6 +445,8 @@ def main():
         line["autotune"] = tuned
     if args.variants and args.config in ("c2", "c5"):
         line["variants"] = variants(m, step, args, value, wall)
+    if args.variants and args.config == "c3":
+        line["variants"] = fp32x_variant(m, step, args, value)
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(desc, args.config, args.cpu_threads, args.cpu_rows_every)

@@ -173,6 +173,7 @@ int nm_autotune_frames(nm_mandelbrot* h, const char* const* widths, int nframes,
 #define NM_SKIP_TILE_SCAN 1
 #define NM_SKIP_TILE_TABLE 2
 #define NM_SKIP_TILE_LEARNED 3
+#define NM_SKIP_NONE 4 /* no series: every sample starts at d_0 = e (plain perturbation) */
 #define NM_SKIPNET_INPUTS 132
 #define NM_SKIPNET_PARAMS 12737
 typedef struct nm_skipnet nm_skipnet;

@@ -69,7 +69,7 @@ class TileStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
-SKIP_PIXEL, SKIP_TILE_SCAN, SKIP_TILE_TABLE, SKIP_TILE_LEARNED = 0, 1, 2, 3
+SKIP_PIXEL, SKIP_TILE_SCAN, SKIP_TILE_TABLE, SKIP_TILE_LEARNED, SKIP_NONE = 0, 1, 2, 3, 4
 SKIPNET_INPUTS, SKIPNET_PARAMS = 132, 12737
 
 

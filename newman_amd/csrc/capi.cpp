@@ -488,7 +488,7 @@ int nm_tile_targets(nm_mandelbrot* h, const int* ua, int n, float* t) {
 }
 
 int nm_set_skip_mode(nm_mandelbrot* h, int mode, int tile, nm_skipnet* net) {
-  if (!h || mode < NM_SKIP_PIXEL || mode > NM_SKIP_TILE_LEARNED || tile < 8) return NM_ERR_ARG;
+  if (!h || mode < NM_SKIP_PIXEL || mode > NM_SKIP_NONE || tile < 8) return NM_ERR_ARG;
   if (mode == NM_SKIP_TILE_LEARNED && !net) return NM_ERR_ARG;
   h->m.skip_mode = mode;
   h->m.skip_tile = tile;

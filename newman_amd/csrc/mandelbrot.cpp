@@ -680,9 +680,16 @@ int Mandelbrot::renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue*
   std::vector<const SaFrameHost*> ptr;
   for (size_t k = first; k < set.s.size(); k += step) ptr.push_back(&set.s[k]);
   if (skip_mode == SKIP_PIXEL) return eng->render_sa(ptr, o, out, &stats_);
-  // one series skip per tile, from the tile probes
+  // one series skip per tile, from the tile probes (SKIP_NONE: first unstable
+  // index 1 everywhere, so every sample starts at d_0 = e)
   std::vector<int> ua;
-  if ((st = tileSkipsFor(eng, ptr, o, ua)) != NM_OK) return st;
+  if (skip_mode == SKIP_NONE) {
+    const int T = skip_tile, ntc = (cols() + T - 1) / T, ntr = (rows() + T - 1) / T;
+    ua.assign((size_t)ntc * ntr * ptr.size(), 1);
+    tile_stats = TileStats();
+  } else if ((st = tileSkipsFor(eng, ptr, o, ua)) != NM_OK) {
+    return st;
+  }
   RenderOpts ot = o;
   ot.tile_ua = ua.data();
   ot.skip_T = skip_tile;

@@ -53,7 +53,10 @@ class Mandelbrot {
   // (its sample farthest from the reference point), found by the reference's scan
   // (SKIP_TILE_SCAN), the threshold tables (SKIP_TILE_TABLE) or predicted by a
   // SkipNet and validated, with the scan as fallback (SKIP_TILE_LEARNED).
-  enum SkipMode { SKIP_PIXEL = 0, SKIP_TILE_SCAN = 1, SKIP_TILE_TABLE = 2, SKIP_TILE_LEARNED = 3 };
+  // SKIP_NONE starts every sample at index 0 (no series: plain perturbation from
+  // delta_0 = e), for views where the reference's double-precision series test
+  // degenerates and for precision studies of the delta arithmetic.
+  enum SkipMode { SKIP_PIXEL = 0, SKIP_TILE_SCAN = 1, SKIP_TILE_TABLE = 2, SKIP_TILE_LEARNED = 3, SKIP_NONE = 4 };
   int skip_mode = SKIP_PIXEL;
   int skip_tile = 64;
   SkipNet* skipnet = nullptr;  // not owned

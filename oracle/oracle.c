@@ -1060,9 +1060,10 @@ void orc_pert_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nt
   free((void*)R.P.Zd);
 }
 
-/* the GPU method with FP32X deltas (block size K of the kernel) */
-void orc_pert_samples_fx(orc_view* v, const int* rc, int n, int K, orc_escape* out, int nthreads) {
-  render_ctx R = {v, rc, out, 2, {0}, 0, NULL, NULL, K};
+/* the GPU method with FP32X deltas (block size K of the kernel); ua as in
+ * orc_pert_samples_ua, or NULL for each sample's own scan */
+void orc_pert_samples_fx(orc_view* v, const int* rc, const int* ua, int n, int K, orc_escape* out, int nthreads) {
+  render_ctx R = {v, rc, out, 2, {0}, 0, NULL, ua, K};
   setup_pert(v, &R);
   parallel_for(n, nthreads, 4, samples_fn, &R);
   free((void*)R.P.Zd);

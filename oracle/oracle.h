@@ -71,7 +71,7 @@ void orc_render_samples(orc_view* v, const int* rc, int n, orc_escape* out, int 
 /* GPU-method restatement (pert64) on selected samples */
 void orc_pert_samples(orc_view* v, const int* rc, int n, orc_escape* out, int nthreads);
 /* the same with FP32X deltas, as the kernel with block size K runs them */
-void orc_pert_samples_fx(orc_view* v, const int* rc, int n, int K, orc_escape* out, int nthreads);
+void orc_pert_samples_fx(orc_view* v, const int* rc, const int* ua, int n, int K, orc_escape* out, int nthreads);
 /* the same with a given first unstable index per sample (a tile's skip; 0 = none) */
 void orc_pert_samples_ua(orc_view* v, const int* rc, const int* ua, int n, orc_escape* out, int nthreads);
 /* per T x T tile (row-major): first unstable index of the reference's scan at the

@@ -45,7 +45,7 @@ _SIGS = [
     ("orc_render_samples", None, [_V, _V, C.c_int, _V, C.c_int]),
     ("orc_pert_samples", None, [_V, _V, C.c_int, _V, C.c_int]),
     ("orc_pert_samples_ua", None, [_V, _V, _V, C.c_int, _V, C.c_int]),
-    ("orc_pert_samples_fx", None, [_V, _V, C.c_int, C.c_int, _V, C.c_int]),
+    ("orc_pert_samples_fx", None, [_V, _V, _V, C.c_int, C.c_int, _V, C.c_int]),
     ("orc_tile_probe_ua", C.c_int, [_V, C.c_int, _V, _V]),
     ("orc_mlp_nparams", C.c_int, []),
     ("orc_mlp_forward", None, [_V, _V, C.c_int, _V]),
@@ -183,11 +183,14 @@ class OView:
         self.L.orc_pert_samples_ua(self.v, rc.ctypes.data, ua.ctypes.data, len(rc), out.ctypes.data, nthreads)
         return out
 
-    def pert_samples_fx(self, rc, K, nthreads=0):
-        """The GPU method with FP32X deltas (kernel block size K)."""
+    def pert_samples_fx(self, rc, K, ua=None, nthreads=0):
+        """The GPU method with FP32X deltas (kernel block size K); ua: given first
+        unstable index per sample, or None for each sample's own scan."""
         rc = np.ascontiguousarray(rc, dtype=np.int32).reshape(-1, 2)
         out = np.zeros(len(rc), dtype=ESCAPE_DTYPE)
-        self.L.orc_pert_samples_fx(self.v, rc.ctypes.data, len(rc), K, out.ctypes.data, nthreads)
+        u = None if ua is None else np.ascontiguousarray(ua, dtype=np.int32)
+        self.L.orc_pert_samples_fx(self.v, rc.ctypes.data, None if u is None else u.ctypes.data, len(rc), K,
+                                   out.ctypes.data, nthreads)
         return out
 
     def tile_probe_ua(self, T):
