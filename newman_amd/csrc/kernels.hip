@@ -72,7 +72,12 @@ constexpr int kThreads = 256;           // workgroup size of every kernel
 constexpr int kPerThread = 4;           // samples per thread per start tile
 constexpr int kTile = kThreads * kPerThread;
 constexpr int kChunk = 64;              // active samples per chunk (one per lane)
-constexpr int kHead = 64;               // orbit entries kept in LDS for rebased lanes (>= max K)
+constexpr int kHead = 64;               // orbit padding past its end (>= max K)
+// orbit windows each iterate workgroup keeps in LDS: the head (where rebased
+// lanes live) and the tail (where samples that never rebased run out the
+// reference); lanes whose block fits a window read it from LDS, the others
+// gather from global memory, whose cache tag lookups bound the gathers
+constexpr int kLdsHead = 1024, kLdsTail = 1024;
 constexpr int kPartSlots = 8192;        // per-workgroup partial counter slots
 
 // partial-slot ranges of the kernels of one render
@@ -131,6 +136,7 @@ struct SeriesArgs {
   int st_levels;
   int len;
   int mend;  // len + 1
+  int zd_count;  // entries allocated in Zd (past len + 1: zero padding)
   double tol;
   // per-tile series skip (the first unstable index of each tile's probe); null:
   // every sample searches its own (the reference's per-pixel scan)
@@ -1026,10 +1032,17 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
   const uint32_t W = gridDim.x * (blockDim.x >> 6);
   uint32_t executed = 0, rebased = 0;  // per lane (< 2^32 steps per lane per launch)
   const int N = c.N, mend = a.mend;
-  // orbit head for lanes that rebase inside a block (Zd is padded past it)
-  __shared__ double2 zhead[kHead + 1];
-  for (int i = threadIdx.x; i <= kHead; i += blockDim.x) zhead[i] = a.Zd[i];
+  // the orbit's head and tail windows (zlds[0, kLdsHead) = Zd[0, kLdsHead),
+  // zlds[kLdsHead + t] = Zd[tail_lo + t]); entries past Zd's allocation read 0
+  __shared__ double2 zlds[kLdsHead + kLdsTail];
+  const int tail_lo = max(kLdsHead, min(a.zd_count, a.len + 2 + K) - kLdsTail);
+  const int tail_hi = tail_lo + kLdsTail;
+  for (int i = threadIdx.x; i < kLdsHead + kLdsTail; i += blockDim.x) {
+    const int src = i < kLdsHead ? i : tail_lo + (i - kLdsHead);
+    zlds[i] = src < a.zd_count ? a.Zd[src] : make_double2(0.0, 0.0);
+  }
   __syncthreads();
+  const double2* zhead = zlds;  // rebased lanes read the orbit from its start
   uint32_t ccur = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   uint32_t ncur = chunk_items(ccur, total), nnext = chunk_items(ccur + W, total);
   uint32_t cur_pix = load_pix(c.active, ccur, total), nxt_pix = load_pix(c.active, ccur + W, total);
@@ -1107,8 +1120,16 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
     // lanes whose block crosses the reference's end rebase there (END variant)
     const bool end_zone = __ballot(act && m + K > mend) != 0 || mend <= K;
     double2 zb[K];
+    const bool in_head = m + K <= kLdsHead;
+    const bool in_tail = !in_head && m >= tail_lo && m + K <= tail_hi;
+    if (in_head || in_tail) {
+      const double2* src = zlds + (in_head ? m : kLdsHead + m - tail_lo);
 #pragma unroll
-    for (int j = 0; j < K; j++) zb[j] = a.Zd[m + j];
+      for (int j = 0; j < K; j++) zb[j] = src[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < K; j++) zb[j] = a.Zd[m + j];
+    }
     int esc = -1;
     double esc_r2 = 0.0;
     unsigned int rb = 0;
@@ -1270,6 +1291,7 @@ class EngineImpl : public Engine {
   const int* rSP = nullptr;
   const double *rST = nullptr, *rBNs = nullptr, *rCNs = nullptr;
   int rst_levels = 0;
+  int ref_zd_count = 0;
 
   int fail(const char* what, hipError_t e) {
     err = std::string(what) + ": " + hipGetErrorString(e);
@@ -1312,6 +1334,7 @@ class EngineImpl : public Engine {
     // Zd padded with zeros so idle lanes can run a block from index 0 and the
     // LDS orbit head (kHead + 1 entries) always exists
     const size_t nz = (size_t)len + 2 + kHead + 64;
+    ref_zd_count = (int)nz;
     size_t off = 0;
     auto place = [&](size_t bytes) {
       size_t o = off;
@@ -1765,6 +1788,7 @@ class EngineImpl : public Engine {
     a.st_levels = rst_levels;
     a.len = ref_len;
     a.mend = ref_len + 1;
+    a.zd_count = ref_zd_count;
     a.tol = opt.tol;
   }
 
