@@ -119,7 +119,8 @@ def make_workload(name):
 FETCH_SCALE = 1048576 / 526656
 WRITE_SCALE = 1048576 / 1084704
 PMC_FILES = {"c2": "profiles/series_iterate_deepzoom_1e-30_pmc.txt",
-             "c1": "profiles/plain_c1_640x480_pmc.txt"}
+             "c1": "profiles/plain_c1_640x480_pmc.txt",
+             "c4": "profiles/series_iterate_zoomvideo_120x1280x720_pmc.txt"}
 
 
 def pmc_traffic(config, kernel):
