@@ -2,32 +2,42 @@
 // device engine that drives them.
 //
 // Layout in HBM (one frame set per render):
-//   frames[]        DevFrame per frame (sample base, size, table offsets, output base)
+//   frames[]        DevFrame per frame (sample base, size, sample order, table
+//                   offsets, output base)
 //   col/row tables  plain path: truncated coordinate + double-double remainder;
 //                   series path: epsilon = descend(Y0 - X[0]) per column / row
-//   out[]           EscapeValue {int32, float} row-major per frame (grid.h layout)
+//   out[]           EscapeValue {int32, float} per sample, in sample order (the
+//                   grid.h layout itself for row-major orders; k_emit scatters
+//                   tiled orders into the row-major grids)
 //   flag[]          u8 per sample: 1 = handed to the iteration kernel
-//   state[]         per sample (series path): delta, epsilon (4 x f64); it, m
+//   state[]         per sample: delta (or z) and epsilon (or c), 4 x f64; it, m
+//   r2[]            |Y|^2 at the escape of iterated samples (smoothing input)
 //   active[]        u32 sample ids that iterate, compacted in sample order
-//   Zd[]            reference orbit as f64 pairs, Zd[0] = 0, padded past its end
+//   reference block Zd (the orbit as f64 pairs, Zd[0] = 0, padded past its end),
+//                   Xlo, A, B, C and the stop-search tables
 //
-// Pipeline per render, with no device-scope atomics on the hot path (one
+// Pipeline per render pass, with no device-scope atomics on the hot path (one
 // contended counter serialises at ~10 ns per atomic on this part):
-//   *_start   tiles of 1024 samples: cardioid/bulb test (plain path) or the
-//             reference's per-sample series start (series path); resolves what it
-//             can, flags the rest and counts them per tile
-//   scan      exclusive scan of the tile counts (one workgroup)
-//   scatter   compacts flagged sample ids into active[] in sample order
-//   *_iterate persistent waves: each lane holds one sample; blocks of K steps
-//             run without exits (the first escape of a lane records its step and
-//             |z|^2 and poisons the lane with NaN: escape is absorbing); lanes are
-//             refilled (ballot + popcount ranks) from 64-sample chunks of
-//             active[], chunk c owned by wave c mod W (interleaved, so every wave
-//             gets an even share of the frame)
-// Counters go to per-workgroup partial slots summed on the host.
+//   k_zero      clears the per-workgroup counter slots (a kernel, so a captured
+//               HIP graph holds no memset node)
+//   *_start     tiles of 1024 samples: cardioid/bulb test and up to pre_iter
+//               updates (plain path), or the reference's per-sample series start
+//               (series path: stop index, series skip, escape inside the skip,
+//               first step); resolves what it can, flags the rest, counts per tile
+//   scan        exclusive scan of the tile counts (one workgroup)
+//   scatter     compacts flagged sample ids into active[] in sample order
+//   *_iterate   persistent waves, one sample per lane; blocks of K steps; the
+//               first escape of a lane records its step and |z|^2 and poisons the
+//               lane with NaN (escape is absorbing); lanes are refilled (ballot +
+//               popcount ranks) from 64-sample chunks of active[], chunk c owned by
+//               wave c mod W, with the next chunk's states prefetched; the series
+//               kernel keeps the orbit's head and tail windows in LDS
+//   finalize    smoothing of the iterated samples from r2
+//   emit        tiled sample orders only: results into the row-major grids
+// Counters go to per-workgroup partial slots reduced on the device.
 // This file is compiled with -ffp-contract=off: the plain path must reproduce the
 // reference's double arithmetic bit for bit (complex.h:19-31, makefile:3 has no
-// FMA); the perturbation path writes its FMAs explicitly.
+// FMA); the perturbation paths write their FMAs explicitly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
