@@ -388,8 +388,9 @@ def main():
     if not m.useHardware():
         m.precompute()
     t_pre = time.perf_counter() - t0
-    print("# precompute %.2f s (host HP: findProbe + series), len(X) = %s" % (t_pre, m.reference_length()),
-          file=sys.stderr, flush=True)
+    n_hp = m.probe_candidates() if not m.useHardware() else 0
+    print("# precompute %.2f s (findProbe screened on the GPU, %d probe orbits in HP; series in HP), len(X) = %s"
+          % (t_pre, n_hp, m.reference_length()), file=sys.stderr, flush=True)
     k, w = args.block_k, args.waves
     if args.config != "c4":
         m.set_launch(tile=args.tile, pre_iter=args.pre_iter)
@@ -440,6 +441,7 @@ def main():
                    "n_cardioid": s["n_cardioid"], "n_rebased_blocks": s["n_rebased"],
                    "ms_start": s["ms_start"], "ms_compact": s["ms_compact"], "ms_iterate": s["ms_iterate"],
                    "ms_finalize": s["ms_finalize"], "ms_kernels": s["ms_total"], "precompute_s": t_pre,
+                   "probe_orbits_hp": n_hp,
                    "kernel_px_it_per_s": credited / (s["ms_total"] * 1e-3)},
         "cpu_baseline": None,
     }

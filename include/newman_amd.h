@@ -126,6 +126,15 @@ int nm_reference_length(const nm_mandelbrot* h);  /* len(X) */
 /* the reference point X[0] as decimal strings (precompute first) */
 int nm_reference_point(const nm_mandelbrot* h, char* re, char* im, size_t cap);
 int nm_probe_winner(const nm_mandelbrot* h);      /* index of findProbe's winner, -1 if none */
+/* how precompute runs findProbe (mandelbrot.cpp:73-95): NM_PROBE_HOST iterates
+ * every probe in HP on the host; NM_PROBE_GPU (the default) screens the probes
+ * by FP64 perturbation on the device and iterates in HP only those within 3
+ * iterations of the best estimate, falling back to the host when no device is
+ * present.  nm_probe_candidates: probes the last search iterated in HP. */
+#define NM_PROBE_HOST 0
+#define NM_PROBE_GPU 1
+int nm_set_probe_mode(nm_mandelbrot* h, int mode);
+int nm_probe_candidates(const nm_mandelbrot* h);
 /* reference series as the device sees it: X (nearest), A, B, C (truncated),
  * 2*len doubles each (re, im interleaved); any pointer may be NULL */
 int nm_get_reference(const nm_mandelbrot* h, double* X, double* A, double* B, double* C);

@@ -70,6 +70,7 @@ class TileStats(C.Structure):
 
 
 SKIP_PIXEL, SKIP_TILE_SCAN, SKIP_TILE_TABLE, SKIP_TILE_LEARNED, SKIP_NONE = 0, 1, 2, 3, 4
+PROBE_HOST, PROBE_GPU = 0, 1
 SKIPNET_INPUTS, SKIPNET_PARAMS = 132, 12737
 
 
@@ -166,6 +167,8 @@ _SIGS = [
     ("nm_precompute_at", C.c_int, [_H, C.c_char_p, C.c_char_p]),
     ("nm_reference_length", C.c_int, [_H]),
     ("nm_probe_winner", C.c_int, [_H]),
+    ("nm_set_probe_mode", C.c_int, [_H, C.c_int]),
+    ("nm_probe_candidates", C.c_int, [_H]),
     ("nm_get_reference", C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("nm_render", C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(RenderStats)]),
     ("nm_render_frames", C.c_int, [_H, C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
@@ -387,6 +390,14 @@ class Mandelbrot:
 
     def probe_winner(self):
         return lib().nm_probe_winner(self._h)
+
+    def set_probe_mode(self, mode):
+        """PROBE_HOST: findProbe in HP for every probe; PROBE_GPU: probes screened
+        by FP64 perturbation on the device, HP only for the near-best ones."""
+        _check(lib().nm_set_probe_mode(self._h, int(mode)), "nm_set_probe_mode")
+
+    def probe_candidates(self):
+        return lib().nm_probe_candidates(self._h)
 
     def reference(self):
         n = self.reference_length()

@@ -288,6 +288,12 @@ int nm_reference_point(const nm_mandelbrot* h, char* re, char* im, size_t cap) {
   });
 }
 int nm_probe_winner(const nm_mandelbrot* h) { return h ? h->m.probeWinner() : NM_ERR_ARG; }
+int nm_set_probe_mode(nm_mandelbrot* h, int mode) {
+  if (!h || (mode != NM_PROBE_HOST && mode != NM_PROBE_GPU)) return NM_ERR_ARG;
+  h->m.probe_mode = mode;
+  return NM_OK;
+}
+int nm_probe_candidates(const nm_mandelbrot* h) { return h ? h->m.probeCandidates() : NM_ERR_ARG; }
 
 int nm_get_reference(const nm_mandelbrot* h, double* X, double* A, double* B, double* C) {
   if (!h) return NM_ERR_ARG;

@@ -237,17 +237,23 @@ int Mandelbrot::computeOrbit(const HPComplex& X0, bool store) {
   return len;
 }
 
+// The probe list of findProbe (mandelbrot.cpp:77-94), in its order.
+static std::vector<std::pair<int, int>> probe_points(int nr, int nc) {
+  std::vector<std::pair<int, int>> pts;
+  for (int c = 0; c < nc; c += 2) {
+    pts.emplace_back(nr / 4, c);
+    pts.emplace_back(nr / 2, c);
+    pts.emplace_back(3 * nr / 4, c);
+  }
+  for (int r = 0; r < nr; r += 2) pts.emplace_back(r, nc / 2);
+  return pts;
+}
+
 // mandelbrot.cpp:73-95.  The probe orbits are independent: their lengths are
 // computed in parallel and the first longest one is kept, as the serial
 // swap-if-longer loop does; its orbit is then recomputed and stored.
 void Mandelbrot::findProbe() {
-  std::vector<std::pair<int, int>> pts;
-  for (int c = 0; c < cols(); c += 2) {
-    pts.emplace_back(rows() / 4, c);
-    pts.emplace_back(rows() / 2, c);
-    pts.emplace_back(3 * rows() / 4, c);
-  }
-  for (int r = 0; r < rows(); r += 2) pts.emplace_back(r, cols() / 2);
+  const auto pts = probe_points(rows(), cols());
   const int L = hp_limbs_for_bits(hp_default_bits());
   std::vector<int> lens(pts.size(), 0);
   parallel_for((long)pts.size(), [&](long i) {
@@ -267,6 +273,7 @@ void Mandelbrot::findProbe() {
       best = (int)i;
     }
   winner_ = best;
+  probe_candidates_ = (int)pts.size();
   HPComplex p;
   p.re = hpf(L); p.im = hpf(L);
   pixel_re(p.re, center.re, sz.re, (long)(pts[best].second - cols() / 2));
@@ -391,9 +398,126 @@ int Mandelbrot::precompute() {
   if (N <= 0 || rows() <= 0 || cols() <= 0) return NM_ERR_ARG;
   X.clear();
   ref_ = SaReferenceHost();
-  findProbe();
+  Engine* eng = nullptr;
+  if (probe_mode == PROBE_HOST || Engine::instance(&eng) != NM_OK || findProbeGPU(eng) != NM_OK) findProbe();
   computeSeries();
   return uploadReference();
+}
+
+// findProbe with the GPU doing the screening: every probe's orbit length is
+// estimated by FP64 perturbation (no series, rebasing) against the HP orbit of
+// the frame centre, in two small renders on the device; only the probes whose
+// estimate is within kProbeMargin of the best are then iterated in HP on the
+// host, in probe order, and the first with the greatest exact length wins, as in
+// the reference.  The estimates agree with HP lengths on all but about 1e-4 of
+// samples and then by 1 iteration, so the margin keeps the true winner among the
+// candidates.  The winner's orbit is computed and stored as findProbe does.
+int Mandelbrot::findProbeGPU(Engine* eng) {
+  constexpr int kProbeMargin = 3;
+  const auto pts = probe_points(rows(), cols());
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  // the centre's orbit as the temporary device reference (A, B, C unused: every
+  // probe starts at d0 = e)
+  HPComplex c0;
+  c0.re = hpf(L); c0.im = hpf(L);
+  hp_set(c0.re, center.re);
+  hp_set(c0.im, center.im);
+  const int lenc = computeOrbit(c0, true);
+  X.resize((size_t)lenc + 1);
+  SaReferenceHost tmp;
+  tmp.len = lenc;
+  fillOrbit(tmp);
+  tmp.A.assign(2 * (size_t)lenc, 0.0);
+  tmp.B = tmp.A;
+  tmp.C = tmp.A;
+  tmp.PM.assign(lenc, 0.0);
+  tmp.BX = tmp.BN = tmp.CX = tmp.CN = tmp.PM;
+  tmp.SP.assign(lenc, 0);
+  tmp.T = tmp.PM;
+  tmp.BNs.assign((size_t)lenc + 1, 0.0);
+  tmp.CNs = tmp.BNs;
+  tmp.st_levels = 0;
+  int st = eng->set_reference(tmp, next_serial());
+  if (st != NM_OK) return st;
+  // frame A: rows nr/4, nr/2, 3nr/4 x even columns; frame B: even rows x column nc/2
+  const int nca = (cols() + 1) / 2, nrb = (rows() + 1) / 2;
+  SaFrameHost fa, fb;
+  fa.nr = 3; fa.nc = nca; fa.card_mode = CARD_NONE;
+  fb.nr = nrb; fb.nc = 1; fb.card_mode = CARD_NONE;
+  hpf p(L), y(L);
+  auto eps_re = [&](int c) {
+    pixel_re(p, center.re, sz.re, (long)(c - cols() / 2));
+    hp_sub(y, p, X[0].re);
+    return y.get_d();
+  };
+  auto eps_im = [&](int r) {
+    pixel_re(p, center.im, sz.im, (long)(rows() / 2 - r - 1));
+    hp_sub(y, p, X[0].im);
+    return y.get_d();
+  };
+  for (int k = 0; k < nca; k++) fa.epsre.push_back(eps_re(2 * k));
+  for (int r : {rows() / 4, rows() / 2, 3 * rows() / 4}) fa.epsim.push_back(eps_im(r));
+  fb.epsre.push_back(eps_re(cols() / 2));
+  for (int k = 0; k < nrb; k++) fb.epsim.push_back(eps_im(2 * k));
+  RenderOpts o;
+  applyLaunch(o);
+  o.tile = 0;
+  o.graph = false;
+  o.repeats = 1;
+  o.delta_mode = DELTA_FP64;
+  std::vector<EscapeValue> outa((size_t)3 * nca), outb((size_t)nrb);
+  RenderStats rs;
+  for (int pass = 0; pass < 2; pass++) {
+    const SaFrameHost& f = pass ? fb : fa;
+    std::vector<int> ua((size_t)((f.nr + 63) / 64) * ((f.nc + 63) / 64), 1);
+    RenderOpts op = o;
+    op.tile_ua = ua.data();
+    op.skip_T = 64;
+    st = eng->render_sa({&f}, op, pass ? outb.data() : outa.data(), &rs);
+    if (st != NM_OK) return st;
+  }
+  std::vector<int> est(pts.size());
+  for (size_t i = 0; i < pts.size(); i++) {
+    if (i < (size_t)3 * nca) est[i] = outa[(i % 3) * nca + i / 3].iterations;
+    else est[i] = outb[i - (size_t)3 * nca].iterations;
+  }
+  const int emax = *std::max_element(est.begin(), est.end());
+  std::vector<int> cand;
+  for (size_t i = 0; i < pts.size(); i++)
+    if (est[i] >= emax - kProbeMargin) cand.push_back((int)i);
+  // exact lengths of the candidates, in order, in parallel batches; a length of
+  // N cannot be beaten, so the scan stops at the first batch holding one
+  int best = -1, bestlen = 0;
+  const size_t batch = 32;
+  for (size_t b0 = 0; b0 < cand.size(); b0 += batch) {
+    const size_t b1 = std::min(cand.size(), b0 + batch);
+    std::vector<int> lens(b1 - b0, 0);
+    parallel_for((long)(b1 - b0), [&](long k) {
+      const auto& pt = pts[(size_t)cand[b0 + k]];
+      HPComplex q;
+      q.re = hpf(L); q.im = hpf(L);
+      pixel_re(q.re, center.re, sz.re, (long)(pt.second - cols() / 2));
+      pixel_re(q.im, center.im, sz.im, (long)(rows() / 2 - pt.first - 1));
+      lens[k] = computeOrbit(q, false);
+    }, 1);
+    for (size_t k = 0; k < lens.size(); k++)
+      if (lens[k] > bestlen) {
+        bestlen = lens[k];
+        best = cand[b0 + k];
+      }
+    if (bestlen >= N) break;
+  }
+  if (best < 0) return NM_ERR_STATE;
+  probe_candidates_ = (int)cand.size();
+  winner_ = best;
+  HPComplex q;
+  q.re = hpf(L); q.im = hpf(L);
+  pixel_re(q.re, center.re, sz.re, (long)(pts[(size_t)best].second - cols() / 2));
+  pixel_re(q.im, center.im, sz.im, (long)(rows() / 2 - pts[(size_t)best].first - 1));
+  const int len = computeOrbit(q, true);
+  X.resize((size_t)len + 1);
+  ref_.len = len;
+  return NM_OK;
 }
 
 int Mandelbrot::precomputeAt(const char* re, const char* im) {
@@ -408,29 +532,36 @@ int Mandelbrot::precomputeAt(const char* re, const char* im) {
   X.resize((size_t)len + 1);
   ref_.len = len;
   winner_ = -1;
+  probe_candidates_ = 0;
   computeSeries();
   return uploadReference();
 }
 
 static std::atomic<uint64_t> g_ref_serial{1};
 
-int Mandelbrot::uploadReference() {
-  const int len = ref_.len;
-  ref_.Zd.assign(2 * (size_t)(len + 2), 0.0);
-  ref_.Xlo.assign(2 * (size_t)len, 0.0);
+// The device orbit of X (len + 1 entries): Zd[0] = 0, Zd[k + 1] = nearest(X[k]),
+// Xlo[k] = nearest(X[k] - Zd[k + 1])
+void Mandelbrot::fillOrbit(SaReferenceHost& r) const {
+  const int len = r.len;
+  r.Zd.assign(2 * (size_t)(len + 2), 0.0);
+  r.Xlo.assign(2 * (size_t)len, 0.0);
   for (int k = 0; k <= len; k++) {
     for (int p = 0; p < 2; p++) {
       const hpf& x = p ? X[k].im : X[k].re;
       double nd = get_d_nearest(x);
-      ref_.Zd[2 * (k + 1) + p] = nd;
+      r.Zd[2 * (k + 1) + p] = nd;
       if (k < len) {
         hpf td(2), rem(x.L);
         hp_set_d(td, nd);
         hp_sub(rem, x, td);
-        ref_.Xlo[2 * k + p] = get_d_nearest(rem);
+        r.Xlo[2 * k + p] = get_d_nearest(rem);
       }
     }
   }
+}
+
+int Mandelbrot::uploadReference() {
+  fillOrbit(ref_);
   buildThresholds();
   have_ref_ = true;
   rendered_ = false;

@@ -60,6 +60,11 @@ class Mandelbrot {
   int skip_mode = SKIP_PIXEL;
   int skip_tile = 64;
   SkipNet* skipnet = nullptr;  // not owned
+  // findProbe on the host for every probe (PROBE_HOST), or screened on the GPU
+  // and verified in HP for the candidates (PROBE_GPU, the default with a device)
+  enum ProbeMode { PROBE_HOST = 0, PROBE_GPU = 1 };
+  int probe_mode = PROBE_GPU;
+  int probe_candidates_ = -1;  // probes iterated in HP by the last GPU screening
   struct TileStats {
     int ntiles = 0, accepted = 0, rejected = 0;
     uint64_t evals = 0;         // exact stability tests evaluated at the probes
@@ -101,6 +106,7 @@ class Mandelbrot {
   int precomputeAt(const char* re, const char* im);  // explicit reference point
   int referenceLength() const { return (int)ref_.len; }
   int probeWinner() const { return winner_; }
+  int probeCandidates() const { return probe_candidates_; }
   bool hasReference() const { return have_ref_; }
   const HPComplex& referencePoint() const { return X[0]; }
   const SaReferenceHost& reference() const { return ref_; }
@@ -166,6 +172,8 @@ class Mandelbrot {
   RenderStats stats_;
 
   void findProbe();
+  int findProbeGPU(Engine* eng);
+  void fillOrbit(SaReferenceHost& r) const;
   int computeOrbit(const HPComplex& X0, bool store);
   void computeSeries();
   void buildThresholds();

@@ -178,3 +178,14 @@ def test_deep_render_before_precompute_is_a_state_error():
     L = nm.lib()
     rc = L.nm_compute_all(m._h)
     assert rc in (nm.NM_ERR_STATE, nm.NM_ERR_NODEVICE)
+
+
+def test_probe_mode_argument_checks():
+    """nm_set_probe_mode takes NM_PROBE_HOST or NM_PROBE_GPU only; without a
+    device precompute falls back to the host search (every probe in HP)."""
+    nm = _nm()
+    m = nm.Mandelbrot(8, 8)
+    with pytest.raises(nm.NewmanError):
+        m.set_probe_mode(2)
+    m.set_probe_mode(nm.PROBE_HOST)
+    m.set_probe_mode(nm.PROBE_GPU)
