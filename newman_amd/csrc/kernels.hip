@@ -962,6 +962,9 @@ __device__ __forceinline__ void pert_block(double& dr, double& di, const double 
     dr = ndr;
     di = ndi;
     hoff += 16;
+    // the escape record is complete at the end of each step (no chain of
+    // per-step values kept live to the end of the block)
+    __asm__ volatile("" ::"v"(esc), "v"(esc_r2));
   }
   m = moved ? (int)(hoff >> 4) : m + K;
   rb += moved ? 1u : 0u;
