@@ -1762,7 +1762,7 @@ class EngineImpl : public Engine {
       a.skip_ntc = ntc;
       a.skip_tpf = ntc * ntr;
     }
-    const int K = opt.block_k > 0 ? opt.block_k : 8;
+    const int K = opt.block_k > 0 ? opt.block_k : 12;
     const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
     const bool fx = opt.delta_mode == DELTA_FP32X;
     if (stats) {
@@ -1773,6 +1773,9 @@ class EngineImpl : public Engine {
       case 8:
         return fx ? run(a, k_series_start, k_series_iterate<8, true>, waves, opt, stats, nout, host_out, emit_to)
                   : run(a, k_series_start, k_series_iterate<8, false>, waves, opt, stats, nout, host_out, emit_to);
+      case 12:
+        return fx ? run(a, k_series_start, k_series_iterate<12, true>, waves, opt, stats, nout, host_out, emit_to)
+                  : run(a, k_series_start, k_series_iterate<12, false>, waves, opt, stats, nout, host_out, emit_to);
       case 16:
         return fx ? run(a, k_series_start, k_series_iterate<16, true>, waves, opt, stats, nout, host_out, emit_to)
                   : run(a, k_series_start, k_series_iterate<16, false>, waves, opt, stats, nout, host_out, emit_to);

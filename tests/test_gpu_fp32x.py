@@ -39,7 +39,7 @@ def _pair(name, nr, nc, width):
 
 
 @pytest.mark.parametrize("name,nr,nc,width", CASES)
-@pytest.mark.parametrize("K", [8, 16])
+@pytest.mark.parametrize("K", [8, 12, 16])
 def test_fp32x_bit_exact_vs_oracle(gpu, name, nr, nc, width, K):
     m, o = _pair(name, nr, nc, width)
     m.set_delta_mode(1)

@@ -158,6 +158,24 @@ def test_series_path_bit_exact_vs_method_oracle(gpu, name, nr, nc, width):
     assert st["n_iterated"] + st["n_sa_resolved"] + st["n_cardioid"] == nr * nc
 
 
+@pytest.mark.parametrize("K", [12, 16, 32])
+@pytest.mark.parametrize("name,nr,nc,width", [DEEP_CASES[4], DEEP_CASES[7]])
+def test_series_block_size_keeps_results(gpu, name, nr, nc, width, K):
+    """A block only groups K steps of the same per-step arithmetic (escape record,
+    rebase, forced rebase at the orbit's end), so every block size gives the
+    oracle's grid bit for bit."""
+    m, o = _deep(name, nr, nc, width)
+    m.precompute()
+    o.precompute()
+    st = m.render(block_k=K)
+    assert st["block_k"] == K
+    g = m.grid()
+    rc = np.array([(r, c) for r in range(nr) for c in range(nc)], dtype=np.int32)
+    ref = o.render_samples(rc, pert=True).reshape(nr, nc)
+    assert np.array_equal(g["iterations"], ref["iterations"])
+    assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
+
+
 @pytest.mark.parametrize("name,nr,nc,width", DEEP_CASES[:4])
 def test_series_path_vs_reference_hp(gpu, name, nr, nc, width):
     """Against the reference method (SA scan + HP continuation, mandelbrot.cpp:144-229):
