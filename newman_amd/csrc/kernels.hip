@@ -88,6 +88,7 @@ constexpr int kHead = 64;               // orbit padding past its end (>= max K)
 // reference); lanes whose block fits a window read it from LDS, the others
 // gather from global memory, whose cache tag lookups bound the gathers
 constexpr int kLdsHead = 1024, kLdsTail = 1024;
+static_assert(kLdsTail >= 32 && kLdsHead >= 32, "a block of up to 32 steps reads inside one window");
 constexpr int kPartSlots = 8192;        // per-workgroup partial counter slots
 
 // partial-slot ranges of the kernels of one render
@@ -922,51 +923,75 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
 // zb holds Zd[m0 .. m0+K-1]; a lane that rebased reads the orbit head from LDS.
 // The first escape (exact sqMag > bailout^2, mandelbrot.cpp:60) records its step
 // and |Y|^2 and poisons the lane with NaN so later steps cannot escape again.
-template <int K, bool END>
+typedef double ld_double2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const ld_double2* lds_orbit;  // the orbit in LDS
+typedef __attribute__((address_space(1))) const ld_double2* hbm_orbit;  // the orbit in HBM
+
+// p - j orbit entries, as an address (the result may point before the array; it
+// is only read at offsets j' > j)
+template <class P>
+__device__ __forceinline__ P orbit_back(P head, int j) {
+  return (P)((uintptr_t)head - (uintptr_t)(16 * j));
+}
+template <class P>
+__device__ __forceinline__ int orbit_index(P p, P head) {
+  return (int)(((intptr_t)p - (intptr_t)head) >> 4);
+}
+
+// One block of K perturbation steps for a lane (Y_it = Zd[m] + d):
+//   d' = d (Zd[m] + Y) + e, rebase to the orbit start (m = 0, d = Y) when
+//   |Y| < |d| (or, in END blocks, when m reaches the end of the reference).
+// Step j reads its orbit entry at zp[j], zp = the lane's Zd[m0] in LDS or HBM
+// (one memory space per wave and block); a rebase in step j moves zp to
+// head - j, so step j' > j reads Zd[j' - j].  Each entry is loaded one step
+// ahead, as soon as that step's rebase is decided.
+// The first escape (exact sqMag > bailout^2, mandelbrot.cpp:60) records its
+// step and |Y|^2.  Once |Y| > bailout >= 2, |Y| grows every step (|Y^2 + c| >=
+// |Y|^2 - 2, and the rounding of the perturbed form is many orders below that
+// growth; overflow gives inf and then NaN, which fails the test), so the first
+// escape is the one step where the test holds and did not hold one step before.
+template <int K, bool END, class P>
 __device__ __forceinline__ void pert_block(double& dr, double& di, const double er, const double ei, int& m,
-                                           const int it, double2* zb, const double2* zhead, const int endj,
-                                           const int mend, const double b2, int& esc, double& esc_r2,
-                                           unsigned int& rb) {
+                                           const int it, P zp, const P head, const int endj, const int mend,
+                                           const double b2, int& esc, double& esc_r2, unsigned int& rb) {
   bool moved = false;
-  unsigned int hoff = 0;
+  int escj = -1;  // step of the block's first escape
+  bool eprev = false;
+  ld_double2 z = zp[0];
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    // zb[j] is dead after this step: a lane that rebased overwrites it in place
-    if (moved) zb[j] = *(const double2*)((const char*)zhead + hoff);
-    const double zr = zb[j].x;
-    const double zi = zb[j].y;
+    const double zr = z.x;
+    const double zi = z.y;
     const double yr = zr + dr, yi = zi + di;
     const double r2 = yr * yr + yi * yi;  // sqMag (mandelbrot.cpp:60)
-    // the first escape records its step and |Y|^2; a NaN high word in dr reaches
-    // every later step, so no later step escapes (selects, no branch)
-    const bool e = r2 > b2;
-    esc = e ? it + j : esc;
-    esc_r2 = e ? r2 : esc_r2;
-    dr = __hiloint2double(e ? 0x7ff80000 : __double2hiint(dr), __double2loint(dr));
     const double dmag = fma(dr, dr, di * di);
-    double wr = zr + yr, wi = zi + yi;
     bool reb = r2 < dmag;
-    // the forced rebase at the orbit's end, m == mend: m = m0 + j until a
-    // rebase, then hoff / 16
-    if (END) reb = reb || (moved ? (int)hoff == 16 * mend : j == endj);
+    // the forced rebase at the orbit's end, m == mend
+    if (END) reb = reb || (moved ? orbit_index(zp, head) + j == mend : j == endj);
+    double wr = zr + yr, wi = zi + yi;
+    double d0r = dr, d0i = di;
     if (reb) {  // rebase: d = Y against Zd[0] = 0, so W = Y
-      dr = yr;
-      di = yi;
+      d0r = yr;
+      d0i = yi;
       wr = yr;
       wi = yi;
-      hoff = 0;
+      zp = orbit_back(head, j);
       moved = true;
     }
-    const double ndr = fma(dr, wr, fma(-di, wi, er));
-    const double ndi = fma(dr, wi, fma(di, wr, ei));
-    dr = ndr;
-    di = ndi;
-    hoff += 16;
+    if (j + 1 < K) z = zp[j + 1];
+    const bool e = r2 > b2;
+    const bool first = e && !eprev;
+    eprev = e;
+    escj = first ? j : escj;
+    esc_r2 = first ? r2 : esc_r2;
+    dr = fma(d0r, wr, fma(-d0i, wi, er));
+    di = fma(d0r, wi, fma(d0i, wr, ei));
     // the escape record is complete at the end of each step (no chain of
     // per-step values kept live to the end of the block)
-    __asm__ volatile("" ::"v"(esc), "v"(esc_r2));
+    __asm__ volatile("" ::"v"(escj), "v"(esc_r2));
   }
-  m = moved ? (int)(hoff >> 4) : m + K;
+  esc = escj >= 0 ? it + escj : esc;
+  m = moved ? orbit_index(zp, head) + K : m + K;
   rb += moved ? 1u : 0u;
 }
 
@@ -1045,17 +1070,19 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
   const uint32_t W = gridDim.x * (blockDim.x >> 6);
   uint32_t executed = 0, rebased = 0;  // per lane (< 2^32 steps per lane per launch)
   const int N = c.N, mend = a.mend;
-  // the orbit's head and tail windows (zlds[0, kLdsHead) = Zd[0, kLdsHead),
-  // zlds[kLdsHead + t] = Zd[tail_lo + t]); entries past Zd's allocation read 0
-  __shared__ double2 zlds[kLdsHead + kLdsTail];
+  // the orbit's tail and head windows (zlds[t] = Zd[tail_lo + t],
+  // zlds[kLdsTail + h] = Zd[h] for h < kLdsHead); entries past Zd's allocation
+  // read 0.  The head sits above 16 K positions so that a rebased lane's base
+  // address (head - 16 j) stays non-negative.
+  __shared__ double2 zlds[kLdsTail + kLdsHead];
   const int tail_lo = max(kLdsHead, min(a.zd_count, a.len + 2 + K) - kLdsTail);
   const int tail_hi = tail_lo + kLdsTail;
   for (int i = threadIdx.x; i < kLdsHead + kLdsTail; i += blockDim.x) {
-    const int src = i < kLdsHead ? i : tail_lo + (i - kLdsHead);
+    const int src = i < kLdsTail ? tail_lo + i : i - kLdsTail;
     zlds[i] = src < a.zd_count ? a.Zd[src] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const double2* zhead = zlds;  // rebased lanes read the orbit from its start
+  const double2* zhead = zlds + kLdsTail;  // rebased lanes read the orbit from its start
   uint32_t ccur = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   uint32_t ncur = chunk_items(ccur, total), nnext = chunk_items(ccur + W, total);
   uint32_t cur_pix = load_pix(c.active, ccur, total), nxt_pix = load_pix(c.active, ccur + W, total);
@@ -1132,22 +1159,22 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
     }
     // lanes whose block crosses the reference's end rebase there (END variant)
     const bool end_zone = __ballot(act && m + K > mend) != 0 || mend <= K;
-    double2 zb[K];
     const bool in_head = m + K <= kLdsHead;
     const bool in_tail = !in_head && m >= tail_lo && m + K <= tail_hi;
-    if (in_head || in_tail) {
-      const double2* src = zlds + (in_head ? m : kLdsHead + m - tail_lo);
-#pragma unroll
-      for (int j = 0; j < K; j++) zb[j] = src[j];
-    } else {
-#pragma unroll
-      for (int j = 0; j < K; j++) zb[j] = a.Zd[m + j];
-    }
     int esc = -1;
     double esc_r2 = 0.0;
     unsigned int rb = 0;
     const int it0 = it;
     if (FX) {
+      double2 zb[K];
+      if (in_head || in_tail) {
+        const double2* src = in_head ? zhead + m : zlds + (m - tail_lo);
+#pragma unroll
+        for (int j = 0; j < K; j++) zb[j] = src[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) zb[j] = a.Zd[m + j];
+      }
       const float b2g = (float)(c.b2 * (1.0 - 1.0 / 1024.0));
       if (end_zone)
         pert_block_fx<K, true>(wr, wi, ex, esr, esi, er, ei, m, it, zb, zhead, mend - m, mend, c.b2, b2g, esc,
@@ -1167,10 +1194,21 @@ __global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
           esi = (float)ldexp(ei, -ex);
         }
       }
-    } else if (end_zone) {
-      pert_block<K, true>(dr, di, er, ei, m, it, zb, zhead, mend - m, mend, c.b2, esc, esc_r2, rb);
+    } else if (__ballot(act && !in_head && !in_tail) == 0) {
+      // every lane's block lies in an LDS window: the whole block reads LDS
+      const lds_orbit lhead = (lds_orbit)(const void*)zhead;
+      const lds_orbit zp = in_head ? lhead + m : (lds_orbit)(const void*)zlds + (m - tail_lo);
+      if (end_zone)
+        pert_block<K, true>(dr, di, er, ei, m, it, zp, lhead, mend - m, mend, c.b2, esc, esc_r2, rb);
+      else
+        pert_block<K, false>(dr, di, er, ei, m, it, zp, lhead, 0, mend, c.b2, esc, esc_r2, rb);
     } else {
-      pert_block<K, false>(dr, di, er, ei, m, it, zb, zhead, 0, mend, c.b2, esc, esc_r2, rb);
+      // the orbit from HBM (L2) for the whole wave
+      const hbm_orbit ghead = (hbm_orbit)(const void*)a.Zd;
+      if (end_zone)
+        pert_block<K, true>(dr, di, er, ei, m, it, ghead + m, ghead, mend - m, mend, c.b2, esc, esc_r2, rb);
+      else
+        pert_block<K, false>(dr, di, er, ei, m, it, ghead + m, ghead, 0, mend, c.b2, esc, esc_r2, rb);
     }
     if (act) {
       it = it0 + K;
@@ -1762,9 +1800,11 @@ class EngineImpl : public Engine {
       a.skip_ntc = ntc;
       a.skip_tpf = ntc * ntr;
     }
-    const int K = opt.block_k > 0 ? opt.block_k : 12;
-    const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
     const bool fx = opt.delta_mode == DELTA_FP32X;
+    // FP64 blocks hold one orbit entry per lane whatever K is (76 VGPRs), so the
+    // longest block amortises its set-up best; FP32X blocks preload K entries
+    const int K = opt.block_k > 0 ? opt.block_k : fx ? 8 : 32;
+    const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
     if (stats) {
       stats->block_k = K;
       stats->waves_per_simd = waves;

@@ -1044,7 +1044,7 @@ int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts*
 
 // The project's launch autotuner.  Candidates: sample order (row-major, 32x32,
 // 64x64 and 128x128 tiles), persistent waves per SIMD (2, 4, 6, 8) and the block
-// size (8, 12, 16 for the series kernel; 16, 32 for the plain one).  Each candidate
+// size (8, 16, 32 for the series kernel; 16, 32 for the plain one).  Each candidate
 // renders an evenly spaced subsample of at most 16 frames (so the set's depth
 // mix is kept) once untimed and once timed; the fastest total kernel time wins.
 int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vector<Launch>* table) {
@@ -1055,7 +1055,7 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
   const size_t step = std::max<size_t>(1, (n + 15) / 16);
   const int tiles[4] = {0, 32, 64, 128};
   const int waves[4] = {2, 4, 6, 8};
-  const int ks_sa[3] = {8, 12, 16}, ks_hw[2] = {16, 32};
+  const int ks_sa[3] = {8, 16, 32}, ks_hw[2] = {16, 32};
   const int nks = set.hw ? 2 : 3;
   Launch best;
   best.ms = 1e300;
