@@ -88,6 +88,10 @@ constexpr int kHead = 64;               // orbit padding past its end (>= max K)
 // reference); lanes whose block fits a window read it from LDS, the others
 // gather from global memory, whose cache tag lookups bound the gathers
 constexpr int kLdsHead = 1024, kLdsTail = 1024;
+// The series iterate kernel runs 8 waves per workgroup: the 32 KB of LDS orbit
+// windows is shared by 8 waves, so LDS does not cap the waves a CU holds below
+// what the registers allow (6 per SIMD at 76 VGPRs).
+constexpr int kSeriesIterThreads = 512;
 static_assert(kLdsTail >= 32 && kLdsHead >= 32, "a block of up to 32 steps reads inside one window");
 constexpr int kPartSlots = 8192;        // per-workgroup partial counter slots
 
@@ -1064,7 +1068,7 @@ __device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, flo
 // from 64-sample chunks; blocks of K steps with the block's orbit entries loaded
 // up front.
 template <int K, bool FX>
-__global__ __launch_bounds__(kThreads) void k_series_iterate(SeriesArgs a) {
+__global__ __launch_bounds__(kSeriesIterThreads) void k_series_iterate(SeriesArgs a) {
   const Common& c = a.c;
   const uint32_t total = *c.nactive;
   const uint32_t W = gridDim.x * (blockDim.x >> 6);
@@ -1307,6 +1311,12 @@ __global__ void k_zero(unsigned long long* p, int n) {
     if (e_ != hipSuccess) return fail(#expr, e_); \
   } while (0)
 
+// threads per workgroup of a path's iterate kernel
+template <class A>
+constexpr int iter_threads() { return kThreads; }
+template <>
+constexpr int iter_threads<SeriesArgs>() { return kSeriesIterThreads; }
+
 class EngineImpl : public Engine {
  public:
   hipStream_t stream = nullptr;
@@ -1494,7 +1504,9 @@ class EngineImpl : public Engine {
     const Common& c = args.c;
     const uint32_t ntiles = (uint32_t)((c.total + kTile - 1) / kTile);
     const int gtile = grid_for_tiles(c.total);
-    const int giter = std::min(num_cu * waves, kSlotSum - kSlotIter);
+    // waves = persistent waves per SIMD: 4 per CU for each 256 threads
+    const int ti = iter_threads<Args>();
+    const int giter = std::max(1, std::min(num_cu * waves * 256 / ti, kSlotSum - kSlotIter));
     hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartSlots * C_COUNT);
     NM_HIP(hipGetLastError());
     if (evs) NM_HIP(hipEventRecord(evs[0], stream));
@@ -1507,7 +1519,7 @@ class EngineImpl : public Engine {
                        (const uint32_t*)c.tile_count, c.total, c.active);
     NM_HIP(hipGetLastError());
     if (evs) NM_HIP(hipEventRecord(evs[2], stream));
-    hipLaunchKernelGGL(iter, dim3(giter), dim3(kThreads), 0, stream, args);
+    hipLaunchKernelGGL(iter, dim3(giter), dim3(ti), 0, stream, args);
     NM_HIP(hipGetLastError());
     if (evs) NM_HIP(hipEventRecord(evs[3], stream));
     hipLaunchKernelGGL(k_finalize, dim3(gtile), dim3(kThreads), 0, stream, c);
