@@ -60,6 +60,12 @@ struct SaReferenceHost {
   // log2 |B_j|^2 and log2 |C_j|^2 (over j >= i).
   std::vector<double> T, ST, BNs, CNs;
   int st_levels = 0;
+  // Lookup of the stop search: PMT[k] = first i >= 1 with PM[i] <= pmt_base + k h
+  // (len if none), k = 0 .. pmt_n, h = 1 / pmt_inv_h, over the finite range of
+  // PM; pmt_n = 0 when PM has fewer than two distinct finite values.
+  std::vector<int32_t> PMT;
+  double pmt_base = 0.0, pmt_inv_h = 0.0;
+  int pmt_n = 0;
 };
 
 struct RenderOpts {

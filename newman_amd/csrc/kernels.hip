@@ -148,6 +148,9 @@ struct SeriesArgs {
   const double* ST;   // range-minimum sparse table of the thresholds (st_levels x len)
   const double* BNs;  // suffix minima of log2 |B_j|^2, |C_j|^2
   const double* CNs;
+  const int* PMT;  // stop-search lookup over the finite PM range (pmt_n + 1 entries)
+  double pmt_base, pmt_inv_h;
+  int pmt_n;
   int st_levels;
   int len;
   int mend;  // len + 1
@@ -644,38 +647,53 @@ __device__ __forceinline__ int first_le(const double* PM, int lo, int hi, double
 // under- or overflow.  Otherwise the reference's linear scan runs.
 constexpr double kMarg = 4.0e-9;
 
-// log2 |e|^2 without under/overflow; -inf for e = 0
+// log2 q for a positive normal q, within 3e-15 absolute (measured against a
+// long-double log2 on 5e7 points of [2^-20, 2^20]): q = m 2^E with m in
+// [1/sqrt 2, sqrt 2), log2 m = (2 / ln 2) atanh(s), s = (m - 1) / (m + 1),
+// |s| < 0.1716, by the odd series through s^17 (the next term is below 4e-16).
+// Every use compares the result with thresholds through a margin of 4e-9, so
+// this bound, not correct rounding, is what the search needs.
+__device__ __forceinline__ double log2_near(double q) {
+  int E;
+  double m = frexp(q, &E);  // [0.5, 1)
+  if (m < 0.70710678118654752) {
+    m *= 2.0;
+    E -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 17.0;
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  p = fma(p, s2, 1.0);
+  return fma(2.8853900817779268147 * s, p, (double)E);  // 2 / ln 2
+}
+
+// log2 |e|^2 without under/overflow (to about 1e-13 absolute); -inf for e = 0
 __device__ __forceinline__ double log2_sqmag(double2 e) {
   double mx = fmax(fabs(e.x), fabs(e.y));
   if (mx == 0.0) return -INFINITY;
   int ex = ilogb(mx);
   double sx = ldexp(e.x, -ex), sy = ldexp(e.y, -ex);
-  return log2(sx * sx + sy * sy) + 2.0 * ex;
+  return log2_near(sx * sx + sy * sy) + 2.0 * ex;
 }
 
-__device__ __forceinline__ double wave_uniform(double x) {  // value of lane 0, in SGPRs
-  long long b = __double_as_longlong(x);
-  int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
-  int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// [lo, hi]: the range that holds first_le(PM, 1, last, L + kMarg) for every lane
-// of the wave with need set.  Called by all lanes; PM is searched with scalar
-// loads at the wave's largest and smallest L.
-__device__ __forceinline__ int2 wave_stop_bracket(const SeriesArgs& a, double L, bool need) {
-  double lmax = need ? L : -INFINITY, lmin = need ? L : INFINITY;
-  for (int off = 32; off > 0; off >>= 1) {
-    lmax = fmax(lmax, __shfl_xor(lmax, off));
-    lmin = fmin(lmin, __shfl_xor(lmin, off));
-  }
-  lmax = wave_uniform(lmax);
-  lmin = wave_uniform(lmin);
+// [lo, hi] holding first_le(PM, 1, last, v) (or hi + 1 = the answer's lower
+// bound when v lies below the lookup's range), from the lookup cell of v widened
+// by one cell on each side against the rounding of the cell index
+__device__ __forceinline__ int2 stop_bracket(const SeriesArgs& a, double v) {
   const int last = a.len - 1;
-  if (last < 1 || !(lmax >= lmin)) return make_int2(1, last);
-  int lo = first_le(a.PM, 1, last, lmax + kMarg);
-  int hi = first_le(a.PM, lo, last, lmin + kMarg);
-  return make_int2(lo, min(hi, last));
+  if (a.pmt_n == 0 || !(v == v)) return make_int2(1, last);
+  const double x = (v - a.pmt_base) * a.pmt_inv_h;
+  const int k = x < 0.0 ? 0 : x >= (double)(a.pmt_n - 1) ? a.pmt_n - 1 : (int)x;
+  const int lo = a.PMT[min(k + 2, a.pmt_n)];
+  const int hi = a.PMT[max(k - 1, 0)];
+  return make_int2(max(lo, 1), min(hi, last));
 }
 
 // last i in [1, last] with the non-decreasing CX[i] < v, or 0
@@ -765,7 +783,7 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
     ua = 0;
     h = 1;
   } else if (mode == 1) {
-    ua = sa_stop_index(a, L, make_int2(1, last), e2, e3);
+    ua = sa_stop_index(a, L, stop_bracket(a, L + kMarg), e2, e3);
     h = 1;
   } else if (mode == 2) {
     const int p = pred[t];
@@ -835,7 +853,7 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
         }
       }
       // phase 2 (all lanes): the wave's bracket of the stop-index search
-      const int2 br = a.tile_ua ? make_int2(1, a.len - 1) : wave_stop_bracket(a, L, valid && !in && L > -INFINITY);
+      const int2 br = a.tile_ua ? make_int2(1, a.len - 1) : stop_bracket(a, L + kMarg);
       uint32_t skip_tile = 0;
       if (a.tile_ua && valid) {
         const uint32_t fi = frame_of(gid, c.inv_frame);
@@ -1352,6 +1370,9 @@ class EngineImpl : public Engine {
   const int* rSP = nullptr;
   const double *rST = nullptr, *rBNs = nullptr, *rCNs = nullptr;
   int rst_levels = 0;
+  const int* rPMT = nullptr;
+  int rpmt_n = 0;
+  double rpmt_base = 0.0, rpmt_inv_h = 0.0;
   int ref_zd_count = 0;
 
   int fail(const char* what, hipError_t e) {
@@ -1406,7 +1427,7 @@ class EngineImpl : public Engine {
            oB = place((size_t)len * 16), oC = place((size_t)len * 16), oPM = place((size_t)len * 8),
            oBX = place((size_t)len * 8), oBN = place((size_t)len * 8), oCX = place((size_t)len * 8),
            oCN = place((size_t)len * 8), oSP = place((size_t)len * 4), oST = place(ref.ST.size() * 8),
-           oBNs = place(ref.BNs.size() * 8), oCNs = place(ref.CNs.size() * 8);
+           oBNs = place(ref.BNs.size() * 8), oCNs = place(ref.CNs.size() * 8), oPMT = place(ref.PMT.size() * 4);
     std::vector<char> h(off, 0);
     std::memcpy(h.data() + oZ, ref.Zd.data(), ref.Zd.size() * 8);
     std::memcpy(h.data() + oXlo, ref.Xlo.data(), ref.Xlo.size() * 8);
@@ -1422,6 +1443,7 @@ class EngineImpl : public Engine {
     std::memcpy(h.data() + oST, ref.ST.data(), ref.ST.size() * 8);
     std::memcpy(h.data() + oBNs, ref.BNs.data(), ref.BNs.size() * 8);
     std::memcpy(h.data() + oCNs, ref.CNs.data(), ref.CNs.size() * 8);
+    if (!ref.PMT.empty()) std::memcpy(h.data() + oPMT, ref.PMT.data(), ref.PMT.size() * 4);
     int st = grow(&d_ref, &n_ref_bytes, off);
     if (st) return st;
     NM_HIP(hipMemcpyAsync(d_ref, h.data(), off, hipMemcpyHostToDevice, stream));
@@ -1442,6 +1464,10 @@ class EngineImpl : public Engine {
     rBNs = (const double*)(b + oBNs);
     rCNs = (const double*)(b + oCNs);
     rst_levels = ref.st_levels;
+    rPMT = (const int*)(b + oPMT);
+    rpmt_n = ref.PMT.empty() ? 0 : ref.pmt_n;
+    rpmt_base = ref.pmt_base;
+    rpmt_inv_h = ref.pmt_inv_h;
     ref_len = len;
     ref_serial = serial;
     return NM_OK;
@@ -1854,6 +1880,10 @@ class EngineImpl : public Engine {
     a.BNs = rBNs;
     a.CNs = rCNs;
     a.st_levels = rst_levels;
+    a.PMT = rPMT;
+    a.pmt_n = rpmt_n;
+    a.pmt_base = rpmt_base;
+    a.pmt_inv_h = rpmt_inv_h;
     a.len = ref_len;
     a.mend = ref_len + 1;
     a.zd_count = ref_zd_count;

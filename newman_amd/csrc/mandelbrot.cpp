@@ -381,6 +381,32 @@ void Mandelbrot::buildThresholds() {
     ref_.BNs[i] = std::min(ref_.BNs[i + 1], special ? INFINITY : log2_sqmag(br, bi));
     ref_.CNs[i] = std::min(ref_.CNs[i + 1], special ? INFINITY : log2_sqmag(cr, ci));
   }
+  // the stop-search lookup: cells of the finite PM range, each holding the first
+  // index whose prefix minimum reaches the cell's lower edge
+  {
+    double pmax = -INFINITY, pmin = INFINITY;
+    for (int i = 1; i < len; i++)
+      if (std::isfinite(ref_.PM[i])) {
+        pmax = std::max(pmax, ref_.PM[i]);
+        pmin = std::min(pmin, ref_.PM[i]);
+      }
+    ref_.PMT.clear();
+    ref_.pmt_n = 0;
+    if (pmax > pmin) {
+      const int n = (int)std::min<long>(65536, 8L * len);
+      const double h = (pmax - pmin) / n;
+      ref_.pmt_base = pmin;
+      ref_.pmt_inv_h = 1.0 / h;
+      ref_.pmt_n = n;
+      ref_.PMT.assign((size_t)n + 1, len);
+      int i = len;  // first index with PM[i] <= v; non-increasing in v
+      for (int k = 0; k <= n; k++) {
+        const double v = pmin + k * h;
+        while (i > 1 && ref_.PM[i - 1] <= v) i--;
+        ref_.PMT[k] = i;
+      }
+    }
+  }
   int K = 1;
   while ((1 << K) <= len) K++;
   ref_.st_levels = K;
