@@ -156,6 +156,7 @@ struct SeriesArgs {
   int mend;  // len + 1
   int zd_count;  // entries allocated in Zd (past len + 1: zero padding)
   double tol;
+  double ltol;  // log2 tol
   // per-tile series skip (the first unstable index of each tile's probe); null:
   // every sample searches its own (the reference's per-pixel scan)
   const int* tile_ua;
@@ -707,6 +708,23 @@ __device__ __forceinline__ int last_lt(const double* CX, int last, double v) {
   return l - 1;
 }
 
+// When a threshold certifies the exact test stable.  T_j > L + kMarg means
+// bmag tol >= cmag (1 + 2.7e-9) in exact arithmetic.  The computed test
+// (bmag tol < cmag, mandelbrot.cpp:138-142) is then stable when bmag tol >=
+// 2^-1030: its own rounding is then under 2^-44 relative; the computed cmag is
+// off by under 2^-37 relative (for |e|^2 > 2^-690, e^3 lies at most 21 bits in
+// the subnormals) plus 2^-1072 absolute from underflow, which is below 2.7e-9 of
+// bmag tol; no product overflows while bmag and cmag stay below 2^1000 (and
+// |e|^2 below 2^600).  No lower bound on cmag is needed: a tiny cmag only makes
+// the test more stable.  bmin: least log2 |B_j|^2, bmax/cmax: greatest
+// log2 |B_j|^2, |C_j|^2 over the indices concerned.
+constexpr double kLogLo = -1030.0, kLogHi = 1000.0, kLMin = -690.0, kLMax = 600.0;
+__device__ __forceinline__ bool rounding_certified(const SeriesArgs& a, double L, double bmin, double bmax,
+                                                   double cmax) {
+  return L > kLMin && L < kLMax && bmin + 2.0 * L + a.ltol > kLogLo && bmax + 2.0 * L < kLogHi &&
+         cmax + 3.0 * L < kLogHi;
+}
+
 // first i >= lo with T_i <= v (last + 1 if none), by the sparse table
 __device__ __forceinline__ int first_le_from(const SeriesArgs& a, int lo, double v) {
   const int len = a.len;
@@ -722,19 +740,19 @@ __device__ __forceinline__ int first_le_from(const SeriesArgs& a, int lo, double
 // threshold says: cmag rounds to 0 and bmag tol < 0 is false; that holds for a
 // whole prefix [1, ic] (CX is a prefix maximum, no special index inside).  From
 // ic + 1 on, the threshold table applies where every exact-test intermediate of
-// the range [ic + 1, ia) stays normal with margin (suffix minima from ic + 1,
-// prefix maxima up to ia - 1; e^3 no more than 30 bits into the subnormals).
+// the range [ic + 1, ia) is certified by rounding_certified (suffix minima from
+// ic + 1, prefix maxima up to ia - 1).
 __device__ int deep_start(const SeriesArgs& a, double L) {
   const int last = a.len - 1;
   int ic = last_lt(a.CX, last, -1080.0 - 3.0 * L);
   if (ic > 0 && a.SP[ic] != 0) ic = 0;
   const int lo = ic + 1;
-  if (lo > last || !(L > -696.0 && L < 600.0)) return lo;
+  if (lo > last || !(L > kLMin && L < kLMax)) return lo;
   const int ia = first_le_from(a, lo, L + kMarg);
   const int sidx = min(ia, last + 1) - 1;  // the last index the table certifies
   if (sidx < lo) return lo;
-  const bool ok = a.SP[sidx] - a.SP[lo - 1] == 0 && a.BNs[lo] + 2.0 * L > -900.0 && a.CNs[lo] + 3.0 * L > -960.0 &&
-                  a.BX[sidx] + 2.0 * L < 960.0 && a.CX[sidx] + 3.0 * L < 960.0;
+  const bool ok = a.SP[sidx] - a.SP[lo - 1] == 0 &&
+                  rounding_certified(a, L, a.BNs[lo], a.BX[sidx], a.CX[sidx]);
   return ok ? ia : lo;
 }
 
@@ -745,11 +763,10 @@ __device__ int sa_stop_index(const SeriesArgs& a, double L, int2 br, double2 e2,
   int ia = first_le(a.PM, br.x, br.y, L + kMarg);
   if (ia > br.y && br.y < last) ia = first_le(a.PM, br.y + 1, last, L + kMarg);
   int safe_idx = ia > last ? last : ia - 1;
-  bool safe = L > -600.0 && L < 600.0;
-  if (safe && safe_idx >= 1) {
-    safe = a.SP[safe_idx] == 0 && a.BX[safe_idx] + 2.0 * L < 960.0 && a.BN[safe_idx] + 2.0 * L > -900.0 &&
-           a.CX[safe_idx] + 3.0 * L < 960.0 && a.CN[safe_idx] + 3.0 * L > -960.0;
-  }
+  bool safe = L > kLMin && L < kLMax;
+  if (safe && safe_idx >= 1)
+    safe = a.SP[safe_idx] == 0 &&
+           rounding_certified(a, L, a.BN[safe_idx], a.BX[safe_idx], a.CX[safe_idx]);
   int from = safe ? ia : deep_start(a, L);
   for (int i = from; i <= last; i++)
     if (unstable_at(a, i, e2, e3)) return i;
@@ -789,8 +806,7 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
     const int p = pred[t];
     const int below = p > 0 ? p - 1 : last;
     bool ok = p >= 0 && p <= last && below >= 1 && a.PM[below] > L + kMarg && a.SP[below] == 0 &&
-              L > -600.0 && L < 600.0 && a.BX[below] + 2.0 * L < 960.0 && a.BN[below] + 2.0 * L > -900.0 &&
-              a.CX[below] + 3.0 * L < 960.0 && a.CN[below] + 3.0 * L > -960.0;
+              rounding_certified(a, L, a.BN[below], a.BX[below], a.CX[below]);
     if (ok && p > 0) {
       ok = unstable_at(a, p, e2, e3);
       ev = 1;
@@ -1888,6 +1904,7 @@ class EngineImpl : public Engine {
     a.mend = ref_len + 1;
     a.zd_count = ref_zd_count;
     a.tol = opt.tol;
+    a.ltol = std::log2(opt.tol);
   }
 
   int tile_skip(const std::vector<const SaFrameHost*>& frames, const RenderOpts& opt, TileSearch& ts) override {
