@@ -1596,11 +1596,11 @@ class EngineImpl : public Engine {
     if ((st = ensure_events(use_graph ? 5 + 2 * (size_t)repeats : 5 * (size_t)repeats))) return st;
     hipEvent_t* ev = evpool.data();
     const int eager = use_graph ? 1 : repeats;
-    for (int rep = 0; rep < eager; rep++)
-      if ((st = launch_pass(args, start, iter, waves, emit_to, ev + 5 * rep))) return st;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (use_graph) {
+      // captured and instantiated before any pass runs, so the host work of the
+      // instantiation does not overlap the timed eager pass
       NM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       st = launch_pass(args, start, iter, waves, emit_to, nullptr);
       hipError_t ce = hipStreamEndCapture(stream, &graph);
@@ -1614,6 +1614,10 @@ class EngineImpl : public Engine {
         (void)hipGraphDestroy(graph);
         return fail("hipGraphInstantiate", ie);
       }
+    }
+    for (int rep = 0; rep < eager && st == NM_OK; rep++)
+      st = launch_pass(args, start, iter, waves, emit_to, ev + 5 * rep);
+    if (use_graph) {
       for (int rep = 1; rep < repeats && st == NM_OK; rep++) {
         hipEvent_t* e = ev + 5 + 2 * (rep - 1);
         hipError_t e1 = hipEventRecord(e[0], stream);

@@ -513,7 +513,7 @@ int Mandelbrot::findProbeGPU(Engine* eng) {
     if (est[i] >= emax - kProbeMargin) cand.push_back((int)i);
   // exact lengths of the candidates, in order, in parallel batches; a length of
   // N cannot be beaten, so the scan stops at the first batch holding one
-  int best = -1, bestlen = 0;
+  int best = -1, bestlen = 0, hp_orbits = 0;
   const size_t batch = 32;
   for (size_t b0 = 0; b0 < cand.size(); b0 += batch) {
     const size_t b1 = std::min(cand.size(), b0 + batch);
@@ -531,10 +531,11 @@ int Mandelbrot::findProbeGPU(Engine* eng) {
         bestlen = lens[k];
         best = cand[b0 + k];
       }
+    hp_orbits += (int)lens.size();
     if (bestlen >= N) break;
   }
   if (best < 0) return NM_ERR_STATE;
-  probe_candidates_ = (int)cand.size();
+  probe_candidates_ = hp_orbits;
   winner_ = best;
   HPComplex q;
   q.re = hpf(L); q.im = hpf(L);
