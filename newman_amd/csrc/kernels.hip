@@ -479,10 +479,10 @@ __device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t ch
 
 // Iterate kernel for the samples that outlive the start kernel: z <- sq(z) + z0
 // with the reference's operation order, count = index of the update that
-// escapes (mandelbrot.cpp:243-246).  A block runs K updates with no test; escape
-// is absorbing (|z| > 1024 keeps growing, then overflows to inf/NaN), so a block
-// whose last |z|^2 is not <= bailout^2 (or that crosses N) is replayed from its
-// saved z with the test, in registers.
+// escapes (mandelbrot.cpp:243-246).  A block runs K updates and records the first
+// escape with selects: escape is absorbing (|z| > bailout >= 2 keeps growing,
+// then overflows to inf and NaN, which fails the test), so the first escape is
+// the one update whose test holds and did not hold one update before.
 template <int K>
 __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
   const Common& c = a.c;
@@ -549,8 +549,10 @@ __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
       need = __ballot(!act);
     }
     if (__ballot(act) == 0) break;
-    const double x0 = x, y0 = y;
     const int k0 = k;
+    int escj = -1;  // the block's first escaping update
+    double esc_mg = 0.0;
+    bool eprev = false;
 #pragma unroll
     for (int j = 0; j < K; j++) {
       double t = x + x;
@@ -558,46 +560,31 @@ __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
       x = r2 - i2 + cx;
       r2 = x * x;
       i2 = y * y;
+      const double mg = r2 + i2;  // sqMag (mandelbrot.cpp:245)
+      const bool e = mg > c.b2;
+      const bool first = e && !eprev;
+      eprev = e;
+      escj = first ? j : escj;
+      esc_mg = first ? mg : esc_mg;
+      // the record is complete at the end of each update (no per-update values
+      // kept live to the end of the block)
+      __asm__ volatile("" ::"v"(escj), "v"(esc_mg));
     }
-    k += K;
-    if (act && (!(r2 + i2 <= c.b2) || k > N)) {  // an escape in the block, or N inside it
-      x = x0;
-      y = y0;
-      k = k0;
-      r2 = x * x;
-      i2 = y * y;
-      bool done = false;
-      for (int j = 0; j < K; j++) {
-        if (k >= N) {
-          done = true;
-          c.out[pix] = EscapeValue{N, 0.0f};
-          break;
-        }
-        double t = x + x;
-        y = t * y + cy;
-        x = r2 - i2 + cx;
-        r2 = x * x;
-        i2 = y * y;
-        executed++;
-        double mg = r2 + i2;  // sqMag (mandelbrot.cpp:245)
-        if (mg > c.b2) {
-          done = true;
-          c.out[pix] = EscapeValue{k, 0.0f};
-          c.r2[pix] = mg;  // smoothing: k_finalize
-          break;
-        }
-        k++;
-      }
-      if (!done && k >= N) {
-        done = true;
-        c.out[pix] = EscapeValue{N, 0.0f};
-      }
-      if (done) act = false;
-    } else if (act) {
-      executed += K;
-      if (k == N) {
-        c.out[pix] = EscapeValue{N, 0.0f};
+    if (act) {
+      // the reference tests k >= N before each update: an escape at k0 + j
+      // counts only below N, and a block that reaches N ends the sample there
+      if (escj >= 0 && k0 + escj < N) {
+        c.out[pix] = EscapeValue{k0 + escj, 0.0f};
+        c.r2[pix] = esc_mg;  // smoothing: k_finalize
+        executed += (unsigned long long)(escj + 1);
         act = false;
+      } else if (k0 + K >= N) {
+        c.out[pix] = EscapeValue{N, 0.0f};
+        executed += (unsigned long long)(N - k0);
+        act = false;
+      } else {
+        executed += K;
+        k = k0 + K;
       }
     }
   }
@@ -1766,7 +1753,7 @@ class EngineImpl : public Engine {
     a.crelo = t + ncol;
     a.cim = t + 2 * ncol;
     a.cimlo = t + 2 * ncol + nrow;
-    const int K = opt.block_k > 0 ? opt.block_k : 16;
+    const int K = opt.block_k > 0 ? opt.block_k : 32;
     const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
     if (stats) {
       stats->block_k = K;
