@@ -975,10 +975,17 @@ __device__ __forceinline__ int orbit_index(P p, P head) {
 // |Y|^2 - 2, and the rounding of the perturbed form is many orders below that
 // growth; overflow gives inf and then NaN, which fails the test), so the first
 // escape is the one step where the test holds and did not hold one step before.
+// A block may end after K/2 or 3K/4 steps when at least kExitDone lanes of the
+// wave have nothing left to do in it (their sample escaped, or the lane was idle)
+// and the wave can refill them; it returns the steps it ran.
+constexpr int kExitDone = 12;
+
 template <int K, bool END, class P>
-__device__ __forceinline__ void pert_block(double& dr, double& di, const double er, const double ei, int& m,
-                                           const int it, P zp, const P head, const int endj, const int mend,
-                                           const double b2, int& esc, double& esc_r2, unsigned int& rb) {
+__device__ __forceinline__ int pert_block(double& dr, double& di, const double er, const double ei, int& m,
+                                          const int it, P zp, const P head, const int endj, const int mend,
+                                          const double b2, const bool idle, const bool can_refill, int& esc,
+                                          double& esc_r2, unsigned int& rb) {
+  int kdone = K;
   bool moved = false;
   int escj = -1;  // step of the block's first escape
   bool eprev = false;
@@ -1014,10 +1021,16 @@ __device__ __forceinline__ void pert_block(double& dr, double& di, const double 
     // the escape record is complete at the end of each step (no chain of
     // per-step values kept live to the end of the block)
     __asm__ volatile("" ::"v"(escj), "v"(esc_r2));
+    if (K >= 8 && (j + 1 == K / 2 || j + 1 == 3 * K / 4) && can_refill &&
+        __popcll(__ballot(idle || escj >= 0)) >= kExitDone) {
+      kdone = j + 1;
+      break;
+    }
   }
   esc = escj >= 0 ? it + escj : esc;
-  m = moved ? orbit_index(zp, head) + K : m + K;
+  m = moved ? orbit_index(zp, head) + kdone : m + kdone;
   rb += moved ? 1u : 0u;
+  return kdone;
 }
 
 // The FP32X form of pert_block: the delta is w * 2^ex with an FP32 mantissa
@@ -1189,7 +1202,9 @@ __global__ __launch_bounds__(kSeriesIterThreads) void k_series_iterate(SeriesArg
     int esc = -1;
     double esc_r2 = 0.0;
     unsigned int rb = 0;
+    int kdone = K;
     const int it0 = it;
+    const bool can_refill = pos < ncur || nnext > 0;  // wave-uniform
     if (FX) {
       double2 zb[K];
       if (in_head || in_tail) {
@@ -1224,21 +1239,25 @@ __global__ __launch_bounds__(kSeriesIterThreads) void k_series_iterate(SeriesArg
       const lds_orbit lhead = (lds_orbit)(const void*)zhead;
       const lds_orbit zp = in_head ? lhead + m : (lds_orbit)(const void*)zlds + (m - tail_lo);
       if (end_zone)
-        pert_block<K, true>(dr, di, er, ei, m, it, zp, lhead, mend - m, mend, c.b2, esc, esc_r2, rb);
+        kdone = pert_block<K, true>(dr, di, er, ei, m, it, zp, lhead, mend - m, mend, c.b2, !act, can_refill, esc,
+                                    esc_r2, rb);
       else
-        pert_block<K, false>(dr, di, er, ei, m, it, zp, lhead, 0, mend, c.b2, esc, esc_r2, rb);
+        kdone = pert_block<K, false>(dr, di, er, ei, m, it, zp, lhead, 0, mend, c.b2, !act, can_refill, esc, esc_r2,
+                                     rb);
     } else {
       // the orbit from HBM (L2) for the whole wave
       const hbm_orbit ghead = (hbm_orbit)(const void*)a.Zd;
       if (end_zone)
-        pert_block<K, true>(dr, di, er, ei, m, it, ghead + m, ghead, mend - m, mend, c.b2, esc, esc_r2, rb);
+        kdone = pert_block<K, true>(dr, di, er, ei, m, it, ghead + m, ghead, mend - m, mend, c.b2, !act, can_refill,
+                                    esc, esc_r2, rb);
       else
-        pert_block<K, false>(dr, di, er, ei, m, it, ghead + m, ghead, 0, mend, c.b2, esc, esc_r2, rb);
+        kdone = pert_block<K, false>(dr, di, er, ei, m, it, ghead + m, ghead, 0, mend, c.b2, !act, can_refill, esc,
+                                     esc_r2, rb);
     }
     if (act) {
-      it = it0 + K;
+      it = it0 + kdone;
       const bool escaped = esc >= 0 && esc <= N - 1;
-      executed += escaped ? (unsigned)(esc - it0 + 1) : (unsigned)min(K, N - it0);
+      executed += escaped ? (unsigned)(esc - it0 + 1) : (unsigned)min(kdone, N - it0);
       rebased += rb;
       if (escaped || it >= N) {  // tops up to N-1 decided
         c.out[pix] = EscapeValue{escaped ? esc : N, 0.0f};
