@@ -223,6 +223,36 @@ int nm_tile_targets(nm_mandelbrot* h, const int* ua, int n, float* t);
  * not owned and must outlive its use */
 int nm_set_skip_mode(nm_mandelbrot* h, int mode, int tile, nm_skipnet* net);
 int nm_get_tile_stats(const nm_mandelbrot* h, nm_tile_stats* stats);
+
+/* ---- colouring: the viewer's palette and colour lookup (next to the grid) ----
+ * MultiWaveGenerator (multiwave.h:7-37): hue cycles, a saturation cycle and
+ * luminance waves; the .pal text format of multiwave.cpp:19-73. */
+typedef struct nm_palette nm_palette;
+int nm_palette_create(nm_palette** out); /* empty: load or parse before use */
+void nm_palette_destroy(nm_palette* p);
+int nm_palette_load(nm_palette* p, const char* filename); /* load_filename, multiwave.cpp:19-48 */
+int nm_palette_save(const nm_palette* p, const char* filename); /* save_filename, multiwave.cpp:50-73 */
+int nm_palette_parse(nm_palette* p, const char* text); /* the same format from a string */
+/* MultiWaveGenerator::cache(N) (multiwave.cpp:75-116): N colours, 3 bytes (R, G,
+ * B) each, into rgb (3 N bytes).  hsl2rgb, interp and clip come from the viewer's
+ * image library, which is not part of the reference: HSL is the standard hexcone
+ * (hue in degrees), HSL channels round to the nearest byte, interpolations and
+ * averages clamp and truncate. */
+int nm_palette_cache(const nm_palette* p, int N, uint8_t* rgb);
+/* FractalViewer::getColor / colorLine (viewer.cpp:89-122) on the GPU: the grid
+ * (frame < 0) or frame `frame` of the last nm_render_frames of h, coloured with
+ * pal (npal >= N entries of 3 bytes; black at or beyond N; smooth != 0
+ * interpolates pal[it - 1] -> pal[it] by the smoothing, with pal[0] standing in
+ * for pal[-1]) and averaged over sc x sc samples, into (rows / sc) x (cols / sc)
+ * x 3 bytes of interleaved RGB, row-major.  The grid is taken from the device
+ * when it still holds h's last render (no host round trip), else from h's host
+ * grid.  ms (NULL ok): the colouring kernel's time. */
+int nm_colour(nm_mandelbrot* h, int frame, const uint8_t* pal, int npal, int smooth, int sc, uint8_t* out,
+              double* ms);
+/* the same for a grid given by the caller (nr x nc escape values, row-major) */
+int nm_colour_grid(const nm_escape_value* grid, int nr, int nc, int N, const uint8_t* pal, int npal, int smooth,
+                   int sc, uint8_t* out, double* ms);
+
 /* the text of the last engine error */
 const char* nm_last_error(void);
 

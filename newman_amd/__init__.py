@@ -74,6 +74,46 @@ PROBE_HOST, PROBE_GPU = 0, 1
 SKIPNET_INPUTS, SKIPNET_PARAMS = 132, 12737
 
 
+class Palette:
+    """newman's multi-wave palette (multiwave.h): load/parse the .pal text, then
+    cache(N) -> (N, 3) uint8 RGB."""
+
+    def __init__(self, text=None, filename=None):
+        h = _H()
+        _check(lib().nm_palette_create(C.byref(h)), "nm_palette_create")
+        self._h = h
+        if text is not None:
+            _check(lib().nm_palette_parse(self._h, _b(text)), "nm_palette_parse")
+        if filename is not None:
+            _check(lib().nm_palette_load(self._h, _b(str(filename))), "nm_palette_load")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.nm_palette_destroy(self._h)
+            self._h = None
+
+    def save(self, filename):
+        _check(lib().nm_palette_save(self._h, _b(str(filename))), "nm_palette_save")
+
+    def cache(self, N):
+        out = np.zeros((int(N), 3), dtype=np.uint8)
+        _check(lib().nm_palette_cache(self._h, int(N), out.ctypes.data), "nm_palette_cache")
+        return out
+
+
+def colour_grid(grid, N, pal, smooth=True, sc=1):
+    """FractalViewer::getColor / colorLine on the GPU for a caller's grid (an
+    ESCAPE_DTYPE array, rows x cols); returns ((rows // sc, cols // sc, 3) uint8, kernel ms)."""
+    g = np.ascontiguousarray(grid, dtype=ESCAPE_DTYPE)
+    p = np.ascontiguousarray(pal, dtype=np.uint8)
+    nr, nc = g.shape
+    out = np.zeros((nr // sc, nc // sc, 3), dtype=np.uint8)
+    ms = C.c_double(0.0)
+    _check(lib().nm_colour_grid(g.ctypes.data, nr, nc, int(N), p.ctypes.data, p.shape[0], 1 if smooth else 0,
+                                int(sc), out.ctypes.data, C.byref(ms)), "nm_colour_grid")
+    return out, ms.value
+
+
 class SkipNet:
     """The learned series-skip predictor (132-64-64-1 ReLU MLP, trained on the GPU)."""
 
@@ -191,6 +231,16 @@ _SIGS = [
     ("nm_reference_point", C.c_int, [_H, C.c_char_p, C.c_char_p, C.c_size_t]),
     ("nm_set_skip_mode", C.c_int, [_H, C.c_int, C.c_int, _H]),
     ("nm_get_tile_stats", C.c_int, [_H, C.c_void_p]),
+    ("nm_palette_create", C.c_int, [C.POINTER(_H)]),
+    ("nm_palette_destroy", None, [_H]),
+    ("nm_palette_load", C.c_int, [_H, C.c_char_p]),
+    ("nm_palette_save", C.c_int, [_H, C.c_char_p]),
+    ("nm_palette_parse", C.c_int, [_H, C.c_char_p]),
+    ("nm_palette_cache", C.c_int, [_H, C.c_int, C.c_void_p]),
+    ("nm_colour", C.c_int, [_H, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                            C.POINTER(C.c_double)]),
+    ("nm_colour_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                 C.c_void_p, C.POINTER(C.c_double)]),
     ("nm_last_error", C.c_char_p, []),
     ("nm_hp_op", C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_long, C.c_double, C.c_long,
                            C.POINTER(C.c_int), C.POINTER(C.c_long), C.c_void_p]),
@@ -339,6 +389,17 @@ class Mandelbrot:
         else:
             _check(lib().nm_at_avg(self._h, r, c, sc, C.byref(e)), "nm_at_avg")
         return e.iterations, e.smoothing
+
+    def colour(self, pal, smooth=True, sc=1, frame=-1):
+        """The viewer's colouring of this grid (frame < 0) or of frame `frame` of the
+        last render_frames, on the GPU; returns ((rows // sc, cols // sc, 3) uint8,
+        kernel ms)."""
+        p = np.ascontiguousarray(pal, dtype=np.uint8)
+        out = np.zeros((self.rows() // sc, self.cols() // sc, 3), dtype=np.uint8)
+        ms = C.c_double(0.0)
+        _check(lib().nm_colour(self._h, int(frame), p.ctypes.data, p.shape[0], 1 if smooth else 0, int(sc),
+                               out.ctypes.data, C.byref(ms)), "nm_colour")
+        return out, ms.value
 
     def grid(self):
         out = np.zeros(self.rows() * self.cols(), dtype=ESCAPE_DTYPE)

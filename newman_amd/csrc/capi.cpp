@@ -9,6 +9,7 @@
 
 #include "../../include/newman_amd.h"
 #include "mandelbrot.hpp"
+#include "palette.hpp"
 
 using nm::Mandelbrot;
 
@@ -536,6 +537,93 @@ int nm_hp_op(int op, const char* a, const char* b, long iv, double dv, long bits
       words[2 * i + 1] = (uint32_t)limb;
     }
     return 2 * L;
+  });
+}
+
+// ---- colouring
+struct nm_palette {
+  nm::MultiWave mw;
+  bool loaded = false;
+};
+
+int nm_palette_create(nm_palette** out) {
+  if (!out) return NM_ERR_ARG;
+  return guard([&] {
+    *out = new nm_palette();
+    return (int)NM_OK;
+  });
+}
+
+void nm_palette_destroy(nm_palette* p) { delete p; }
+
+int nm_palette_parse(nm_palette* p, const char* text) {
+  if (!p || !text) return NM_ERR_ARG;
+  return guard([&] {
+    nm::MultiWave w;
+    if (!w.load(text) || !w.valid()) return (int)NM_ERR_ARG;
+    p->mw = w;
+    p->loaded = true;
+    return (int)NM_OK;
+  });
+}
+
+int nm_palette_load(nm_palette* p, const char* filename) {
+  if (!p || !filename) return NM_ERR_ARG;
+  return guard([&] {
+    FILE* fp = std::fopen(filename, "r");
+    if (!fp) return (int)NM_ERR_IO;
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, fp)) > 0) text.append(buf, n);
+    std::fclose(fp);
+    nm::MultiWave w;
+    if (!w.load(text) || !w.valid()) return (int)NM_ERR_IO;
+    p->mw = w;
+    p->loaded = true;
+    return (int)NM_OK;
+  });
+}
+
+int nm_palette_save(const nm_palette* p, const char* filename) {
+  if (!p || !filename || !p->loaded) return NM_ERR_ARG;
+  return guard([&] {
+    FILE* fp = std::fopen(filename, "w");
+    if (!fp) return (int)NM_ERR_IO;
+    const std::string s = p->mw.save();
+    std::fwrite(s.data(), 1, s.size(), fp);
+    std::fclose(fp);
+    return (int)NM_OK;
+  });
+}
+
+int nm_palette_cache(const nm_palette* p, int N, uint8_t* rgb) {
+  if (!p || !rgb || N <= 0 || !p->loaded) return NM_ERR_ARG;
+  return guard([&] {
+    const std::vector<nm::Rgb> pal = p->mw.cache(N);
+    for (int i = 0; i < N; i++) {
+      rgb[3 * (size_t)i] = pal[i].r;
+      rgb[3 * (size_t)i + 1] = pal[i].g;
+      rgb[3 * (size_t)i + 2] = pal[i].b;
+    }
+    return (int)NM_OK;
+  });
+}
+
+int nm_colour(nm_mandelbrot* h, int frame, const uint8_t* pal, int npal, int smooth, int sc, uint8_t* out,
+              double* ms) {
+  if (!h || !pal || !out || sc <= 0) return NM_ERR_ARG;
+  return guard([&] { return h->m.colour(frame, pal, npal, smooth, sc, out, ms); });
+}
+
+int nm_colour_grid(const nm_escape_value* grid, int nr, int nc, int N, const uint8_t* pal, int npal, int smooth,
+                   int sc, uint8_t* out, double* ms) {
+  if (!grid || !pal || !out || nr <= 0 || nc <= 0 || sc <= 0 || N <= 0 || npal < N) return NM_ERR_ARG;
+  return guard([&] {
+    nm::Engine* eng = nullptr;
+    int st = nm::Engine::instance(&eng);
+    if (st != NM_OK) return st;
+    return eng->colour((const nm::EscapeValue*)grid, 0, 0, nr, nc, sc, N, pal, npal, smooth, out, ms);
   });
 }
 

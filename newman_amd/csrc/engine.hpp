@@ -128,6 +128,15 @@ class Engine {
   virtual int tile_skip(const std::vector<const SaFrameHost*>& frames, const RenderOpts& opt, TileSearch& ts) = 0;
   // multisample average (mandelbrot.cpp:320-332) of a fine grid on the device
   virtual int downsample(const EscapeValue* fine, int nr, int nc, int sc, EscapeValue* out) = 0;
+  // Counts renders: the result of the render that returned tag t stays on the
+  // device (row-major grids, frame after frame) while render_tag() == t.
+  virtual uint64_t render_tag() const = 0;
+  // The viewer's colouring (viewer.cpp:89-122) into (nr / sc) x (nc / sc) x 3
+  // interleaved RGB bytes: of host_grid (uploaded) when it is given, else of the
+  // device result of render `tag` from escape value `offset` on.  pal_rgb: npal
+  // entries of 3 bytes (npal >= N).  *ms: the colouring kernel's time.
+  virtual int colour(const EscapeValue* host_grid, uint64_t tag, size_t offset, int nr, int nc, int sc, int N,
+                     const uint8_t* pal_rgb, int npal, int smooth, uint8_t* host_out, double* ms) = 0;
   virtual const char* last_error() const = 0;
 };
 

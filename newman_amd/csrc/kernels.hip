@@ -48,6 +48,7 @@
 #include <string>
 #include <vector>
 
+#include "colour.hpp"
 #include "engine.hpp"
 
 namespace nm {
@@ -1378,6 +1379,11 @@ class EngineImpl : public Engine {
   void* d_state_i = nullptr; size_t n_state_i_bytes = 0;
   void* d_ref = nullptr; size_t n_ref_bytes = 0;
   void* d_grid = nullptr; size_t n_grid_bytes = 0;  // downsample staging
+  void* d_pal = nullptr; size_t n_pal_bytes = 0;    // colouring: palette (RGBA) and image
+  void* d_img = nullptr; size_t n_img_bytes = 0;
+  uint64_t tag_ = 0;                                 // render count (render_tag)
+  const EscapeValue* last_result = nullptr;          // the last render's row-major grids
+  uint64_t last_nout = 0;
   void* d_emit = nullptr; size_t n_emit_bytes = 0;  // row-major grids of a tiled sample order
   void* d_ttabs = nullptr; size_t n_ttabs_bytes = 0;  // tile search: epsilon tables
   void* d_tio = nullptr; size_t n_tio_bytes = 0;      // tile search: probes, predictions, results
@@ -1651,6 +1657,9 @@ class EngineImpl : public Engine {
       ms_graph += t / (repeats - 1);
     }
     const EscapeValue* result = emit_to ? emit_to : (const EscapeValue*)d_out;
+    tag_++;
+    last_result = result;
+    last_nout = nout;
     hipLaunchKernelGGL(k_sum_iterations, dim3(std::min(num_cu * 4, kPartSlots - kSlotSum)), dim3(kThreads), 0,
                        stream, result, (unsigned long long)nout, d_part);
     NM_HIP(hipGetLastError());
@@ -1982,6 +1991,41 @@ class EngineImpl : public Engine {
     float ms = 0;
     NM_HIP(hipEventElapsedTime(&ms, evpool[0], evpool[1]));
     ts.ms = ms;
+    return NM_OK;
+  }
+
+  uint64_t render_tag() const override { return tag_; }
+
+  int colour(const EscapeValue* host_grid, uint64_t tag, size_t offset, int nr, int nc, int sc, int N,
+             const uint8_t* pal_rgb, int npal, int smooth, uint8_t* host_out, double* ms) override {
+    if (sc <= 0 || nr <= 0 || nc <= 0 || N <= 0 || npal < N || !pal_rgb || !host_out) return NM_ERR_ARG;
+    const size_t nin = (size_t)nr * nc, nout = (size_t)(nr / sc) * (nc / sc);
+    int st;
+    const EscapeValue* grid;
+    if (host_grid) {
+      if ((st = grow(&d_grid, &n_grid_bytes, nin * sizeof(EscapeValue)))) return st;
+      NM_HIP(hipMemcpyAsync(d_grid, host_grid, nin * sizeof(EscapeValue), hipMemcpyHostToDevice, stream));
+      grid = (const EscapeValue*)d_grid;
+    } else {
+      if (tag != tag_ || !last_result || offset + nin > last_nout) return NM_ERR_STATE;
+      grid = last_result + offset;
+    }
+    std::vector<uint8_t> rgba((size_t)npal * 4, 0);
+    for (int i = 0; i < npal; i++)
+      for (int k = 0; k < 3; k++) rgba[(size_t)4 * i + k] = pal_rgb[(size_t)3 * i + k];
+    if ((st = grow(&d_pal, &n_pal_bytes, rgba.size()))) return st;
+    if ((st = grow(&d_img, &n_img_bytes, std::max<size_t>(nout * 3, 1)))) return st;
+    NM_HIP(hipMemcpyAsync(d_pal, rgba.data(), rgba.size(), hipMemcpyHostToDevice, stream));
+    if ((st = ensure_events(2))) return st;
+    NM_HIP(hipEventRecord(evpool[0], stream));
+    if (colour_launch(stream, num_cu, grid, nr, nc, sc, N, d_pal, smooth, (uint8_t*)d_img))
+      return fail("k_colour launch", hipGetLastError());
+    NM_HIP(hipEventRecord(evpool[1], stream));
+    if (nout) NM_HIP(hipMemcpyAsync(host_out, d_img, nout * 3, hipMemcpyDeviceToHost, stream));
+    NM_HIP(hipStreamSynchronize(stream));
+    float t = 0.0f;
+    NM_HIP(hipEventElapsedTime(&t, evpool[0], evpool[1]));
+    if (ms) *ms = t;
     return NM_OK;
   }
 

@@ -773,6 +773,7 @@ int Mandelbrot::render(RenderOpts* opts, bool copy_back) {
   if (st == NM_OK) {
     rendered_ = copy_back;
     snap_ = snapshot();
+    dev_grid_ = true;
   }
   return st;
 }
@@ -828,6 +829,18 @@ int Mandelbrot::renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue*
   Engine* eng = nullptr;
   int st = Engine::instance(&eng);
   if (st != NM_OK) return st;
+  dev_tag_ = 0;
+  st = renderSetOn(eng, set, o, out, first, step);
+  if (st == NM_OK) {
+    dev_tag_ = eng->render_tag();
+    dev_frames_ = (int)(((set.hw ? set.h.size() : set.s.size()) - first + step - 1) / step);
+  }
+  return st;
+}
+
+int Mandelbrot::renderSetOn(Engine* eng, const FrameSet& set, const RenderOpts& o, EscapeValue* out, size_t first,
+                            size_t step) {
+  int st = NM_OK;
   if (set.hw) {
     std::vector<const HwFrameHost*> ptr;
     for (size_t k = first; k < set.h.size(); k += step) ptr.push_back(&set.h[k]);
@@ -1066,7 +1079,30 @@ int Mandelbrot::renderFrames(const std::vector<std::string>& widths, RenderOpts*
     cache_serial_ = next_serial();
   }
   o.set_serial = cache_serial_;
-  return renderSet(cache_, o, out, 0, 1);
+  const int st = renderSet(cache_, o, out, 0, 1);
+  dev_grid_ = false;
+  return st;
+}
+
+// viewer.cpp:89-122 on the device.  A frame of the last renderFrames exists only
+// on the device; the object's grid is coloured from the device copy of its last
+// render when that copy is still the engine's and the view has not changed since
+// (scaleUp / scaleDown change it), else from the host grid, as the viewer colours
+// whatever the grid holds.
+int Mandelbrot::colour(int frame, const uint8_t* pal_rgb, int npal, int smooth, int sc, uint8_t* out, double* ms) {
+  if (sc <= 0 || !pal_rgb || !out || npal < N) return NM_ERR_ARG;
+  Engine* eng = nullptr;
+  int st = Engine::instance(&eng);
+  if (st != NM_OK) return st;
+  const bool on_device = dev_tag_ != 0 && dev_tag_ == eng->render_tag();
+  if (frame >= 0) {
+    if (dev_grid_ || !on_device || frame >= dev_frames_) return NM_ERR_STATE;
+    return eng->colour(nullptr, dev_tag_, (size_t)frame * rows() * cols(), rows(), cols(), sc, N, pal_rgb, npal, smooth,
+                       out, ms);
+  }
+  if (dev_grid_ && on_device && snap_ == snapshot())
+    return eng->colour(nullptr, dev_tag_, 0, rows(), cols(), sc, N, pal_rgb, npal, smooth, out, ms);
+  return eng->colour(grid.values.data(), 0, 0, rows(), cols(), sc, N, pal_rgb, npal, smooth, out, ms);
 }
 
 // The project's launch autotuner.  Candidates: sample order (row-major, 32x32,

@@ -65,6 +65,9 @@ class Mandelbrot {
   enum ProbeMode { PROBE_HOST = 0, PROBE_GPU = 1 };
   int probe_mode = PROBE_GPU;
   int probe_candidates_ = -1;  // probes iterated in HP by the last GPU screening
+  uint64_t dev_tag_ = 0;       // engine render tag of this object's last render
+  int dev_frames_ = 0;         // frames in that render
+  bool dev_grid_ = false;      // that render was of this object's grid (not a frame set)
   struct TileStats {
     int ntiles = 0, accepted = 0, rejected = 0;
     uint64_t evals = 0;         // exact stability tests evaluated at the probes
@@ -106,6 +109,10 @@ class Mandelbrot {
   int precomputeAt(const char* re, const char* im);  // explicit reference point
   int referenceLength() const { return (int)ref_.len; }
   int probeWinner() const { return winner_; }
+  // the viewer's colouring (viewer.cpp:89-122) of this object's grid (frame < 0)
+  // or of frame `frame` of its last renderFrames; from the device copy of its
+  // last render while that is still the engine's, else from the host grid
+  int colour(int frame, const uint8_t* pal_rgb, int npal, int smooth, int sc, uint8_t* out, double* ms);
   int probeCandidates() const { return probe_candidates_; }
   bool hasReference() const { return have_ref_; }
   const HPComplex& referencePoint() const { return X[0]; }
@@ -127,6 +134,8 @@ class Mandelbrot {
   int buildFrames(const std::vector<std::string>& widths, FrameSet& set);
   // frames first, first + step, ... of a built set, one launch set
   int renderSet(const FrameSet& set, const RenderOpts& o, EscapeValue* out, size_t first, size_t step);
+  int renderSetOn(Engine* eng, const FrameSet& set, const RenderOpts& o, EscapeValue* out, size_t first,
+                  size_t step);
   static uint64_t next_serial();
   // per-tile features of the learned skip predictor for this view (kIn floats per
   // tile, tiles row-major): log10 width; the probe's offset from the reference

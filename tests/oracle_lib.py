@@ -48,6 +48,12 @@ _SIGS = [
     ("orc_pert_samples_fx", None, [_V, _V, _V, C.c_int, C.c_int, _V, C.c_int]),
     ("orc_tile_probe_ua", C.c_int, [_V, C.c_int, _V, _V]),
     ("orc_mlp_nparams", C.c_int, []),
+    ("orc_pal_parse", C.c_int, [C.c_char_p]),
+    ("orc_pal_cache", None, [C.c_int, _V]),
+    ("orc_hsl2rgb", None, [C.c_float, C.c_float, C.c_float, _V]),
+    ("orc_interp", None, [_V, _V, C.c_float, _V]),
+    ("orc_clip", C.c_uint8, [C.c_float]),
+    ("orc_colour", None, [_V, _V, C.c_int, C.c_int, C.c_int, _V, C.c_int, C.c_int, _V]),
     ("orc_mlp_forward", None, [_V, _V, C.c_int, _V]),
     ("orc_mlp_grad", C.c_double, [_V, _V, _V, C.c_int, _V]),
     ("orc_sa_start", C.c_int, [_V, C.c_int, C.c_int, C.POINTER(C.c_int), _V, C.POINTER(OEscape)]),
@@ -277,3 +283,38 @@ def center_render(cre, cim, width, res, N, nthreads=0):
     out = np.zeros(res * res, dtype=np.int32)
     lib().orc_center_render(_b(cre), _b(cim), _b(width), res, N, nthreads, out.ctypes.data)
     return out.reshape(res, res)
+
+
+# ---- colouring (oracle/colour.c)
+def pal_cache(text, N):
+    """MultiWaveGenerator::cache(N) of the .pal text: (N, 3) uint8."""
+    assert lib().orc_pal_parse(text.encode()) == 0
+    out = np.zeros((N, 3), dtype=np.uint8)
+    lib().orc_pal_cache(N, out.ctypes.data)
+    return out
+
+
+def hsl2rgb(h, s, l):
+    out = np.zeros(3, dtype=np.uint8)
+    lib().orc_hsl2rgb(h, s, l, out.ctypes.data)
+    return out
+
+
+def interp(a, b, t):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    out = np.zeros(3, dtype=np.uint8)
+    lib().orc_interp(a.ctypes.data, b.ctypes.data, t, out.ctypes.data)
+    return out
+
+
+def colour(grid, N, pal, smooth, sc):
+    """FractalViewer::getColor / colorLine of an ESCAPE_DTYPE grid."""
+    its = np.ascontiguousarray(grid["iterations"], dtype=np.int32)
+    sms = np.ascontiguousarray(grid["smoothing"], dtype=np.float32)
+    p = np.ascontiguousarray(pal, dtype=np.uint8)
+    nr, nc = its.shape
+    out = np.zeros((nr // sc, nc // sc, 3), dtype=np.uint8)
+    lib().orc_colour(its.ctypes.data, sms.ctypes.data, nr, nc, int(N), p.ctypes.data, 1 if smooth else 0, int(sc),
+                     out.ctypes.data)
+    return out
