@@ -259,6 +259,26 @@ def colour_leg(m, sc):
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0}}
 
 
+def video_leg(m, rate):
+    """k_video_frames: `rate` in-between frames of a (rows/3) x (cols/3) canvas from
+    two keyframes of (rows/2) x (cols/2) (the grid coloured 2 x 2), as the viewer's
+    zoom loop makes them; algorithmic bytes per output pixel = 3 written + 8 read
+    (one 4-byte pixel of each keyframe: both scaled keyframes cover the canvas, at
+    about one source pixel per output pixel), over the kernels' HIP-event time."""
+    import newman_amd as nm
+    pal = nm.Palette(filename=os.path.join(ROOT, "tests", "golden", "default.pal")).cache(m.N)
+    key, _ = m.colour(pal, True, 2)
+    nr, nc = m.rows() // 3, m.cols() // 3
+    nm.video_interpolate(key, key, nr, nc, rate)  # warm
+    runs = [nm.video_interpolate(key, key, nr, nc, rate)[1] for _ in range(3)]
+    ms = sum(runs) / len(runs)
+    nbytes = 11 * nr * nc * rate
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": "k_video_frames", "output": "%d frames of %dx%d RGB from %dx%d keyframes" % (
+        rate, nc, nr, key.shape[1], key.shape[0]), "ms": ms, "bytes": nbytes,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0}}
+
+
 def tune(step, ks, waves):
     """Sweep block size and persistent waves per SIMD; kernel time by HIP events."""
     best = None
@@ -467,8 +487,10 @@ def main():
         line["autotune"] = tuned
     if args.config == "c2":
         # the viewer's colouring of the rendered grid, on the device copy (no host
-        # round trip): 4 x 4 average into 1920x1080 RGB with smoothing
+        # round trip): 4 x 4 average into 1920x1080 RGB with smoothing; then the
+        # zoom video's in-between frames from that grid's 2 x 2 colouring
         line["colour"] = colour_leg(m, 4)
+        line["video"] = video_leg(m, 15)
     if args.variants and args.config in ("c2", "c5"):
         line["variants"] = variants(m, step, args, value, wall)
     if args.variants and args.config == "c3":

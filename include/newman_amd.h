@@ -253,6 +253,27 @@ int nm_colour(nm_mandelbrot* h, int frame, const uint8_t* pal, int npal, int smo
 int nm_colour_grid(const nm_escape_value* grid, int nr, int nc, int N, const uint8_t* pal, int npal, int smooth,
                    int sc, uint8_t* out, double* ms);
 
+/* ---- the zoom video: keyframes 1.5x apart and the frames between them ----
+ * VideoZoom::nextFrame (video.cpp:14-34) on the GPU: the `rate` frames (nr x nc
+ * x 3 bytes each, into out) between keyframes prev and next (kr x kc x 3 bytes,
+ * kr >= nr, kc >= nc): frame i blits prev scaled by lg and blends next scaled by
+ * sm with weight i / rate (lg from 1, sm from 2/3, both times 1.5^(1/rate) per
+ * frame).  byteimage's scaled/blit/blend are restated: bilinear with
+ * pixel-centre alignment rounded to the nearest byte, truncating blend. */
+int nm_video_interpolate(const uint8_t* prev, const uint8_t* next, int kr, int kc, int nr, int nc, int rate,
+                         uint8_t* out, double* ms);
+/* initAutoZoom (viewer.cpp:441-453) for a grid holding sc x sc samples per
+ * output pixel: full view (sz = 4 / cols, 3 / rows), scaleDown(sc), scaleUp(3) */
+int nm_zoom_video_start(nm_mandelbrot* h, int sc);
+/* one keyframe of the viewer's zoom loop (viewer.cpp:270-286): precompute and
+ * render the 3 x 3-sampled grid, colour it 2 x 2-averaged into key_out
+ * ((rows / 2) x (cols / 2) x 3), write the rate in-between frames from prev_key
+ * (NULL for the first keyframe: no frames) into frames ((rows / 3) x (cols / 3)
+ * x 3 each), *nframes = their count, then zoom(1.5).  ms (NULL ok): colouring +
+ * interpolation kernel time. */
+int nm_zoom_video_step(nm_mandelbrot* h, const uint8_t* pal, int npal, int smooth, int rate, const uint8_t* prev_key,
+                       uint8_t* key_out, uint8_t* frames, int* nframes, double* ms);
+
 /* the text of the last engine error */
 const char* nm_last_error(void);
 

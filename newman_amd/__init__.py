@@ -114,6 +114,19 @@ def colour_grid(grid, N, pal, smooth=True, sc=1):
     return out, ms.value
 
 
+def video_interpolate(prev, nxt, nr, nc, rate):
+    """VideoZoom::nextFrame's in-between frames on the GPU: (rate, nr, nc, 3)
+    uint8 between keyframes prev and nxt ((kr, kc, 3) uint8), and the kernel ms."""
+    p = np.ascontiguousarray(prev, dtype=np.uint8)
+    q = np.ascontiguousarray(nxt, dtype=np.uint8)
+    assert p.shape == q.shape and p.ndim == 3 and p.shape[2] == 3
+    out = np.zeros((int(rate), int(nr), int(nc), 3), dtype=np.uint8)
+    ms = C.c_double(0.0)
+    _check(lib().nm_video_interpolate(p.ctypes.data, q.ctypes.data, p.shape[0], p.shape[1], int(nr), int(nc),
+                                      int(rate), out.ctypes.data, C.byref(ms)), "nm_video_interpolate")
+    return out, ms.value
+
+
 class SkipNet:
     """The learned series-skip predictor (132-64-64-1 ReLU MLP, trained on the GPU)."""
 
@@ -241,6 +254,11 @@ _SIGS = [
                             C.POINTER(C.c_double)]),
     ("nm_colour_grid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                  C.c_void_p, C.POINTER(C.c_double)]),
+    ("nm_video_interpolate", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_void_p, C.POINTER(C.c_double)]),
+    ("nm_zoom_video_start", C.c_int, [_H, C.c_int]),
+    ("nm_zoom_video_step", C.c_int, [_H, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("nm_last_error", C.c_char_p, []),
     ("nm_hp_op", C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_long, C.c_double, C.c_long,
                            C.POINTER(C.c_int), C.POINTER(C.c_long), C.c_void_p]),
@@ -400,6 +418,27 @@ class Mandelbrot:
         _check(lib().nm_colour(self._h, int(frame), p.ctypes.data, p.shape[0], 1 if smooth else 0, int(sc),
                                out.ctypes.data, C.byref(ms)), "nm_colour")
         return out, ms.value
+
+    def zoom_video_start(self, sc=1):
+        """initAutoZoom (viewer.cpp:441-453)."""
+        _check(lib().nm_zoom_video_start(self._h, int(sc)), "nm_zoom_video_start")
+
+    def zoom_video_step(self, pal, smooth=True, rate=45, prev_key=None):
+        """One keyframe of the viewer's zoom loop; returns (keyframe, frames or
+        None, kernel ms)."""
+        p = np.ascontiguousarray(pal, dtype=np.uint8)
+        key = np.zeros((self.rows() // 2, self.cols() // 2, 3), dtype=np.uint8)
+        frames = None
+        if prev_key is not None:
+            prev_key = np.ascontiguousarray(prev_key, dtype=np.uint8)
+            frames = np.zeros((int(rate), self.rows() // 3, self.cols() // 3, 3), dtype=np.uint8)
+        n = C.c_int(0)
+        ms = C.c_double(0.0)
+        _check(lib().nm_zoom_video_step(self._h, p.ctypes.data, p.shape[0], 1 if smooth else 0, int(rate),
+                                        None if prev_key is None else prev_key.ctypes.data, key.ctypes.data,
+                                        None if frames is None else frames.ctypes.data, C.byref(n), C.byref(ms)),
+               "nm_zoom_video_step")
+        return key, frames, ms.value
 
     def grid(self):
         out = np.zeros(self.rows() * self.cols(), dtype=ESCAPE_DTYPE)

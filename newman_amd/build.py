@@ -17,8 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnewman_amd.so")
 ARCH = os.environ.get("NEWMAN_AMD_ARCH", "gfx950")
 
-HOST_SRCS = ["hpfloat.cpp", "mandelbrot.cpp", "palette.cpp", "capi.cpp"]
-HIP_SRCS = ["kernels.hip", "skipnet.hip", "colour.hip"]
+HOST_SRCS = ["hpfloat.cpp", "mandelbrot.cpp", "palette.cpp", "video.cpp", "capi.cpp"]
+HIP_SRCS = ["kernels.hip", "skipnet.hip", "colour.hip", "video.hip"]
 
 CXXFLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-function",
             "-pthread"]

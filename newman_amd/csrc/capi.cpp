@@ -627,4 +627,27 @@ int nm_colour_grid(const nm_escape_value* grid, int nr, int nc, int N, const uin
   });
 }
 
+// ---- zoom video
+int nm_video_interpolate(const uint8_t* prev, const uint8_t* next, int kr, int kc, int nr, int nc, int rate,
+                         uint8_t* out, double* ms) {
+  if (!prev || !next || !out || kr < nr || kc < nc || nr <= 0 || nc <= 0 || rate <= 0) return NM_ERR_ARG;
+  return guard([&] {
+    nm::Engine* eng = nullptr;
+    int st = nm::Engine::instance(&eng);
+    if (st != NM_OK) return st;
+    return eng->video(prev, next, kr, kc, nr, nc, rate, out, ms);
+  });
+}
+
+int nm_zoom_video_start(nm_mandelbrot* h, int sc) {
+  if (!h) return NM_ERR_ARG;
+  return guard([&] { return h->m.zoomVideoStart(sc); });
+}
+
+int nm_zoom_video_step(nm_mandelbrot* h, const uint8_t* pal, int npal, int smooth, int rate, const uint8_t* prev_key,
+                       uint8_t* key_out, uint8_t* frames, int* nframes, double* ms) {
+  if (!h) return NM_ERR_ARG;
+  return guard([&] { return h->m.zoomVideoStep(pal, npal, smooth, rate, prev_key, key_out, frames, nframes, ms); });
+}
+
 }  // extern "C"

@@ -47,8 +47,13 @@ __device__ __forceinline__ void get_color(const EscapeValue e, const int N, cons
   b = clip_u8(u * (float)c0.z + t * (float)c1.z);
 }
 
-__global__ __launch_bounds__(256) void k_colour(const EscapeValue* grid, int nc, int sc, int nro, int nco, int N,
+// SC > 0: the sample count per side is a compile-time constant, so all SC^2
+// escape values are loaded before the palette lookups that depend on them;
+// SC = 0: any sc at run time
+template <int SC>
+__global__ __launch_bounds__(256) void k_colour(const EscapeValue* grid, int nc, int sc_rt, int nro, int nco, int N,
                                                 const uchar4* pal, int smooth, uint8_t* out) {
+  const int sc = SC > 0 ? SC : sc_rt;
   const uint64_t n = (uint64_t)nro * nco;
   const float area = (float)(sc * sc);
   for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (uint64_t)gridDim.x * blockDim.x) {
@@ -56,9 +61,27 @@ __global__ __launch_bounds__(256) void k_colour(const EscapeValue* grid, int nc,
     uint32_t cr, cg, cb;
     if (sc == 1) {
       get_color(grid[(uint64_t)r * nc + c], N, pal, smooth, cr, cg, cb);
-    } else {
+    } else if (SC > 0) {
       // colorLine (viewer.cpp:110-121): float sums in row-then-column order,
       // divided by sc * sc, then clipped
+      EscapeValue e[SC > 0 ? SC * SC : 1];
+#pragma unroll
+      for (int i = 0; i < SC; i++)
+#pragma unroll
+        for (int j = 0; j < SC; j++) e[i * SC + j] = grid[(uint64_t)(r * SC + i) * nc + (uint64_t)c * SC + j];
+      float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+#pragma unroll
+      for (int k = 0; k < SC * SC; k++) {
+        uint32_t xr, xg, xb;
+        get_color(e[k], N, pal, smooth, xr, xg, xb);
+        sr += (float)xr;
+        sg += (float)xg;
+        sb += (float)xb;
+      }
+      cr = clip_u8(sr / area);
+      cg = clip_u8(sg / area);
+      cb = clip_u8(sb / area);
+    } else {
       float sr = 0.0f, sg = 0.0f, sb = 0.0f;
       for (int r1 = r * sc; r1 < (r + 1) * sc; r1++)
         for (int c1 = c * sc; c1 < (c + 1) * sc; c1++) {
@@ -88,8 +111,19 @@ int colour_launch(hipStream_t stream, int num_cu, const EscapeValue* grid, int n
   if (n == 0) return 0;
   const uint64_t want = (n + 255) / 256;
   const int grid_dim = (int)(want < (uint64_t)num_cu * 16 ? want : (uint64_t)num_cu * 16);
-  hipLaunchKernelGGL(k_colour, dim3(grid_dim), dim3(256), 0, stream, grid, nc, sc, nro, nco, N,
-                     (const uchar4*)pal_rgba, smooth, out);
+  const uchar4* pal = (const uchar4*)pal_rgba;
+  switch (sc) {
+    case 1: hipLaunchKernelGGL(k_colour<1>, dim3(grid_dim), dim3(256), 0, stream, grid, nc, sc, nro, nco, N, pal,
+                               smooth, out); break;
+    case 2: hipLaunchKernelGGL(k_colour<2>, dim3(grid_dim), dim3(256), 0, stream, grid, nc, sc, nro, nco, N, pal,
+                               smooth, out); break;
+    case 3: hipLaunchKernelGGL(k_colour<3>, dim3(grid_dim), dim3(256), 0, stream, grid, nc, sc, nro, nco, N, pal,
+                               smooth, out); break;
+    case 4: hipLaunchKernelGGL(k_colour<4>, dim3(grid_dim), dim3(256), 0, stream, grid, nc, sc, nro, nco, N, pal,
+                               smooth, out); break;
+    default: hipLaunchKernelGGL(k_colour<0>, dim3(grid_dim), dim3(256), 0, stream, grid, nc, sc, nro, nco, N, pal,
+                                smooth, out);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -137,6 +137,10 @@ class Engine {
   // entries of 3 bytes (npal >= N).  *ms: the colouring kernel's time.
   virtual int colour(const EscapeValue* host_grid, uint64_t tag, size_t offset, int nr, int nc, int sc, int N,
                      const uint8_t* pal_rgb, int npal, int smooth, uint8_t* host_out, double* ms) = 0;
+  // The zoom video's `rate` in-between frames (video.cpp:14-34) from keyframes
+  // prev and next (kr x kc x 3 bytes, host) into out (rate x nr x nc x 3, host).
+  virtual int video(const uint8_t* prev, const uint8_t* next, int kr, int kc, int nr, int nc, int rate,
+                    uint8_t* out, double* ms) = 0;
   virtual const char* last_error() const = 0;
 };
 

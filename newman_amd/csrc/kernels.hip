@@ -50,6 +50,7 @@
 
 #include "colour.hpp"
 #include "engine.hpp"
+#include "video.hpp"
 
 namespace nm {
 
@@ -1381,6 +1382,8 @@ class EngineImpl : public Engine {
   void* d_grid = nullptr; size_t n_grid_bytes = 0;  // downsample staging
   void* d_pal = nullptr; size_t n_pal_bytes = 0;    // colouring: palette (RGBA) and image
   void* d_img = nullptr; size_t n_img_bytes = 0;
+  void* d_keys = nullptr; size_t n_keys_bytes = 0;   // video: two keyframes and the schedule
+  void* d_vframes = nullptr; size_t n_vframes_bytes = 0;
   uint64_t tag_ = 0;                                 // render count (render_tag)
   const EscapeValue* last_result = nullptr;          // the last render's row-major grids
   uint64_t last_nout = 0;
@@ -2022,6 +2025,36 @@ class EngineImpl : public Engine {
       return fail("k_colour launch", hipGetLastError());
     NM_HIP(hipEventRecord(evpool[1], stream));
     if (nout) NM_HIP(hipMemcpyAsync(host_out, d_img, nout * 3, hipMemcpyDeviceToHost, stream));
+    NM_HIP(hipStreamSynchronize(stream));
+    float t = 0.0f;
+    NM_HIP(hipEventElapsedTime(&t, evpool[0], evpool[1]));
+    if (ms) *ms = t;
+    return NM_OK;
+  }
+
+  int video(const uint8_t* prev, const uint8_t* next, int kr, int kc, int nr, int nc, int rate, uint8_t* out,
+            double* ms) override {
+    // the previous keyframe scaled by lg >= 1 must cover the canvas (blit)
+    if (!prev || !next || !out || kr < nr || kc < nc || nr <= 0 || nc <= 0 || rate <= 0) return NM_ERR_ARG;
+    const std::vector<VideoFrame> sched = video_schedule(kr, kc, nr, nc, rate);
+    const size_t kbytes = (size_t)kr * kc * 3, sbytes = sched.size() * sizeof(VideoFrame);
+    const size_t obytes = (size_t)nr * nc * 3 * (size_t)rate;
+    int st;
+    if ((st = grow(&d_keys, &n_keys_bytes, 2 * kbytes + sbytes + 512 + 8 * (size_t)kr * kc))) return st;
+    if ((st = grow(&d_vframes, &n_vframes_bytes, obytes))) return st;
+    uint8_t* kp = (uint8_t*)d_keys;
+    uint8_t* kn = kp + kbytes;
+    VideoFrame* ks = (VideoFrame*)(((uintptr_t)(kn + kbytes) + 255) & ~(uintptr_t)255);
+    NM_HIP(hipMemcpyAsync(kp, prev, kbytes, hipMemcpyHostToDevice, stream));
+    NM_HIP(hipMemcpyAsync(kn, next, kbytes, hipMemcpyHostToDevice, stream));
+    NM_HIP(hipMemcpyAsync(ks, sched.data(), sbytes, hipMemcpyHostToDevice, stream));
+    if ((st = ensure_events(2))) return st;
+    NM_HIP(hipEventRecord(evpool[0], stream));
+    uint8_t* wide = (uint8_t*)(((uintptr_t)(ks + sched.size()) + 255) & ~(uintptr_t)255);
+    if (video_launch((void*)stream, kp, kn, kr, kc, nr, nc, ks, rate, wide, (uint8_t*)d_vframes))
+      return fail("k_video_frames launch", hipGetLastError());
+    NM_HIP(hipEventRecord(evpool[1], stream));
+    NM_HIP(hipMemcpyAsync(out, d_vframes, obytes, hipMemcpyDeviceToHost, stream));
     NM_HIP(hipStreamSynchronize(stream));
     float t = 0.0f;
     NM_HIP(hipEventElapsedTime(&t, evpool[0], evpool[1]));

@@ -1105,6 +1105,47 @@ int Mandelbrot::colour(int frame, const uint8_t* pal_rgb, int npal, int smooth, 
   return eng->colour(grid.values.data(), 0, 0, rows(), cols(), sc, N, pal_rgb, npal, smooth, out, ms);
 }
 
+// initAutoZoom (viewer.cpp:441-453): the full view of the set (sz = 4 / cols,
+// 3 / rows of the current grid), back to one sample per pixel, then 3 x 3 samples
+int Mandelbrot::zoomVideoStart(int sc) {
+  if (sc <= 0 || rows() % sc || cols() % sc) return NM_ERR_ARG;
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  sz.re = hpf(L);
+  sz.im = hpf(L);
+  hp_set_d(sz.re, 4.0 / cols());
+  hp_set_d(sz.im, 3.0 / rows());
+  int st = scaleDown(sc);
+  if (st != NM_OK) return st;
+  return scaleUp(3);
+}
+
+// update()'s zoom branch (viewer.cpp:270-286): render the 3 x 3-sampled grid,
+// colour it 2 x 2-averaged into a keyframe 1.5 times the output size, make the
+// in-between frames from the previous keyframe (none for the first), then zoom
+// in 1.5 times about the unchanged centre.
+int Mandelbrot::zoomVideoStep(const uint8_t* pal_rgb, int npal, int smooth, int rate, const uint8_t* prev_key,
+                              uint8_t* key_out, uint8_t* frames_out, int* nframes, double* ms) {
+  if (!pal_rgb || !key_out || rate <= 0 || rows() % 3 || cols() % 3) return NM_ERR_ARG;
+  if (prev_key && !frames_out) return NM_ERR_ARG;
+  int st = precompute();
+  if (st != NM_OK) return st;
+  if ((st = render(nullptr, true)) != NM_OK) return st;
+  double t0 = 0.0, t1 = 0.0;
+  if ((st = colour(-1, pal_rgb, npal, smooth, 2, key_out, &t0)) != NM_OK) return st;
+  int n = 0;
+  if (prev_key) {
+    Engine* eng = nullptr;
+    if ((st = Engine::instance(&eng)) != NM_OK) return st;
+    st = eng->video(prev_key, key_out, rows() / 2, cols() / 2, rows() / 3, cols() / 3, rate, frames_out, &t1);
+    if (st != NM_OK) return st;
+    n = rate;
+  }
+  if (nframes) *nframes = n;
+  if (ms) *ms = t0 + t1;
+  zoom(1.5f);
+  return NM_OK;
+}
+
 // The project's launch autotuner.  Candidates: sample order (row-major, 32x32,
 // 64x64 and 128x128 tiles), persistent waves per SIMD (2, 4, 6, 8) and the block
 // size (8, 16, 32 for the series kernel; 16, 32 for the plain one).  Each candidate

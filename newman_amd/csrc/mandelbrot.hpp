@@ -113,6 +113,12 @@ class Mandelbrot {
   // or of frame `frame` of its last renderFrames; from the device copy of its
   // last render while that is still the engine's, else from the host grid
   int colour(int frame, const uint8_t* pal_rgb, int npal, int smooth, int sc, uint8_t* out, double* ms);
+  // the viewer's zoom video: initAutoZoom (viewer.cpp:441-453) and one keyframe
+  // of update()'s zoom branch (viewer.cpp:270-286) with VideoZoom::nextFrame's
+  // in-between frames (video.cpp:14-34)
+  int zoomVideoStart(int sc);
+  int zoomVideoStep(const uint8_t* pal_rgb, int npal, int smooth, int rate, const uint8_t* prev_key,
+                    uint8_t* key_out, uint8_t* frames_out, int* nframes, double* ms);
   int probeCandidates() const { return probe_candidates_; }
   bool hasReference() const { return have_ref_; }
   const HPComplex& referencePoint() const { return X[0]; }

@@ -207,3 +207,73 @@ void orc_colour(const int32_t* its, const float* sms, int nr, int nc, int N, con
       for (int k = 0; k < 3; k++) o[k] = orc_clip(sum[k] / (float)(sc * sc));
     }
 }
+
+/* ---- VideoZoom::nextFrame (video.cpp:14-34) with its images materialised as
+ * the reference does: small = next.scaled(), large = prev.scaled(), then
+ * canvas.blit(large, ...) and canvas.blend(small, t, ...).  scaled/blit/blend
+ * belong to byteimage (absent): bilinear with pixel-centre alignment and clamped
+ * edges rounded to the nearest byte; blit copies with clipping; blend clamps
+ * and truncates (1 - t) canvas + t image. */
+static void scale_image(const uint8_t* src, int sr, int sc, uint8_t* dst, int dr, int dc) {
+  for (int i = 0; i < dr; i++)
+    for (int j = 0; j < dc; j++) {
+      float y = ((float)i + 0.5f) * ((float)sr / (float)dr) - 0.5f;
+      float x = ((float)j + 0.5f) * ((float)sc / (float)dc) - 0.5f;
+      if (y < 0.0f) y = 0.0f;
+      if (y > (float)(sr - 1)) y = (float)(sr - 1);
+      if (x < 0.0f) x = 0.0f;
+      if (x > (float)(sc - 1)) x = (float)(sc - 1);
+      int y0 = (int)y, x0 = (int)x;
+      int y1 = y0 + 1 < sr ? y0 + 1 : sr - 1;
+      int x1 = x0 + 1 < sc ? x0 + 1 : sc - 1;
+      float fy = y - (float)y0, fx = x - (float)x0;
+      for (int k = 0; k < 3; k++) {
+        float a = src[3 * ((size_t)y0 * sc + x0) + k], b = src[3 * ((size_t)y0 * sc + x1) + k];
+        float c = src[3 * ((size_t)y1 * sc + x0) + k], d = src[3 * ((size_t)y1 * sc + x1) + k];
+        float top = (1.0f - fx) * a + fx * b;
+        float bot = (1.0f - fx) * c + fx * d;
+        dst[3 * ((size_t)i * dc + j) + k] = orc_clip((1.0f - fy) * top + fy * bot + 0.5f);
+      }
+    }
+}
+
+/* frames: rate x nr x nc x 3 */
+int orc_video_frames(const uint8_t* prev, const uint8_t* next, int kr, int kc, int nr, int nc, int rate,
+                     uint8_t* frames) {
+  uint8_t* canvas = (uint8_t*)calloc((size_t)nr * nc * 3, 1);
+  uint8_t* small = (uint8_t*)malloc((size_t)kr * kc * 3 + 3);
+  uint8_t* large = (uint8_t*)malloc((size_t)kr * kc * 12 + 3);
+  if (!canvas || !small || !large) return 1;
+  float v = (float)pow(1.5, 1.0 / rate), sm = (float)(2.0 / 3.0), lg = 1.0f;
+  for (int i = 0; i < rate; i++) {
+    float t = (float)i / (float)rate;
+    int srr = (int)((float)kr * sm), scc = (int)((float)kc * sm);
+    int lrr = (int)((float)kr * lg), lcc = (int)((float)kc * lg);
+    if ((size_t)lrr * lcc > (size_t)kr * kc * 4) return 2; /* lg stays below 1.5 */
+    scale_image(next, kr, kc, small, srr, scc);
+    scale_image(prev, kr, kc, large, lrr, lcc);
+    int lr0 = (nr - lrr) / 2, lc0 = (nc - lcc) / 2;
+    for (int r = 0; r < lrr; r++)
+      for (int c = 0; c < lcc; c++) {
+        int rr = r + lr0, cc = c + lc0;
+        if (rr < 0 || rr >= nr || cc < 0 || cc >= nc) continue;
+        memcpy(canvas + 3 * ((size_t)rr * nc + cc), large + 3 * ((size_t)r * lcc + c), 3);
+      }
+    int sr0 = (nr - srr) / 2, sc0 = (nc - scc) / 2;
+    for (int r = 0; r < srr; r++)
+      for (int c = 0; c < scc; c++) {
+        int rr = r + sr0, cc = c + sc0;
+        if (rr < 0 || rr >= nr || cc < 0 || cc >= nc) continue;
+        uint8_t* o = canvas + 3 * ((size_t)rr * nc + cc);
+        const uint8_t* q = small + 3 * ((size_t)r * scc + c);
+        for (int k = 0; k < 3; k++) o[k] = orc_clip((1.0f - t) * (float)o[k] + t * (float)q[k]);
+      }
+    sm *= v;
+    lg *= v;
+    memcpy(frames + (size_t)i * nr * nc * 3, canvas, (size_t)nr * nc * 3);
+  }
+  free(canvas);
+  free(small);
+  free(large);
+  return 0;
+}

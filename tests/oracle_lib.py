@@ -54,6 +54,7 @@ _SIGS = [
     ("orc_interp", None, [_V, _V, C.c_float, _V]),
     ("orc_clip", C.c_uint8, [C.c_float]),
     ("orc_colour", None, [_V, _V, C.c_int, C.c_int, C.c_int, _V, C.c_int, C.c_int, _V]),
+    ("orc_video_frames", C.c_int, [_V, _V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _V]),
     ("orc_mlp_forward", None, [_V, _V, C.c_int, _V]),
     ("orc_mlp_grad", C.c_double, [_V, _V, _V, C.c_int, _V]),
     ("orc_sa_start", C.c_int, [_V, C.c_int, C.c_int, C.POINTER(C.c_int), _V, C.POINTER(OEscape)]),
@@ -317,4 +318,14 @@ def colour(grid, N, pal, smooth, sc):
     out = np.zeros((nr // sc, nc // sc, 3), dtype=np.uint8)
     lib().orc_colour(its.ctypes.data, sms.ctypes.data, nr, nc, int(N), p.ctypes.data, 1 if smooth else 0, int(sc),
                      out.ctypes.data)
+    return out
+
+
+def video_frames(prev, nxt, nr, nc, rate):
+    """VideoZoom::nextFrame's frames: (rate, nr, nc, 3) uint8."""
+    p = np.ascontiguousarray(prev, dtype=np.uint8)
+    q = np.ascontiguousarray(nxt, dtype=np.uint8)
+    out = np.zeros((rate, nr, nc, 3), dtype=np.uint8)
+    assert lib().orc_video_frames(p.ctypes.data, q.ctypes.data, p.shape[0], p.shape[1], nr, nc, rate,
+                                  out.ctypes.data) == 0
     return out

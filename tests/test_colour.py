@@ -107,3 +107,39 @@ def test_palette_rejects_bad_input(tmp_path):
         nm.Palette(filename=tmp_path / "missing.pal")
     with pytest.raises(nm.NewmanError):
         nm.Palette().cache(10)  # nothing loaded
+
+
+def _keyframes(rng, kr, kc):
+    return (rng.integers(0, 256, size=(kr, kc, 3), dtype=np.uint8),
+            rng.integers(0, 256, size=(kr, kc, 3), dtype=np.uint8))
+
+
+def test_video_first_frame_is_the_centre_crop():
+    """Frame 0 of VideoZoom::nextFrame: lg = 1 (the previous keyframe at its own
+    size, which bilinear scaling leaves unchanged) blitted centred, and the next
+    keyframe blended with t = 0: the centre crop of the previous keyframe."""
+    rng = np.random.default_rng(4)
+    prev, nxt = _keyframes(rng, 27, 36)
+    f = O.video_frames(prev, nxt, 18, 24, 5)
+    r0, c0 = (27 - 18) // 2, (36 - 24) // 2
+    assert np.array_equal(f[0], prev[r0:r0 + 18, c0:c0 + 24])
+
+
+def test_video_constant_keyframes():
+    """Constant keyframes A and B: outside the scaled next keyframe every frame is
+    A; inside it, (1 - t) A + t B truncated, with t = i / rate."""
+    prev = np.full((30, 45, 3), 200, np.uint8)
+    nxt = np.full((30, 45, 3), 10, np.uint8)
+    rate = 6
+    f = O.video_frames(prev, nxt, 20, 30, rate)
+    v, sm = np.float32(1.5 ** (1.0 / rate)), np.float32(2.0 / 3.0)
+    for i in range(rate):
+        t = np.float32(i) / np.float32(rate)
+        srr, scc = int(np.float32(30) * sm), int(np.float32(45) * sm)
+        r0, c0 = (20 - srr) // 2, (30 - scc) // 2
+        inner = int((np.float32(1.0) - t) * np.float32(200) + t * np.float32(10))
+        assert (f[i][max(r0, 0):r0 + srr, max(c0, 0):c0 + scc] == inner).all()
+        mask = np.ones((20, 30), bool)
+        mask[max(r0, 0):r0 + srr, max(c0, 0):c0 + scc] = False
+        assert (f[i][mask] == 200).all()
+        sm = np.float32(sm * v)

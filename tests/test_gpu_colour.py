@@ -31,7 +31,7 @@ def _grid(rng, nr, nc, N):
     return g
 
 
-@pytest.mark.parametrize("sc", [1, 2, 3, 4])
+@pytest.mark.parametrize("sc", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("smooth", [False, True])
 def test_colour_grid_equals_oracle(gpu, sc, smooth):
     nm = gpu
@@ -92,3 +92,33 @@ def test_colour_of_batch_frames(gpu):
         m.colour(pal, True, 1, frame=len(widths))
     with pytest.raises(nm.NewmanError):
         m.colour(pal[: v["N"] - 1], True, 1)  # the palette must cover N entries
+
+
+@pytest.mark.parametrize("kr,kc,nr,nc,rate", [(27, 36, 18, 24, 5), (45, 60, 30, 40, 45), (31, 50, 20, 33, 7)])
+def test_video_interpolate_equals_oracle(gpu, kr, kc, nr, nc, rate):
+    nm = gpu
+    rng = np.random.default_rng(kr + rate)
+    prev = rng.integers(0, 256, size=(kr, kc, 3), dtype=np.uint8)
+    nxt = rng.integers(0, 256, size=(kr, kc, 3), dtype=np.uint8)
+    frames, ms = nm.video_interpolate(prev, nxt, nr, nc, rate)
+    want = O.video_frames(prev, nxt, nr, nc, rate)
+    assert frames.shape == (rate, nr, nc, 3)
+    assert np.array_equal(frames, want), np.argwhere(frames != want)[:5]
+
+
+def test_zoom_video_keyframes(gpu):
+    """The viewer's zoom loop from the full view: initAutoZoom, then keyframes 1.5x
+    apart; each keyframe is the 2 x 2 colour average of the 3 x 3-sampled grid,
+    and the frames between two keyframes are VideoZoom::nextFrame's."""
+    nm = gpu
+    m = nm.Mandelbrot(12, 16)
+    m.N = 256
+    m.zoom_video_start(1)
+    assert (m.rows(), m.cols()) == (36, 48)
+    pal = nm.Palette(text=DEFAULT_PAL).cache(256)
+    key1, frames, _ = m.zoom_video_step(pal, True, 9)
+    assert frames is None and key1.shape == (18, 24, 3)
+    assert np.array_equal(key1, O.colour(m.grid(), 256, pal, True, 2))
+    key2, frames, _ = m.zoom_video_step(pal, True, 9, prev_key=key1)
+    assert np.array_equal(key2, O.colour(m.grid(), 256, pal, True, 2))
+    assert np.array_equal(frames, O.video_frames(key1, key2, 12, 16, 9))
