@@ -1051,19 +1051,18 @@ __device__ __forceinline__ void fx_normalise(double yr, double yi, double er, do
   esi = (float)ldexp(ei, -ex);
 }
 
-template <int K, bool END>
+template <int K, bool END, class P>
 __device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, float& esr, float& esi, const double er,
-                                              const double ei, int& m, const int it, double2* zb,
-                                              const double2* zhead, const int endj, const int mend, const double b2,
-                                              const float b2g, int& esc, double& esc_r2, unsigned int& rb) {
+                                              const double ei, int& m, const int it, P zp, const P head,
+                                              const int endj, const int mend, const double b2, const float b2g,
+                                              int& esc, double& esc_r2, unsigned int& rb) {
   const float qnanf = __int_as_float(0x7fc00000);
   bool moved = false;
-  unsigned int hoff = 0;
+  ld_double2 z = zp[0];
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    if (moved) zb[j] = *(const double2*)((const char*)zhead + hoff);
-    const double zrd = zb[j].x;
-    const double zid = zb[j].y;
+    const double zrd = z.x;
+    const double zid = z.y;
     const float zr = (float)zrd, zi = (float)zid;
     const float dr = ldexpf(wr, ex), di = ldexpf(wi, ex);
     const float yr = zr + dr, yi = zi + di;
@@ -1079,30 +1078,30 @@ __device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, flo
     }
     const float dmag = fmaf(dr, dr, di * di);
     bool reb = r2 < dmag;
-    if (END) reb = reb || (moved ? (int)hoff == 16 * mend : j == endj);
+    if (END) reb = reb || (moved ? orbit_index(zp, head) + j == mend : j == endj);
     float Wr = zr + yr, Wi = zi + yi;
     if (reb) {  // rebase in FP64: d = Y against Zd[0] = 0, so W = Y
       const double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
       fx_normalise(ydr, ydi, er, ei, wr, wi, ex, esr, esi);
       Wr = (float)ydr;
       Wi = (float)ydi;
-      hoff = 0;
+      zp = orbit_back(head, j);
       moved = true;
     }
+    if (j + 1 < K) z = zp[j + 1];
     const float nwr = fmaf(wr, Wr, fmaf(-wi, Wi, esr));
     const float nwi = fmaf(wr, Wi, fmaf(wi, Wr, esi));
     wr = nwr;
     wi = nwi;
-    hoff += 16;
   }
-  m = moved ? (int)(hoff >> 4) : m + K;
+  m = moved ? orbit_index(zp, head) + K : m + K;
   rb += moved ? 1u : 0u;
 }
 
 // FP64 (FX = false) or FP32X (FX = true) perturbation with rebasing (replaces the
 // HP continuation of mandelbrot.cpp:209-224).  Persistent waves, lanes refilled
-// from 64-sample chunks; blocks of K steps with the block's orbit entries loaded
-// up front.
+// from 64-sample chunks; blocks of K steps, each lane reading the orbit through
+// its own cursor into the workgroup's LDS windows or HBM.
 template <int K, bool FX>
 __global__ __launch_bounds__(kSeriesIterThreads) void k_series_iterate(SeriesArgs a) {
   const Common& c = a.c;
@@ -1208,22 +1207,25 @@ __global__ __launch_bounds__(kSeriesIterThreads) void k_series_iterate(SeriesArg
     const int it0 = it;
     const bool can_refill = pos < ncur || nnext > 0;  // wave-uniform
     if (FX) {
-      double2 zb[K];
-      if (in_head || in_tail) {
-        const double2* src = in_head ? zhead + m : zlds + (m - tail_lo);
-#pragma unroll
-        for (int j = 0; j < K; j++) zb[j] = src[j];
-      } else {
-#pragma unroll
-        for (int j = 0; j < K; j++) zb[j] = a.Zd[m + j];
-      }
       const float b2g = (float)(c.b2 * (1.0 - 1.0 / 1024.0));
-      if (end_zone)
-        pert_block_fx<K, true>(wr, wi, ex, esr, esi, er, ei, m, it, zb, zhead, mend - m, mend, c.b2, b2g, esc,
-                               esc_r2, rb);
-      else
-        pert_block_fx<K, false>(wr, wi, ex, esr, esi, er, ei, m, it, zb, zhead, 0, mend, c.b2, b2g, esc, esc_r2,
-                                rb);
+      if (__ballot(act && !in_head && !in_tail) == 0) {
+        const lds_orbit lhead = (lds_orbit)(const void*)zhead;
+        const lds_orbit zp = in_head ? lhead + m : (lds_orbit)(const void*)zlds + (m - tail_lo);
+        if (end_zone)
+          pert_block_fx<K, true>(wr, wi, ex, esr, esi, er, ei, m, it, zp, lhead, mend - m, mend, c.b2, b2g, esc,
+                                 esc_r2, rb);
+        else
+          pert_block_fx<K, false>(wr, wi, ex, esr, esi, er, ei, m, it, zp, lhead, 0, mend, c.b2, b2g, esc, esc_r2,
+                                  rb);
+      } else {
+        const hbm_orbit ghead = (hbm_orbit)(const void*)a.Zd;
+        if (end_zone)
+          pert_block_fx<K, true>(wr, wi, ex, esr, esi, er, ei, m, it, ghead + m, ghead, mend - m, mend, c.b2, b2g,
+                                 esc, esc_r2, rb);
+        else
+          pert_block_fx<K, false>(wr, wi, ex, esr, esi, er, ei, m, it, ghead + m, ghead, 0, mend, c.b2, b2g, esc,
+                                  esc_r2, rb);
+      }
       // keep the mantissa near 1: |w| grows by at most |W| per step
       const float mx = fmaxf(fabsf(wr), fabsf(wi));
       if (mx > 0.0f && mx < INFINITY) {
