@@ -157,7 +157,7 @@ typedef struct nm_launch {
   int32_t waves_per_simd;
   int32_t tile;
   int32_t graph;
-  int32_t pre_iter; /* plain path: updates tried in the start kernel (0 = default 32) */
+  int32_t pre_iter; /* plain path: updates tried in the start kernel (0 = default 16) */
   int32_t reserved;
   double ms;
 } nm_launch;

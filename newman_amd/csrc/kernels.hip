@@ -331,7 +331,7 @@ __device__ __forceinline__ void tile_count_store(uint32_t* tile_count, uint32_t 
 // test, then up to kPreIter updates per sample with the exact escape test (most
 // samples of a shallow view escape here, converged); the rest are flagged for
 // the persistent kernel with their state (z, c, updates done).
-constexpr int kPreIterDefault = 32;
+constexpr int kPreIterDefault = 16;
 
 __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   const Common& c = a.c;
