@@ -273,6 +273,11 @@ int nm_zoom_video_start(nm_mandelbrot* h, int sc);
  * interpolation kernel time. */
 int nm_zoom_video_step(nm_mandelbrot* h, const uint8_t* pal, int npal, int smooth, int rate, const uint8_t* prev_key,
                        uint8_t* key_out, uint8_t* frames, int* nframes, double* ms);
+/* beautyRender (viewer.cpp:186-253): h's view (N, centre, sz scaled by rows /
+ * 3240; error tolerance and bailout at their defaults, as there) rendered on a
+ * 3240 x 5760 grid and coloured 3 x 3-averaged into out (1080 x 1920 x 3 bytes);
+ * h is unchanged.  ms (NULL ok): colouring time. */
+int nm_beauty_render(const nm_mandelbrot* h, const uint8_t* pal, int npal, int smooth, uint8_t* out, double* ms);
 
 /* the text of the last engine error */
 const char* nm_last_error(void);

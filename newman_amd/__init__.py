@@ -259,6 +259,7 @@ _SIGS = [
     ("nm_zoom_video_start", C.c_int, [_H, C.c_int]),
     ("nm_zoom_video_step", C.c_int, [_H, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    ("nm_beauty_render", C.c_int, [_H, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_double)]),
     ("nm_last_error", C.c_char_p, []),
     ("nm_hp_op", C.c_int, [C.c_int, C.c_char_p, C.c_char_p, C.c_long, C.c_double, C.c_long,
                            C.POINTER(C.c_int), C.POINTER(C.c_long), C.c_void_p]),
@@ -418,6 +419,16 @@ class Mandelbrot:
         _check(lib().nm_colour(self._h, int(frame), p.ctypes.data, p.shape[0], 1 if smooth else 0, int(sc),
                                out.ctypes.data, C.byref(ms)), "nm_colour")
         return out, ms.value
+
+    def beauty_render(self, pal, smooth=True):
+        """beautyRender (viewer.cpp:186-253): (1080, 1920, 3) uint8 of this view
+        rendered 3 x 3-multisampled; this object is unchanged."""
+        p = np.ascontiguousarray(pal, dtype=np.uint8)
+        out = np.zeros((1080, 1920, 3), dtype=np.uint8)
+        ms = C.c_double(0.0)
+        _check(lib().nm_beauty_render(self._h, p.ctypes.data, p.shape[0], 1 if smooth else 0, out.ctypes.data,
+                                      C.byref(ms)), "nm_beauty_render")
+        return out
 
     def zoom_video_start(self, sc=1):
         """initAutoZoom (viewer.cpp:441-453)."""

@@ -650,4 +650,9 @@ int nm_zoom_video_step(nm_mandelbrot* h, const uint8_t* pal, int npal, int smoot
   return guard([&] { return h->m.zoomVideoStep(pal, npal, smooth, rate, prev_key, key_out, frames, nframes, ms); });
 }
 
+int nm_beauty_render(const nm_mandelbrot* h, const uint8_t* pal, int npal, int smooth, uint8_t* out, double* ms) {
+  if (!h || !pal || !out) return NM_ERR_ARG;
+  return guard([&] { return h->m.beautyRender(pal, npal, smooth, out, ms); });
+}
+
 }  // extern "C"

@@ -1105,6 +1105,39 @@ int Mandelbrot::colour(int frame, const uint8_t* pal_rgb, int npal, int smooth, 
   return eng->colour(grid.values.data(), 0, 0, rows(), cols(), sc, N, pal_rgb, npal, smooth, out, ms);
 }
 
+// viewer.cpp:186-253: a 1080 x 1920 x 3 x 3 grid with this object's N and centre
+// and sz scaled by rows() / 3240 (a double factor, as the viewer computes it);
+// rendered, then coloured with 3 x 3 colour averaging.  This object is unchanged,
+// and so is the process-wide default precision (which the viewer leaves at the
+// beauty grid's).
+int Mandelbrot::beautyRender(const uint8_t* pal_rgb, int npal, int smooth, uint8_t* out, double* ms) const {
+  if (!pal_rgb || !out) return NM_ERR_ARG;
+  struct KeepBits {
+    long bits = hp_default_bits();
+    ~KeepBits() { hp_set_default_bits(bits); }
+  } keep;
+  const int sc = 3;
+  Mandelbrot b(1080 * sc, 1920 * sc);
+  // N, centre and the rescaled sz are copied (viewer.cpp:195-200); the error
+  // tolerance and bailout are the new object's defaults, as there.  How the grid
+  // is computed (delta arithmetic, launch shape, probe screening) follows this one.
+  b.N = N;
+  b.delta_mode = delta_mode;
+  b.launch = launch;
+  b.probe_mode = probe_mode;
+  b.center = center;
+  const double f = (double)rows() / b.rows();
+  b.sz.re = hpf(sz.re.L);
+  b.sz.im = hpf(sz.im.L);
+  hp_mul_d(b.sz.re, sz.re, f);
+  hp_mul_d(b.sz.im, sz.im, f);
+  b.setPrecision();
+  int st = b.precompute();
+  if (st != NM_OK) return st;
+  if ((st = b.render(nullptr, true)) != NM_OK) return st;
+  return b.colour(-1, pal_rgb, npal, smooth, sc, out, ms);
+}
+
 // initAutoZoom (viewer.cpp:441-453): the full view of the set (sz = 4 / cols,
 // 3 / rows of the current grid), back to one sample per pixel, then 3 x 3 samples
 int Mandelbrot::zoomVideoStart(int sc) {

@@ -117,6 +117,9 @@ class Mandelbrot {
   // of update()'s zoom branch (viewer.cpp:270-286) with VideoZoom::nextFrame's
   // in-between frames (video.cpp:14-34)
   int zoomVideoStart(int sc);
+  // beautyRender (viewer.cpp:186-253): this view on a 3240 x 5760 grid, coloured
+  // 3 x 3-averaged into 1080 x 1920 x 3 bytes
+  int beautyRender(const uint8_t* pal_rgb, int npal, int smooth, uint8_t* out, double* ms) const;
   int zoomVideoStep(const uint8_t* pal_rgb, int npal, int smooth, int rate, const uint8_t* prev_key,
                     uint8_t* key_out, uint8_t* frames_out, int* nframes, double* ms);
   int probeCandidates() const { return probe_candidates_; }

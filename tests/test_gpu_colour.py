@@ -122,3 +122,32 @@ def test_zoom_video_keyframes(gpu):
     key2, frames, _ = m.zoom_video_step(pal, True, 9, prev_key=key1)
     assert np.array_equal(key2, O.colour(m.grid(), 256, pal, True, 2))
     assert np.array_equal(frames, O.video_frames(key1, key2, 12, 16, 9))
+
+
+def test_beauty_render(gpu):
+    """beautyRender (viewer.cpp:186-253): the view re-rendered on a 3240 x 5760
+    grid and coloured 3 x 3-averaged.  On a 3240-row handle the beauty grid is the
+    handle's own view, so the image equals the handle's render coloured with
+    sc = 3, byte for byte; the handle itself (view and grid) is unchanged."""
+    nm = gpu
+    v = _VIEWS["t13"]
+    m = nm.Mandelbrot(3240, 5760)
+    m.set_view(v["N"], v["re"], v["im"], "3e-24")
+    m.precompute()
+    m.render()
+    g = m.grid().copy()
+    pal = nm.Palette(text=DEFAULT_PAL).cache(v["N"])
+    img = m.beauty_render(pal, True)
+    assert img.shape == (1080, 1920, 3)
+    assert np.array_equal(img, O.colour(g, v["N"], pal, True, 3))
+    assert (m.rows(), m.cols()) == (3240, 5760)
+    assert np.array_equal(m.grid(), g)
+    small = nm.Mandelbrot(12, 16)  # the full view, sampled 270 times finer
+    small.N = 256
+    pal256 = nm.Palette(text=DEFAULT_PAL).cache(256)
+    img = small.beauty_render(pal256, False)
+    assert img.shape == (1080, 1920, 3)
+    assert (small.rows(), small.cols()) == (12, 16)
+    assert img[540, 960].tolist() != [0, 0, 0] or img[0, 0].tolist() != [0, 0, 0]
+    with pytest.raises(nm.NewmanError):
+        m.beauty_render(pal[: v["N"] - 1], True)
