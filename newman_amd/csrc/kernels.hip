@@ -95,12 +95,12 @@ constexpr int kLdsHead = 1024, kLdsTail = 1024;
 // what the registers allow (6 per SIMD at 76 VGPRs).
 constexpr int kSeriesIterThreads = 512;
 static_assert(kLdsTail >= 32 && kLdsHead >= 32, "a block of up to 32 steps reads inside one window");
-constexpr int kPartSlots = 8192;        // per-workgroup partial counter slots
+constexpr int kPartSlots = 10240;       // per-workgroup partial counter slots
 
 // partial-slot ranges of the kernels of one render
 constexpr int kSlotStart = 0;
 constexpr int kSlotIter = 4096;
-constexpr int kSlotSum = 6144;
+constexpr int kSlotFinal = 6144;
 
 struct Common {
   const DevFrame* frames;
@@ -119,6 +119,7 @@ struct Common {
   unsigned long long* part;
   int N;
   double b2, b2m, log_b;
+  double smooth_k;  // 0.5 ln 2 / ln(bailout), or 0 for a bailout below 2
 };
 
 struct PlainArgs {
@@ -199,9 +200,43 @@ __device__ __forceinline__ Loc locate(const DevFrame& f, uint32_t local) {
   return l;
 }
 
-// mandelbrot.cpp:133-136 (log(bailout) comes from the host)
-__device__ __forceinline__ float smooth_of(double r2, double log_b) {
-  return (float)(1.0 - log2(0.5 * log(r2) / log_b));
+// log2 q for a positive normal q, within 3e-15 absolute (measured against a
+// long-double log2 on 5e7 points of [2^-20, 2^20]): q = m 2^E with m in
+// [1/sqrt 2, sqrt 2), log2 m = (2 / ln 2) atanh(s), s = (m - 1) / (m + 1),
+// |s| < 0.1716, by the odd series through s^17 (the next term is below 4e-16).
+// The stop search compares the result with thresholds through a margin of
+// 4e-9, so this bound, not correct rounding, is what it needs; the smoothing
+// stores its result as a float.
+__device__ __forceinline__ double log2_near(double q) {
+  int E;
+  double m = frexp(q, &E);  // [0.5, 1)
+  if (m < 0.70710678118654752) {
+    m *= 2.0;
+    E -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 17.0;
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  p = fma(p, s2, 1.0);
+  return fma(2.8853900817779268147 * s, p, (double)E);  // 2 / ln 2
+}
+
+// mandelbrot.cpp:133-136: 1 - log2(0.5 ln r2 / ln b), with 0.5 ln r2 / ln b =
+// log2(r2) * (0.5 ln 2 / ln b) (smooth_k, from the host) and both base-2
+// logarithms by log2_near: within 1e-14 of the formula evaluated in double
+// precision (log2 r2 >= 2 log2 b >= 2, so its relative error is below 2e-15),
+// far below the float the value is stored in.  An infinite r2, or a bailout
+// below 2 (smooth_k = 0), takes the library logarithms.
+__device__ __forceinline__ float smooth_of(double r2, const Common& c) {
+  if (c.smooth_k == 0.0 || !(r2 < INFINITY)) return (float)(1.0 - log2(0.5 * log(r2) / c.log_b));
+  return (float)(1.0 - log2_near(log2_near(r2) * c.smooth_k));
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -335,7 +370,7 @@ constexpr int kPreIterDefault = 16;
 
 __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   const Common& c = a.c;
-  unsigned long long ncard = 0, nres = 0, executed = 0;
+  unsigned long long ncard = 0, nres = 0, executed = 0, credited = 0;
   const uint32_t ntiles = (c.total + kTile - 1) / kTile;
   const int N = c.N;
   const int pre = min(a.pre_iter, N);
@@ -378,11 +413,13 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
         }
         executed += (unsigned long long)min(k + 1, pre);
         if (k < pre) {
-          c.out[gid] = EscapeValue{k, smooth_of(mg, c.log_b)};
+          c.out[gid] = EscapeValue{k, smooth_of(mg, c)};
           nres++;
+          credited += (unsigned long long)k;
         } else if (k >= N) {
           c.out[gid] = EscapeValue{N, 0.0f};
           nres++;
+          credited += (unsigned long long)N;
         } else {
           c.state[gid] = make_double4(x, y, cx, cy);
           c.state_i[gid] = make_int2(k, 0);
@@ -394,25 +431,51 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
     }
     tile_count_store(c.tile_count, tile, mine);
   }
-  const unsigned long long v[3] = {ncard, nres, executed};
-  const int idx[3] = {C_CARDIOID, C_SA_RES, C_EXEC_START};
-  block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 3);
+  const unsigned long long v[4] = {ncard, nres, executed, credited};
+  const int idx[4] = {C_CARDIOID, C_SA_RES, C_EXEC_START, C_CREDITED};
+  block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 4);
 }
 
 // ============================================================ compaction
 // exclusive scan of tile_count (in place) by one workgroup; total -> *nactive.
-// Rounds of 4096 consecutive counts: 4 per thread, wave scans, a carry between
-// rounds (coalesced loads and stores).
+// Rounds of 16384 consecutive counts: 16 per thread (four 16-byte loads), wave
+// scans, a carry between rounds; the next round's loads are issued before this
+// round's scan, so one memory latency is paid per round at most.
+constexpr int kScanPer = 16;
+__device__ __forceinline__ void scan_load(const uint32_t* tc, uint32_t i0, uint32_t ntiles, uint32_t v[kScanPer]) {
+  if (i0 + kScanPer <= ntiles) {
+    const uint4* q = (const uint4*)(tc + i0);
+#pragma unroll
+    for (int k = 0; k < kScanPer / 4; k++) {
+      const uint4 u = q[k];
+      v[4 * k] = u.x;
+      v[4 * k + 1] = u.y;
+      v[4 * k + 2] = u.z;
+      v[4 * k + 3] = u.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanPer; k++) v[k] = i0 + k < ntiles ? tc[i0 + k] : 0u;
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint32_t ntiles, uint32_t* nactive) {
   __shared__ uint32_t wsum[16];
   const int lane = lane_id(), wave = threadIdx.x >> 6;
+  constexpr uint32_t kRound = 1024 * kScanPer;
   uint32_t carry = 0;
-  for (uint32_t base = 0; base < ntiles; base += 4096) {
-    const uint32_t i0 = base + 4 * threadIdx.x;
-    uint32_t v[4];
+  uint32_t nxt[kScanPer];
+  scan_load(tile_count, kScanPer * threadIdx.x, ntiles, nxt);
+  for (uint32_t base = 0; base < ntiles; base += kRound) {
+    const uint32_t i0 = base + kScanPer * threadIdx.x;
+    uint32_t v[kScanPer];
+    uint32_t mine = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = i0 + k < ntiles ? tile_count[i0 + k] : 0u;
-    const uint32_t mine = v[0] + v[1] + v[2] + v[3];
+    for (int k = 0; k < kScanPer; k++) {
+      v[k] = nxt[k];
+      mine += v[k];
+    }
+    if (base + kRound < ntiles) scan_load(tile_count, i0 + kRound, ntiles, nxt);
     uint32_t x = mine;  // inclusive wave scan of the thread totals
     for (int off = 1; off < 64; off <<= 1) {
       uint32_t y = __shfl_up(x, off);
@@ -427,10 +490,27 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint3
       all += t;
     }
     uint32_t run = carry + before + x - mine;
+    if (i0 + kScanPer <= ntiles) {
+      uint4* q = (uint4*)(tile_count + i0);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (i0 + k < ntiles) tile_count[i0 + k] = run;
-      run += v[k];
+      for (int k = 0; k < kScanPer / 4; k++) {
+        uint4 u;
+        u.x = run;
+        run += v[4 * k];
+        u.y = run;
+        run += v[4 * k + 1];
+        u.z = run;
+        run += v[4 * k + 2];
+        u.w = run;
+        run += v[4 * k + 3];
+        q[k] = u;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanPer; k++) {
+        if (i0 + k < ntiles) tile_count[i0 + k] = run;
+        run += v[k];
+      }
     }
     carry += all;
     __syncthreads();
@@ -438,29 +518,49 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint3
   if (threadIdx.x == 0) *nactive = carry;
 }
 
-// flagged sample ids -> active[] in sample order
+// flagged sample ids -> active[] in sample order.  A thread takes 4 consecutive
+// flags (one 32-bit load: flags are 0/1 bytes, kTile = 4 x the workgroup), so a
+// tile is one wave scan and one barrier pair; the next tile's loads are issued
+// before this tile's scan.
+static_assert(kTile == 4 * kThreads, "k_scatter: 4 flags per thread per tile");
+__device__ __forceinline__ uint32_t scatter_flags(const uint8_t* flag, uint32_t g0, uint32_t total) {
+  if (g0 + 4 <= total) return *(const uint32_t*)(flag + g0);
+  uint32_t w = 0;
+  for (int k = 0; k < 4; k++)
+    if (g0 + k < total && flag[g0 + k]) w |= 1u << (8 * k);
+  return w;
+}
+
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint8_t* flag, const uint32_t* tile_off, uint32_t total,
                                                       uint32_t* active) {
   __shared__ uint32_t wcount[kThreads / 64];
   const uint32_t ntiles = (total + kTile - 1) / kTile;
-  const int wave = threadIdx.x >> 6;
-  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    uint32_t run = tile_off[tile];
-    for (int j = 0; j < kPerThread; j++) {
-      uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
-      bool f = gid < total && flag[gid];
-      unsigned long long m = __ballot(f);
-      if (lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
-      __syncthreads();
-      uint32_t before = 0, all = 0;
-      for (int w = 0; w < kThreads / 64; w++) {
-        before += w < wave ? wcount[w] : 0u;
-        all += wcount[w];
-      }
-      if (f) active[run + before + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = gid;
-      run += all;
-      __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = lane_id();
+  uint32_t tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  uint32_t w_next = scatter_flags(flag, tile * kTile + 4 * threadIdx.x, total), off_next = tile_off[tile];
+  for (; tile < ntiles; tile += gridDim.x) {
+    const uint32_t g0 = tile * kTile + 4 * threadIdx.x;
+    const uint32_t w = w_next, off = off_next;
+    const uint32_t nt = tile + gridDim.x;
+    if (nt < ntiles) {
+      w_next = scatter_flags(flag, nt * kTile + 4 * threadIdx.x, total);
+      off_next = tile_off[nt];
     }
+    const uint32_t n = (uint32_t)__popc(w);
+    uint32_t x = n;  // inclusive wave scan
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wcount[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int v = 0; v < kThreads / 64; v++) before += v < wave ? wcount[v] : 0u;
+    uint32_t pos = off + before + x - n;
+    for (int k = 0; k < 4; k++)
+      if ((w >> (8 * k)) & 1u) active[pos++] = g0 + k;
+    __syncthreads();
   }
 }
 
@@ -637,33 +737,6 @@ __device__ __forceinline__ int first_le(const double* PM, int lo, int hi, double
 // under- or overflow.  Otherwise the reference's linear scan runs.
 constexpr double kMarg = 4.0e-9;
 
-// log2 q for a positive normal q, within 3e-15 absolute (measured against a
-// long-double log2 on 5e7 points of [2^-20, 2^20]): q = m 2^E with m in
-// [1/sqrt 2, sqrt 2), log2 m = (2 / ln 2) atanh(s), s = (m - 1) / (m + 1),
-// |s| < 0.1716, by the odd series through s^17 (the next term is below 4e-16).
-// Every use compares the result with thresholds through a margin of 4e-9, so
-// this bound, not correct rounding, is what the search needs.
-__device__ __forceinline__ double log2_near(double q) {
-  int E;
-  double m = frexp(q, &E);  // [0.5, 1)
-  if (m < 0.70710678118654752) {
-    m *= 2.0;
-    E -= 1;
-  }
-  const double s = (m - 1.0) / (m + 1.0);
-  const double s2 = s * s;
-  double p = 1.0 / 17.0;
-  p = fma(p, s2, 1.0 / 15.0);
-  p = fma(p, s2, 1.0 / 13.0);
-  p = fma(p, s2, 1.0 / 11.0);
-  p = fma(p, s2, 1.0 / 9.0);
-  p = fma(p, s2, 1.0 / 7.0);
-  p = fma(p, s2, 1.0 / 5.0);
-  p = fma(p, s2, 1.0 / 3.0);
-  p = fma(p, s2, 1.0);
-  return fma(2.8853900817779268147 * s, p, (double)E);  // 2 / ln 2
-}
-
 // log2 |e|^2 without under/overflow (to about 1e-13 absolute); -inf for e = 0
 __device__ __forceinline__ double log2_sqmag(double2 e) {
   double mx = fmax(fabs(e.x), fabs(e.y));
@@ -831,7 +904,7 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
 
 __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
   const Common& c = a.c;
-  unsigned long long ncard = 0, nres = 0, skipped = 0;
+  unsigned long long ncard = 0, nres = 0, skipped = 0, credited = 0;
   const uint32_t ntiles = (c.total + kTile - 1) / kTile;
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t mine = 0;
@@ -904,7 +977,7 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
           xl = a.Xlo[found];
           yr = trunc_sum3(zf.x, xl.x, d.x);
           yi = trunc_sum3(zf.y, xl.y, d.y);
-          res = EscapeValue{found, smooth_of(yr * yr + yi * yi, c.log_b)};
+          res = EscapeValue{found, smooth_of(yr * yr + yi * yi, c)};
           nres++;
         } else if (found + 1 < c.N) {
           // first perturbation step from Y[found] = X[found] + d (its escape
@@ -934,14 +1007,15 @@ __global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
         }
       }
       if (!want) c.out[gid] = res;
+      if (!want && !in) credited += (unsigned long long)res.iterations;
       c.flag[gid] = want ? 1 : 0;
       mine += want ? 1 : 0;
     }
     tile_count_store(c.tile_count, tile, mine);
   }
-  const unsigned long long v[3] = {ncard, nres, skipped};
-  const int idx[3] = {C_CARDIOID, C_SA_RES, C_SKIPPED};
-  block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 3);
+  const unsigned long long v[4] = {ncard, nres, skipped, credited};
+  const int idx[4] = {C_CARDIOID, C_SA_RES, C_SKIPPED, C_CREDITED};
+  block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 4);
 }
 
 // One block of K perturbation steps for a lane (Y_it = Zd[m] + d):
@@ -1288,10 +1362,29 @@ __global__ __launch_bounds__(kThreads) void k_emit(Common c, EscapeValue* grid) 
 
 __global__ __launch_bounds__(kThreads) void k_finalize(Common c) {
   const uint32_t total = *c.nactive;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-    uint32_t pix = c.active[i];
-    if (c.out[pix].iterations < c.N) c.out[pix].smoothing = smooth_of(c.r2[pix], c.log_b);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  unsigned long long credited = 0;  // the iterated samples' counts (the start kernel adds the rest)
+  // 4 samples per thread per round, their loads issued together
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += 4 * stride) {
+    uint32_t pix[4];
+    int it[4];
+    double r2[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) pix[k] = i + k * stride < total ? c.active[i + k * stride] : c.total;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      it[k] = pix[k] < c.total ? c.out[pix[k]].iterations : c.N;
+      r2[k] = pix[k] < c.total ? c.r2[pix[k]] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (pix[k] >= c.total) continue;
+      credited += (unsigned long long)it[k];
+      if (it[k] < c.N) c.out[pix[k]].smoothing = smooth_of(r2[k], c);
+    }
   }
+  const int idx[1] = {C_CREDITED};
+  block_partial(c.part, kSlotFinal + blockIdx.x, &credited, idx, 1);
 }
 
 // ============================================================ grid utilities
@@ -1332,16 +1425,6 @@ __global__ __launch_bounds__(kThreads) void k_reduce_partials(const unsigned lon
     }
     __syncthreads();
   }
-}
-
-__global__ __launch_bounds__(kThreads) void k_sum_iterations(const EscapeValue* out, unsigned long long n,
-                                                             unsigned long long* part) {
-  unsigned long long s = 0;
-  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
-       i += (unsigned long long)gridDim.x * blockDim.x)
-    s += (unsigned long long)out[i].iterations;
-  const int idx[1] = {C_CREDITED};
-  block_partial(part, kSlotSum + blockIdx.x, &s, idx, 1);
 }
 
 __global__ void k_zero(unsigned long long* p, int n) {
@@ -1551,6 +1634,7 @@ class EngineImpl : public Engine {
     c.b2 = opt.bailout * opt.bailout;
     c.b2m = c.b2 * (1.0 - 1.0 / 1099511627776.0);
     c.log_b = std::log(opt.bailout);
+    c.smooth_k = opt.bailout >= 2.0 ? 0.5 * std::log(2.0) / c.log_b : 0.0;
     return NM_OK;
   }
 
@@ -1565,7 +1649,7 @@ class EngineImpl : public Engine {
     const int gtile = grid_for_tiles(c.total);
     // waves = persistent waves per SIMD: 4 per CU for each 256 threads
     const int ti = iter_threads<Args>();
-    const int giter = std::max(1, std::min(num_cu * waves * 256 / ti, kSlotSum - kSlotIter));
+    const int giter = std::max(1, std::min(num_cu * waves * 256 / ti, kSlotFinal - kSlotIter));
     hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartSlots * C_COUNT);
     NM_HIP(hipGetLastError());
     if (evs) NM_HIP(hipEventRecord(evs[0], stream));
@@ -1581,6 +1665,7 @@ class EngineImpl : public Engine {
     hipLaunchKernelGGL(iter, dim3(giter), dim3(ti), 0, stream, args);
     NM_HIP(hipGetLastError());
     if (evs) NM_HIP(hipEventRecord(evs[3], stream));
+    static_assert(kPartSlots - kSlotFinal >= kSlotIter - kSlotStart, "finalize runs on the start grid");
     hipLaunchKernelGGL(k_finalize, dim3(gtile), dim3(kThreads), 0, stream, c);
     NM_HIP(hipGetLastError());
     if (emit_to) {
@@ -1665,9 +1750,6 @@ class EngineImpl : public Engine {
     tag_++;
     last_result = result;
     last_nout = nout;
-    hipLaunchKernelGGL(k_sum_iterations, dim3(std::min(num_cu * 4, kPartSlots - kSlotSum)), dim3(kThreads), 0,
-                       stream, result, (unsigned long long)nout, d_part);
-    NM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, stream, (const unsigned long long*)d_part,
                        kPartSlots, d_totals);
     NM_HIP(hipGetLastError());
@@ -1687,7 +1769,7 @@ class EngineImpl : public Engine {
       stats->executed_start = h[C_EXEC_START];
       stats->skipped = h[C_SKIPPED];
       stats->n_rebased = h[C_REBASED];
-      stats->credited = h[C_CREDITED] - h[C_CARDIOID] * (unsigned long long)opt.N;
+      stats->credited = h[C_CREDITED];
       stats->ms_start = ms[0];
       stats->ms_compact = ms[1];
       stats->ms_iterate = ms[2];
@@ -1739,20 +1821,17 @@ class EngineImpl : public Engine {
       ncol += f->nc;
       nrow += f->nr;
     }
-    std::vector<double> cre, crelo, cim, cimlo;
-    cre.reserve(ncol); crelo.reserve(ncol); cim.reserve(nrow); cimlo.reserve(nrow);
     const uint64_t fs = frame_samples(frames[0]->nr, frames[0]->nc, opt.tile);
     if (fs == 0) return NM_ERR_ARG;
     uint64_t nout = 0;
+    size_t tc = 0, tr = 0;
     for (auto* f : frames) {
       DevFrame d = make_frame(df.size(), f->nr, f->nc, opt.tile, (uint32_t)fs);
-      d.tab_c = (uint32_t)cre.size();
-      d.tab_r = (uint32_t)cim.size();
+      d.tab_c = (uint32_t)tc;
+      d.tab_r = (uint32_t)tr;
       d.card_mode = f->card_mode;
-      cre.insert(cre.end(), f->cre.begin(), f->cre.end());
-      crelo.insert(crelo.end(), f->crelo.begin(), f->crelo.end());
-      cim.insert(cim.end(), f->cim.begin(), f->cim.end());
-      cimlo.insert(cimlo.end(), f->cimlo.begin(), f->cimlo.end());
+      tc += f->cre.size();
+      tr += f->cim.size();
       total += fs;
       nout += (uint64_t)f->nr * f->nc;
       df.push_back(d);
@@ -1774,11 +1853,22 @@ class EngineImpl : public Engine {
     a.c.inv_frame = 1.0 / (double)a.c.frame_size;
     double* t = (double*)d_tabs;
     if (upload) {
+      // the coordinate tables of every frame, concatenated (built only when the
+      // device copy is stale: a repeated frame set reuses it)
+      std::vector<double> cre, crelo, cim, cimlo;
+      cre.reserve(ncol); crelo.reserve(ncol); cim.reserve(nrow); cimlo.reserve(nrow);
+      for (auto* f : frames) {
+        cre.insert(cre.end(), f->cre.begin(), f->cre.end());
+        crelo.insert(crelo.end(), f->crelo.begin(), f->crelo.end());
+        cim.insert(cim.end(), f->cim.begin(), f->cim.end());
+        cimlo.insert(cimlo.end(), f->cimlo.begin(), f->cimlo.end());
+      }
       NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
       NM_HIP(hipMemcpyAsync(t, cre.data(), ncol * 8, hipMemcpyHostToDevice, stream));
       NM_HIP(hipMemcpyAsync(t + ncol, crelo.data(), ncol * 8, hipMemcpyHostToDevice, stream));
       NM_HIP(hipMemcpyAsync(t + 2 * ncol, cim.data(), nrow * 8, hipMemcpyHostToDevice, stream));
       NM_HIP(hipMemcpyAsync(t + 2 * ncol + nrow, cimlo.data(), nrow * 8, hipMemcpyHostToDevice, stream));
+      NM_HIP(hipStreamSynchronize(stream));  // the host vectors end with this scope
       tables_uploaded(opt);
     }
     a.pre_iter = opt.pre_iter > 0 ? opt.pre_iter : kPreIterDefault;
@@ -1814,24 +1904,22 @@ class EngineImpl : public Engine {
       nrow += f->nr;
       if (f->card_mode == CARD_MASK) nmask += f->mask.size();
     }
-    std::vector<double> er, ei;
-    std::vector<uint32_t> mask;
-    er.reserve(ncol); ei.reserve(nrow); mask.reserve(nmask);
     uint64_t total = 0, nout = 0;
     const uint64_t fs = frame_samples(frames[0]->nr, frames[0]->nc, opt.tile);
     if (fs == 0) return NM_ERR_ARG;
+    size_t tc = 0, tr = 0, tm = 0;
     for (auto* f : frames) {
       DevFrame d = make_frame(df.size(), f->nr, f->nc, opt.tile, (uint32_t)fs);
-      d.tab_c = (uint32_t)er.size();
-      d.tab_r = (uint32_t)ei.size();
+      d.tab_c = (uint32_t)tc;
+      d.tab_r = (uint32_t)tr;
       d.card_mode = f->card_mode;
-      d.mask_base = (uint64_t)mask.size() * 32;
+      d.mask_base = (uint64_t)tm * 32;
       if (f->card_mode == CARD_MASK) {
         if (f->mask.size() * 32 < (size_t)f->nr * f->nc) return NM_ERR_ARG;
-        mask.insert(mask.end(), f->mask.begin(), f->mask.end());
+        tm += f->mask.size();
       }
-      er.insert(er.end(), f->epsre.begin(), f->epsre.end());
-      ei.insert(ei.end(), f->epsim.begin(), f->epsim.end());
+      tc += f->epsre.size();
+      tr += f->epsim.size();
       total += fs;
       nout += (uint64_t)f->nr * f->nc;
       df.push_back(d);
@@ -1840,7 +1928,7 @@ class EngineImpl : public Engine {
     const void *pf = d_frames, *pt = d_tabs, *pm = d_mask;
     if ((st = grow(&d_frames, &n_frames_bytes, df.size() * sizeof(DevFrame)))) return st;
     if ((st = grow(&d_tabs, &n_tabs_bytes, (ncol + nrow) * sizeof(double)))) return st;
-    if ((st = grow(&d_mask, &n_mask_bytes, std::max<size_t>(mask.size(), 1) * 4))) return st;
+    if ((st = grow(&d_mask, &n_mask_bytes, std::max<size_t>(nmask, 1) * 4))) return st;
     const bool upload = tables_stale(opt, pf == d_frames && pt == d_tabs && pm == d_mask);
     SeriesArgs a;
     if ((st = prepare_common(a.c, total, opt))) return st;
@@ -1854,11 +1942,22 @@ class EngineImpl : public Engine {
     a.c.inv_frame = 1.0 / (double)a.c.frame_size;
     double* t = (double*)d_tabs;
     if (upload) {
+      // every frame's delta tables and cardioid masks, concatenated (built only
+      // when the device copy is stale: a repeated frame set reuses it)
+      std::vector<double> er, ei;
+      std::vector<uint32_t> mask;
+      er.reserve(ncol); ei.reserve(nrow); mask.reserve(nmask);
+      for (auto* f : frames) {
+        if (f->card_mode == CARD_MASK) mask.insert(mask.end(), f->mask.begin(), f->mask.end());
+        er.insert(er.end(), f->epsre.begin(), f->epsre.end());
+        ei.insert(ei.end(), f->epsim.begin(), f->epsim.end());
+      }
       NM_HIP(hipMemcpyAsync(d_frames, df.data(), df.size() * sizeof(DevFrame), hipMemcpyHostToDevice, stream));
       NM_HIP(hipMemcpyAsync(t, er.data(), ncol * 8, hipMemcpyHostToDevice, stream));
       NM_HIP(hipMemcpyAsync(t + ncol, ei.data(), nrow * 8, hipMemcpyHostToDevice, stream));
       if (!mask.empty())
         NM_HIP(hipMemcpyAsync(d_mask, mask.data(), mask.size() * 4, hipMemcpyHostToDevice, stream));
+      NM_HIP(hipStreamSynchronize(stream));  // the host vectors end with this scope
       tables_uploaded(opt);
     }
     a.epsre = t;
