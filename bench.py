@@ -294,9 +294,36 @@ def tune(step, ks, waves):
     return best, rows
 
 
+def single_frame_leg(reps=50):
+    """One 640x480 frame per call, as the viewer renders it: the device pass, and
+    the whole call with the grid copied back to the host (what computeRow's
+    caller waits for)."""
+    import newman_amd as nm
+    m = nm.Mandelbrot(480, 640)
+    m.set_view(1024, "-0.75", "0", "3.0")
+    m.render()
+    s = m.render(0, 0, reps, copy_back=False)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.render()
+    wall = (time.perf_counter() - t0) / reps
+    return {"frame": "640x480, centre -0.75+0i, width 3.0, N 1024", "ms_device_pass": s["ms_total"],
+            "ms_call_with_copy_back": wall * 1e3, "calls": reps}
+
+
+def _one_thread(run, what):
+    """The reference renders on one thread: the same oracle on one core, over a
+    smaller share of the same sample (`run` returns (credited px-it, samples))."""
+    t0 = time.perf_counter()
+    credited, n = run()
+    t = time.perf_counter() - t0
+    return {"value": credited / t, "unit": "pixel-iterations/s", "cores": 1,
+            "sample": "%s (%d samples, %.3g credited px-it, %.1f s)" % (what, n, credited, t)}
+
+
 def cpu_baseline(desc, name, nthreads, rows_every):
     """The oracle (the reference's method on the CPU) on a bounded sample of the
-    same workload, timed on this host's cores."""
+    same workload, timed on this host's cores, and on one core (`single_thread`)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_lib as O
@@ -315,10 +342,17 @@ def cpu_baseline(desc, name, nthreads, rows_every):
             if time.perf_counter() - t0 > 10.0:
                 break
         t_run = time.perf_counter() - t0
+
+        def one():
+            c = 0
+            for _ in range(4):
+                c += int(o.render_rows(rows, 1)["iterations"].astype(np.int64)[~card].sum())
+            return c, 4 * rows.size * 640
         return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
                 "sample": "oracle plain-double path (mandelbrot.cpp:231-254 with the HP cardioid test) on the full "
                           "640x480 frame, repeated %d times (%.1f s); credited = iterations of non-cardioid samples"
-                          % (reps, t_run)}
+                          % (reps, t_run),
+                "single_thread": _one_thread(one, "the full frame 4 times")}
     views = load_views()
     if name == "c3":
         v = views["c3_seed1"]
@@ -334,10 +368,16 @@ def cpu_baseline(desc, name, nthreads, rows_every):
         out = o.pert_samples_ua(rc, np.ones(len(rc), dtype=np.int32), nthreads)
         t_run = time.perf_counter() - t0
         credited = int(out["iterations"].astype(np.int64).sum())
+        rc1 = np.array([(r, c) for r in rows[::4] for c in range(nc)], dtype=np.int32)
+
+        def one():
+            o1 = o.pert_samples_ua(rc1, np.ones(len(rc1), dtype=np.int32), 1)
+            return int(o1["iterations"].astype(np.int64).sum()), o1.size
         return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
                 "sample": "oracle FP64 perturbation without series (the method the GPU runs here) on rows %d::%d of "
                           "the 2560x1440 frame (%d samples, %.3g px-it, %.1f s); precompute %.1f s, not included"
-                          % (rows_every // 2, rows_every * 5, out.size, credited, t_run, t_pre)}
+                          % (rows_every // 2, rows_every * 5, out.size, credited, t_run, t_pre),
+                "single_thread": _one_thread(one, "rows %d::%d" % (rows_every // 2, rows_every * 20))}
     if name == "c5":
         v = views["c5_seed1"]
         nr, nc = 2160, 3840
@@ -351,10 +391,15 @@ def cpu_baseline(desc, name, nthreads, rows_every):
         out = o.render_rows(rows, nthreads)
         t_run = time.perf_counter() - t0
         credited = int(out["iterations"].astype(np.int64).sum())
+
+        def one():
+            o1 = o.render_rows(rows[::2], 1)
+            return int(o1["iterations"].astype(np.int64).sum()), o1.size
         return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
                 "sample": "oracle reference method (series scan + HP continuation, %d-bit) on rows %d::%d of the "
                           "3840x2160 frame (%d samples, %.3g credited px-it, %.1f s); precompute %.1f s, not "
-                          "included" % (o.bits(), rows_every // 2, rows_every * 4, out.size, credited, t_run, t_pre)}
+                          "included" % (o.bits(), rows_every // 2, rows_every * 4, out.size, credited, t_run, t_pre),
+                "single_thread": _one_thread(one, "rows %d::%d" % (rows_every // 2, rows_every * 8))}
     if name == "c4":
         v = views["c4_seed1"]
         nr, nc = 720, 1280
@@ -362,6 +407,7 @@ def cpu_baseline(desc, name, nthreads, rows_every):
         pre = 0.0
         nsamp = 0
         frames = (0, 60, 119)
+        single = None
         for k in frames:  # the reference method precomputes per keyframe (viewer.cpp:263)
             o = O.OView(nr, nc)
             o.set_view(v["N"], v["re"], v["im"], "%.17e" % (1e-40 * 1.1 ** (C4_FRAMES - 1 - k)))
@@ -374,10 +420,16 @@ def cpu_baseline(desc, name, nthreads, rows_every):
             total_s += time.perf_counter() - t0
             total_c += float(out["iterations"].astype(np.int64).sum())
             nsamp += out.size
+            if k == frames[-1]:
+                def one():
+                    o1 = o.render_rows(rows[::4], 1)
+                    return int(o1["iterations"].astype(np.int64).sum()), o1.size
+                single = _one_thread(one, "rows %d::%d of frame %d" % (rows_every // 4, rows_every * 2, k))
         return {"value": total_c / total_s, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
                 "sample": "oracle reference method (series scan + HP continuation) on rows %d::%d of frames %s "
                           "(%d samples, %.3g credited px-it, %.1f s); per-frame precompute %.1f s, not included"
-                          % (rows_every // 4, rows_every // 2, frames, nsamp, total_c, total_s, pre)}
+                          % (rows_every // 4, rows_every // 2, frames, nsamp, total_c, total_s, pre),
+                "single_thread": single}
     v = views["c2_seed1"]
     nr, nc = 1080 * 4, 1920 * 4
     o = O.OView(nr, nc)
@@ -391,11 +443,16 @@ def cpu_baseline(desc, name, nthreads, rows_every):
     t_run = time.perf_counter() - t0
     it = out["iterations"].astype(np.int64)
     credited = int(it.sum())  # deep frame: no cardioid samples
+
+    def one():
+        o1 = o.render_rows(rows[::8], 1)
+        return int(o1["iterations"].astype(np.int64).sum()), o1.size
     return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
             "sample": "oracle reference method (series scan + HP continuation, %d-bit) on rows %d::%d of the same "
                       "7680x4320 frame (%d samples, %.3g credited px-it, %.1f s); precompute (findProbe over "
                       "13680 probes + series) %.1f s, not included" % (
-                          o.bits(), rows_every // 2, rows_every, out.size, credited, t_run, t_pre)}
+                          o.bits(), rows_every // 2, rows_every, out.size, credited, t_run, t_pre),
+            "single_thread": _one_thread(one, "rows %d::%d" % (rows_every // 2, rows_every * 8))}
 
 
 def main():
@@ -485,6 +542,8 @@ def main():
     }
     if tuned:
         line["autotune"] = tuned
+    if args.config == "c1":
+        line["single_frame"] = single_frame_leg()
     if args.config == "c2":
         # the viewer's colouring of the rendered grid, on the device copy (no host
         # round trip): 4 x 4 average into 1920x1080 RGB with smoothing; then the

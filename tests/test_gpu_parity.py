@@ -76,7 +76,10 @@ def test_plain_path_symmetry(gpu):
 # error bound of the boundary (the device then decides in double-double)
 _BOUNDARY = [("0.36552730612364814", "0.29715374784582538", "1e-3"),  # t = 1.2
              ("-0.044662513057668218", "0.64384933723982285", "1e-9"),  # t = 2.0
-             ("-1.0", "0.25", "1e-13"), ("-1.1767766952966369", "0.1767766952966369", "1e-11")]
+             ("-1.0", "0.25", "1e-13"), ("-1.1767766952966369", "0.1767766952966369", "1e-11"),
+             # the cardioid's rightmost (t = pi/3) and topmost (t = 2 pi / 3) points,
+             # nearest the device's bounding box of the two components
+             ("0.375", "0.21650635094610965", "1e-6"), ("-0.125", "0.649519052838329", "1e-6")]
 
 
 @pytest.mark.parametrize("cre,cim,width", _BOUNDARY)
