@@ -121,11 +121,13 @@ WRITE_SCALE = 1048576 / 1084704
 # (the zoom-video summary averages over the autotuner's subsample dispatches too,
 # so it does not give the full frame set's traffic)
 PMC_FILES = {"c2": "profiles/series_iterate_deepzoom_1e-30_pmc.txt",
-             "c1": "profiles/plain_c1_640x480_pmc.txt"}
+             "c1": "profiles/plain_c1_640x480_pmc.txt",
+             "c4": "profiles/series_iterate_zoomvideo_120x1280x720_pmc.txt"}
 
 
 def pmc_traffic(config, kernel):
-    """HBM bytes per dispatch of `kernel` from the committed PMC summary, or None."""
+    """HBM bytes per dispatch of `kernel` (its full template name, e.g.
+    k_series_iterate<32,false>) from the committed PMC summary, or None."""
     path = os.path.join(ROOT, PMC_FILES.get(config, ""))
     if not os.path.isfile(path):
         return None
@@ -133,7 +135,7 @@ def pmc_traffic(config, kernel):
     with open(path) as f:
         for ln in f:
             p = ln.split()
-            name = p[0].split("<")[0] if p else ""
+            name = p[0] if p else ""
             if len(p) >= 3 and name == kernel and p[1] == "FETCH_SIZE_KiB":
                 fetch = float(p[2])
             if len(p) >= 3 and name == kernel and p[1] == "WRITE_SIZE_KiB":
@@ -148,7 +150,10 @@ def roofline(config, flop, s):
     over its HIP-event duration (mean over the timed launches), against the FP64
     vector peak; both kernels are listed."""
     exec_iter = s["executed"] - s["executed_start"]
+    K = s["block_k"]
     kernels = {"iterate": {"name": "k_series_iterate" if flop == FLOP_PER_PERT_STEP else "k_plain_iterate",
+                           "instance": ("k_series_iterate<%d,false>" if flop == FLOP_PER_PERT_STEP
+                                        else "k_plain_iterate<%d>") % K,
                            "ms": s["ms_iterate"], "steps": exec_iter},
                "start": {"name": "k_series_start" if flop == FLOP_PER_PERT_STEP else "k_plain_start",
                          "ms": s["ms_start"], "steps": s["executed_start"]}}
@@ -156,7 +161,8 @@ def roofline(config, flop, s):
         k["tflops"] = flop * k["steps"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
     dom = max(kernels.values(), key=lambda k: k["ms"])
     return {"bound": "valu-fp64", "achieved": dom["tflops"], "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": dom["tflops"] / FP64_VECTOR_PEAK_TFLOPS, "traffic": pmc_traffic(config, dom["name"]),
+            "frac": dom["tflops"] / FP64_VECTOR_PEAK_TFLOPS,
+            "traffic": pmc_traffic(config, dom.get("instance", dom["name"])),
             "kernel": dom["name"], "flop_per_step": flop, "executed_steps_per_launch": dom["steps"],
             "ms_per_launch": dom["ms"], "kernels": list(kernels.values())}
 
@@ -466,6 +472,9 @@ def main():
     ap.add_argument("--pre-iter", type=int, default=0, help="plain path: updates in the start kernel")
     ap.add_argument("--tile", type=int, default=0, help="sample order: 0 row-major, T = T x T tiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", type=int, default=1,
+                    help="also time the colouring and video legs (c2) and the single-frame latency (c1); 0 for the "
+                         "main line only, as the profiles are taken")
     ap.add_argument("--variants", type=int, default=1,
                     help="also time FP32X deltas and the learned tile skip on the same frame (c2, c5)")
     ap.add_argument("--tune", action="store_true", help="sweep block size and waves per SIMD first")
@@ -542,9 +551,9 @@ def main():
     }
     if tuned:
         line["autotune"] = tuned
-    if args.config == "c1":
+    if args.legs and args.config == "c1":
         line["single_frame"] = single_frame_leg()
-    if args.config == "c2":
+    if args.legs and args.config == "c2":
         # the viewer's colouring of the rendered grid, on the device copy (no host
         # round trip): 4 x 4 average into 1920x1080 RGB with smoothing; then the
         # zoom video's in-between frames from that grid's 2 x 2 colouring
