@@ -18,11 +18,9 @@ namespace nm {
 namespace {
 
 // the same conventions as clip_byte / interp in palette.cpp: clamp and truncate
-__device__ __forceinline__ uint32_t clip_u8(float v) {
-  if (!(v > 0.0f)) return 0u;
-  if (v >= 255.0f) return 255u;
-  return (uint32_t)(int)v;
-}
+// (0 for v <= 0 or NaN, 255 for v >= 255; fmaxf returns the operand that is not
+// NaN), without branches
+__device__ __forceinline__ uint32_t clip_u8(float v) { return (uint32_t)fminf(fmaxf(v, 0.0f), 255.0f); }
 
 // getColor (viewer.cpp:89-95): black at or beyond N; the palette entry; or with
 // smoothing interp(pal[it - 1], pal[it], smoothing).  pal[-1] (iterations 0 with

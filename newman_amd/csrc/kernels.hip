@@ -2141,10 +2141,12 @@ class EngineImpl : public Engine {
     const size_t kbytes = (size_t)kr * kc * 3, sbytes = sched.size() * sizeof(VideoFrame);
     const size_t obytes = (size_t)nr * nc * 3 * (size_t)rate;
     int st;
-    if ((st = grow(&d_keys, &n_keys_bytes, 2 * kbytes + sbytes + 512 + 8 * (size_t)kr * kc))) return st;
+    // layout: prev | next | schedule | widened keyframes, each 256-byte aligned
+    const size_t kstep = (kbytes + 255) & ~(size_t)255;
+    if ((st = grow(&d_keys, &n_keys_bytes, 2 * kstep + sbytes + 512 + (size_t)video_scratch_bytes(kr, kc)))) return st;
     if ((st = grow(&d_vframes, &n_vframes_bytes, obytes))) return st;
     uint8_t* kp = (uint8_t*)d_keys;
-    uint8_t* kn = kp + kbytes;
+    uint8_t* kn = kp + kstep;
     VideoFrame* ks = (VideoFrame*)(((uintptr_t)(kn + kbytes) + 255) & ~(uintptr_t)255);
     NM_HIP(hipMemcpyAsync(kp, prev, kbytes, hipMemcpyHostToDevice, stream));
     NM_HIP(hipMemcpyAsync(kn, next, kbytes, hipMemcpyHostToDevice, stream));

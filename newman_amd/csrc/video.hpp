@@ -21,11 +21,18 @@ struct VideoFrame {
 // t = i / rate; sizes truncate (int)(n * s), offsets (canvas - size) / 2.
 std::vector<VideoFrame> video_schedule(int kr, int kc, int nr, int nc, int rate);
 
+// Row pitch, in pixels, of a keyframe widened to 4 bytes per pixel: one padding
+// pixel past the last column, rounded up to 4 pixels (16-byte aligned rows).
+uint32_t widen_pitch(int kc);
+
+// Device scratch video_launch needs for the two widened keyframes.
+uint64_t video_scratch_bytes(int kr, int kc);
+
 // Renders the frames of `sched` on the device from keyframes prev and next
 // (kr x kc x 3 bytes each, interleaved RGB, on the device) into out (frames x nr x
-// nc x 3, on the device); rgba: 8 kr kc bytes of device scratch for the keyframes
-// widened to 4 bytes per pixel.  Defined in video.hip; 0 when the launches were
-// accepted.
+// nc x 3, on the device); prev and next 4-byte aligned; rgba:
+// video_scratch_bytes(kr, kc) bytes of 16-byte aligned device scratch for the
+// widened keyframes.  Defined in video.hip; 0 when the launches were accepted.
 int video_launch(void* stream, const uint8_t* prev, const uint8_t* next, int kr, int kc, int nr, int nc,
                  const VideoFrame* sched_dev, int frames, uint8_t* rgba, uint8_t* out);
 
