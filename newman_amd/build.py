@@ -55,7 +55,7 @@ def build_library(verbose=True):
 
 def build_oracle():
     """The CPU oracle is test infrastructure (oracle/); built here for convenience."""
-    _run(["make", "-C", os.path.join(ROOT, "oracle")])
+    _run(["make", "-B", "-C", os.path.join(ROOT, "oracle")])  # always recompiled, like the library
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
