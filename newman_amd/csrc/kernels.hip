@@ -27,8 +27,8 @@
 //   scan        exclusive scan of the tile counts (one workgroup)
 //   scatter     compacts flagged sample ids into active[] in sample order
 //   *_iterate   persistent waves, one sample per lane; blocks of K steps; the
-//               first escape of a lane records its step and |z|^2 and poisons the
-//               lane with NaN (escape is absorbing); lanes are refilled (ballot +
+//               first escape of a lane in a block records its step and |z|^2
+//               (FP32X: and poisons the lane with NaN); lanes are refilled (ballot +
 //               popcount ranks) from 64-sample chunks of active[], chunk c owned by
 //               wave c mod W, with the next chunk's states prefetched; the series
 //               kernel keeps the orbit's head and tail windows in LDS
@@ -582,9 +582,10 @@ __device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t ch
 // Iterate kernel for the samples that outlive the start kernel: z <- sq(z) + z0
 // with the reference's operation order, count = index of the update that
 // escapes (mandelbrot.cpp:243-246).  A block runs K updates and records the first
-// escape with selects: escape is absorbing (|z| > bailout >= 2 keeps growing,
-// then overflows to inf and NaN, which fails the test), so the first escape is
-// the one update whose test holds and did not hold one update before.
+// escape with selects: the update whose test holds while no earlier update's of
+// the block did (a flag in a scalar mask; escape need not be absorbing: |z| can
+// fall back under a bailout below 2 + |c|, and overflow turns into NaN, which
+// fails the test).
 template <int K>
 __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
   const Common& c = a.c;
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
     const int k0 = k;
     int escj = -1;  // the block's first escaping update
     double esc_mg = 0.0;
-    bool eprev = false;
+    bool seen = false;  // an escape earlier in the block
 #pragma unroll
     for (int j = 0; j < K; j++) {
       double t = x + x;
@@ -664,8 +665,8 @@ __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
       i2 = y * y;
       const double mg = r2 + i2;  // sqMag (mandelbrot.cpp:245)
       const bool e = mg > c.b2;
-      const bool first = e && !eprev;
-      eprev = e;
+      const bool first = e && !seen;
+      seen = seen || e;
       escj = first ? j : escj;
       esc_mg = first ? mg : esc_mg;
       // the record is complete at the end of each update (no per-update values
@@ -1047,10 +1048,10 @@ __device__ __forceinline__ int orbit_index(P p, P head) {
 // head - j, so step j' > j reads Zd[j' - j].  Each entry is loaded one step
 // ahead, as soon as that step's rebase is decided.
 // The first escape (exact sqMag > bailout^2, mandelbrot.cpp:60) records its
-// step and |Y|^2.  Once |Y| > bailout >= 2, |Y| grows every step (|Y^2 + c| >=
-// |Y|^2 - 2, and the rounding of the perturbed form is many orders below that
-// growth; overflow gives inf and then NaN, which fails the test), so the first
-// escape is the one step where the test holds and did not hold one step before.
+// step and |Y|^2: the step whose test holds while no earlier step's of the block
+// did (a flag in a scalar mask: escape need not be absorbing, |Y| can fall back
+// under a bailout below 2 + |c|, and overflow turns into NaN, which fails the
+// test).
 // A block may end after K/2 or 3K/4 steps when at least kExitDone lanes of the
 // wave have nothing left to do in it (their sample escaped, or the lane was idle)
 // and the wave can refill them; it returns the steps it ran.
@@ -1064,7 +1065,7 @@ __device__ __forceinline__ int pert_block(double& dr, double& di, const double e
   int kdone = K;
   bool moved = false;
   int escj = -1;  // step of the block's first escape
-  bool eprev = false;
+  bool seen = false;  // an escape earlier in the block
   ld_double2 z = zp[0];
 #pragma unroll
   for (int j = 0; j < K; j++) {
@@ -1088,8 +1089,8 @@ __device__ __forceinline__ int pert_block(double& dr, double& di, const double e
     }
     if (j + 1 < K) z = zp[j + 1];
     const bool e = r2 > b2;
-    const bool first = e && !eprev;
-    eprev = e;
+    const bool first = e && !seen;
+    seen = seen || e;
     escj = first ? j : escj;
     esc_r2 = first ? r2 : esc_r2;
     dr = fma(d0r, wr, fma(-d0i, wi, er));

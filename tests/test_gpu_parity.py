@@ -51,8 +51,11 @@ def test_plain_path_config1_full_frame(gpu):
     assert st["credited"] == esc.sum() - st["n_cardioid"] * 1024
 
 
-def test_plain_path_bailout_2(gpu):
-    m, o = _pair(120, 160, 1024, "-0.75", "0", "3.0", bailout=2.0)
+@pytest.mark.parametrize("bailout", [2.0, 1.5, 1e6])
+def test_plain_path_bailout_2(gpu, bailout):
+    """Other bailout radii (the configuration names radius 2): the smoothing's
+    series logarithms for radii >= 2, the library's below 2."""
+    m, o = _pair(120, 160, 1024, "-0.75", "0", "3.0", bailout=bailout)
     m.render()
     g = m.grid()
     ref = o.render_rows(np.arange(120))
