@@ -30,8 +30,9 @@
 //               first escape of a lane in a block records its step and |z|^2
 //               (FP32X: and poisons the lane with NaN); lanes are refilled (ballot +
 //               popcount ranks) from 64-sample chunks of active[], chunk c owned by
-//               wave c mod W, with the next chunk's states prefetched; the series
-//               kernel keeps the orbit's head and tail windows in LDS
+//               wave c mod W, with the next chunk prefetched (plain: its states;
+//               series: its ids); the series kernel keeps the orbit's head and
+//               tail windows in LDS
 //   finalize    smoothing of the iterated samples from r2
 //   emit        tiled sample orders only: results into the row-major grids
 // Counters go to per-workgroup partial slots reduced on the device.
@@ -92,7 +93,7 @@ constexpr int kHead = 64;               // orbit padding past its end (>= max K)
 constexpr int kLdsHead = 1024, kLdsTail = 1024;
 // The series iterate kernel runs 8 waves per workgroup: the 32 KB of LDS orbit
 // windows is shared by 8 waves, so LDS does not cap the waves a CU holds below
-// what the registers allow (6 per SIMD at 76 VGPRs).
+// what the registers allow (8 per SIMD at 64 VGPRs: four workgroups, 128 KB).
 constexpr int kSeriesIterThreads = 512;
 static_assert(kLdsTail >= 32 && kLdsHead >= 32, "a block of up to 32 steps reads inside one window");
 constexpr int kPartSlots = 10240;       // per-workgroup partial counter slots
@@ -1176,9 +1177,14 @@ __device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, flo
 // FP64 (FX = false) or FP32X (FX = true) perturbation with rebasing (replaces the
 // HP continuation of mandelbrot.cpp:209-224).  Persistent waves, lanes refilled
 // from 64-sample chunks; blocks of K steps, each lane reading the orbit through
-// its own cursor into the workgroup's LDS windows or HBM.
+// its own cursor into the workgroup's LDS windows or HBM.  The FP64 kernel is
+// held to 64 VGPRs (8 waves per SIMD, four 512-thread workgroups per CU): only
+// the next chunk's sample ids are prefetched, its states are loaded when it
+// becomes current (a wave waits one load per 64 refills, while the other waves
+// of its SIMD run); one 8-byte value spills, outside the loop.
 template <int K, bool FX>
-__global__ __launch_bounds__(kSeriesIterThreads) void k_series_iterate(SeriesArgs a) {
+__global__ __launch_bounds__(kSeriesIterThreads) __attribute__((amdgpu_waves_per_eu(FX || K < 16 ? 1 : 8, 8)))
+void k_series_iterate(SeriesArgs a) {
   const Common& c = a.c;
   const uint32_t total = *c.nactive;
   const uint32_t W = gridDim.x * (blockDim.x >> 6);
@@ -1200,16 +1206,11 @@ __global__ __launch_bounds__(kSeriesIterThreads) void k_series_iterate(SeriesArg
   uint32_t ccur = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   uint32_t ncur = chunk_items(ccur, total), nnext = chunk_items(ccur + W, total);
   uint32_t cur_pix = load_pix(c.active, ccur, total), nxt_pix = load_pix(c.active, ccur + W, total);
-  uint32_t far_pix = load_pix(c.active, ccur + 2 * W, total);
-  double4 cur_s = make_double4(0.0, 0.0, 0.0, 0.0), nxt_s = cur_s;
-  int2 cur_i = make_int2(0, 0), nxt_i = cur_i;
+  double4 cur_s = make_double4(0.0, 0.0, 0.0, 0.0);
+  int2 cur_i = make_int2(0, 0);
   if ((uint32_t)lane_id() < ncur) {
     cur_s = c.state[cur_pix];
     cur_i = c.state_i[cur_pix];
-  }
-  if ((uint32_t)lane_id() < nnext) {
-    nxt_s = c.state[nxt_pix];
-    nxt_i = c.state_i[nxt_pix];
   }
   uint32_t pos = 0;
   bool act = false;
@@ -1246,19 +1247,16 @@ __global__ __launch_bounds__(kSeriesIterThreads) void k_series_iterate(SeriesArg
         }
       }
       pos += take;
-      if (pos == ncur) {  // rotate: NEXT becomes CUR, the far chunk is decoded
+      if (pos == ncur) {  // rotate: NEXT becomes CUR, its states loaded now
         ccur += W;
         ncur = nnext;
         cur_pix = nxt_pix;
-        cur_s = nxt_s;
-        cur_i = nxt_i;
-        nnext = chunk_items(ccur + W, total);
-        nxt_pix = far_pix;
-        if ((uint32_t)lane_id() < nnext) {
-          nxt_s = c.state[nxt_pix];
-          nxt_i = c.state_i[nxt_pix];
+        if ((uint32_t)lane_id() < ncur) {
+          cur_s = c.state[cur_pix];
+          cur_i = c.state_i[cur_pix];
         }
-        far_pix = load_pix(c.active, ccur + 2 * W, total);
+        nnext = chunk_items(ccur + W, total);
+        nxt_pix = load_pix(c.active, ccur + W, total);
         pos = 0;
       }
       need = __ballot(!act);
