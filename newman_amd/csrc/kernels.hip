@@ -904,7 +904,12 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_series_start(SeriesArgs a) {
+// The series start kernel: per sample the reference's series start
+// (mandelbrot.cpp:144-206) and the first perturbation step; samples that need
+// more are flagged for the iterate kernel.  Held to 80 VGPRs (6 waves per SIMD;
+// one 8-byte value spills): 5% less time than at its natural 89 (5 waves) where
+// the stop search dominates.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_series_start(SeriesArgs a) {
   const Common& c = a.c;
   unsigned long long ncard = 0, nres = 0, skipped = 0, credited = 0;
   const uint32_t ntiles = (c.total + kTile - 1) / kTile;
