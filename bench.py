@@ -510,7 +510,7 @@ def main():
         tuned = {"best": best, "sweep": table, "seconds": time.perf_counter() - t0}
         print("# autotune: %s (%.1f s)" % (best, tuned["seconds"]), file=sys.stderr, flush=True)
     elif args.tune:
-        best, rows = tune(step, [8, 16, 32] if args.config == "c1" else [8, 12, 16, 32], [2, 4, 6, 8, 12])
+        best, rows = tune(step, [8, 16, 32, 64] if args.config == "c1" else [12, 16, 32, 64], [2, 4, 6, 8])
         k, w = best[0], best[1]
         tuned = {"best": {"block_k": k, "waves_per_simd": w, "ms": best[2]},
                  "sweep": [{"block_k": a, "waves_per_simd": b, "ms": c} for a, b, c in rows]}

@@ -95,7 +95,7 @@ constexpr int kLdsHead = 1024, kLdsTail = 1024;
 // windows is shared by 8 waves, so LDS does not cap the waves a CU holds below
 // what the registers allow (8 per SIMD at 64 VGPRs: four workgroups, 128 KB).
 constexpr int kSeriesIterThreads = 512;
-static_assert(kLdsTail >= 32 && kLdsHead >= 32, "a block of up to 32 steps reads inside one window");
+static_assert(kLdsTail >= 64 && kLdsHead >= 64, "a block of up to 64 steps reads inside one window");
 constexpr int kPartSlots = 10240;       // per-workgroup partial counter slots
 
 // partial-slot ranges of the kernels of one render
@@ -1982,9 +1982,10 @@ class EngineImpl : public Engine {
       a.skip_tpf = ntc * ntr;
     }
     const bool fx = opt.delta_mode == DELTA_FP32X;
-    // FP64 blocks hold one orbit entry per lane whatever K is (76 VGPRs), so the
-    // longest block amortises its set-up best; FP32X blocks preload K entries
-    const int K = opt.block_k > 0 ? opt.block_k : fx ? 8 : 32;
+    // FP64 blocks hold one orbit entry per lane whatever K is (64 VGPRs), so the
+    // longest block amortises its set-up best; FP32X blocks renormalise their
+    // mantissa at block ends, so they stay short
+    const int K = opt.block_k > 0 ? opt.block_k : fx ? 8 : 64;
     const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
     if (stats) {
       stats->block_k = K;
@@ -2003,6 +2004,9 @@ class EngineImpl : public Engine {
       case 32:
         return fx ? run(a, k_series_start, k_series_iterate<32, true>, waves, opt, stats, nout, host_out, emit_to)
                   : run(a, k_series_start, k_series_iterate<32, false>, waves, opt, stats, nout, host_out, emit_to);
+      case 64:  // FP64 only: an FP32X mantissa is renormalised at block ends
+        if (fx) return NM_ERR_ARG;
+        return run(a, k_series_start, k_series_iterate<64, false>, waves, opt, stats, nout, host_out, emit_to);
       default: return NM_ERR_ARG;
     }
   }

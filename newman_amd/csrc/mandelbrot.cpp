@@ -1192,13 +1192,14 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
   const size_t step = std::max<size_t>(1, (n + 15) / 16);
   const int tiles[4] = {0, 32, 64, 128};
   const int waves[4] = {2, 4, 6, 8};
-  const int ks_sa[3] = {8, 16, 32}, ks_hw[2] = {16, 32};
+  // series blocks: FP64 up to 64 steps, FP32X (renormalised at block ends) up to 32
+  const int ks_sa[3] = {16, 32, 64}, ks_fx[3] = {8, 16, 32}, ks_hw[2] = {16, 32};
   const int nks = set.hw ? 2 : 3;
   Launch best;
   best.ms = 1e300;
   if (table) table->clear();
   RenderStats keep = stats_;
-  const int* ks = set.hw ? ks_hw : ks_sa;
+  const int* ks = set.hw ? ks_hw : delta_mode == DELTA_FP32X ? ks_fx : ks_sa;
   const uint64_t subset_serial = next_serial();  // the subsample's tables stay on the device
   for (int tile : tiles)
     for (int w : waves)

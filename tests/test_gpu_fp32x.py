@@ -53,6 +53,18 @@ def test_fp32x_bit_exact_vs_oracle(gpu, name, nr, nc, width, K):
     assert st["block_k"] == K
 
 
+def test_fp32x_refuses_64_step_blocks(gpu):
+    """FP32X mantissas are renormalised at block ends: 64-step blocks are FP64 only."""
+    nm = gpu
+    m, _ = _pair(*CASES[0])
+    m.set_delta_mode(1)
+    with pytest.raises(nm.NewmanError) as e:
+        m.render(block_k=64)
+    assert e.value.code == nm.NM_ERR_ARG
+    m.set_delta_mode(0)
+    assert m.render(block_k=64)["block_k"] == 64
+
+
 @pytest.mark.parametrize("name,nr,nc,width", CASES)
 def test_fp32x_error_against_fp64_and_reference(gpu, name, nr, nc, width):
     """The per-sample error of FP32X deltas, reported and bounded: counts equal to

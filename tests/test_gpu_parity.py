@@ -164,7 +164,7 @@ def test_series_path_bit_exact_vs_method_oracle(gpu, name, nr, nc, width):
     assert st["n_iterated"] + st["n_sa_resolved"] + st["n_cardioid"] == nr * nc
 
 
-@pytest.mark.parametrize("K", [12, 16, 32])
+@pytest.mark.parametrize("K", [12, 16, 32, 64])
 @pytest.mark.parametrize("name,nr,nc,width", [DEEP_CASES[4], DEEP_CASES[7]])
 def test_series_block_size_keeps_results(gpu, name, nr, nc, width, K):
     """A block only groups K steps of the same per-step arithmetic (escape record,
