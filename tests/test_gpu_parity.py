@@ -102,11 +102,12 @@ def test_plain_path_cardioid_boundary_frames(gpu, cre, cim, width):
     assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
 
 
-@pytest.mark.parametrize("pre", [1, 16, 200])
-def test_plain_path_pre_iteration_depth(gpu, pre):
-    """The start kernel's pre-iteration depth only moves work between kernels."""
+@pytest.mark.parametrize("pre,K", [(1, 16), (16, 16), (200, 16), (1, 8), (1, 64), (16, 64)])
+def test_plain_path_pre_iteration_depth(gpu, pre, K):
+    """The start kernel's pre-iteration depth and the iterate kernel's block size
+    only move work between kernels and blocks."""
     m, o = _pair(120, 160, 700, "-0.75", "0.05", "2.9")
-    m.set_launch(pre_iter=pre, block_k=16)
+    m.set_launch(pre_iter=pre, block_k=K)
     m.render()
     ref = o.render_rows(np.arange(120))
     g = m.grid()
