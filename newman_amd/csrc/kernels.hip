@@ -1413,21 +1413,26 @@ __global__ void k_downsample(const EscapeValue* fine, int nr, int nc, int sc, Es
 
 // sum of reported iterations over the output (credited work)
 // partial slots -> one total per counter (one workgroup)
-__global__ __launch_bounds__(kThreads) void k_reduce_partials(const unsigned long long* part, int nslots,
-                                                              unsigned long long* totals) {
-  __shared__ unsigned long long acc[kThreads / 64];
+// one workgroup of 1024: each thread sums every counter of its slots (one
+// contiguous C_COUNT-entry record per slot), then one wave reduction per counter
+constexpr int kReduceThreads = 1024;
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_partials(const unsigned long long* part, int nslots,
+                                                                    unsigned long long* totals) {
+  __shared__ unsigned long long acc[kReduceThreads / 64][C_COUNT];
+  unsigned long long v[C_COUNT] = {};
+  for (int sl = threadIdx.x; sl < nslots; sl += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < C_COUNT; k++) v[k] += part[(size_t)sl * C_COUNT + k];
+#pragma unroll
   for (int k = 0; k < C_COUNT; k++) {
-    unsigned long long v = 0;
-    for (int sl = threadIdx.x; sl < nslots; sl += blockDim.x) v += part[(size_t)sl * C_COUNT + k];
-    v = wave_sum(v);
-    if (lane_id() == 0) acc[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long t = 0;
-      for (int w = 0; w < kThreads / 64; w++) t += acc[w];
-      totals[k] = t;
-    }
-    __syncthreads();
+    const unsigned long long sk = wave_sum(v[k]);
+    if (lane_id() == 0) acc[threadIdx.x >> 6][k] = sk;
+  }
+  __syncthreads();
+  if (threadIdx.x < C_COUNT) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kReduceThreads / 64; w++) t += acc[w][threadIdx.x];
+    totals[threadIdx.x] = t;
   }
 }
 
@@ -1754,7 +1759,7 @@ class EngineImpl : public Engine {
     tag_++;
     last_result = result;
     last_nout = nout;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, stream, (const unsigned long long*)d_part,
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kReduceThreads), 0, stream, (const unsigned long long*)d_part,
                        kPartSlots, d_totals);
     NM_HIP(hipGetLastError());
     uint32_t nact = 0;
