@@ -111,7 +111,7 @@ struct Common {
   uint32_t total;
   EscapeValue* out;
   double* r2;  // |z|^2 at the escape of iterated samples (per sample)
-  double4* state;  // per sample: plain (x, y, cx, cy); series (dr, di, er, ei)
+  double4* state;  // per sample: plain (x, y, cx, cy); series: (dr, di) as double2[]
   int2* state_i;   // per sample: plain (k, -); series (it, m)
   uint8_t* flag;
   uint32_t* tile_count;
@@ -1005,7 +1005,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
           }
           double ndr = fma(dr, wr, fma(-di, wi, e.x));
           double ndi = fma(dr, wi, fma(di, wr, e.y));
-          c.state[gid] = make_double4(ndr, ndi, e.x, e.y);
+          // the delta only: the iterate kernel reads epsilon from the tables
+          ((double2*)c.state)[gid] = make_double2(ndr, ndi);
           c.state_i[gid] = make_int2(found + 1, m + 1);
           want = true;
           skipped += (unsigned long long)(found + 1);
@@ -1025,12 +1026,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
   block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 4);
 }
 
-// One block of K perturbation steps for a lane (Y_it = Zd[m] + d):
-//   d' = d (Zd[m] + Y) + e, rebase to the orbit start (m = 0, d = Y) when
-//   |Y| < |d| (or, in END blocks, when m reaches the end of the reference).
-// zb holds Zd[m0 .. m0+K-1]; a lane that rebased reads the orbit head from LDS.
-// The first escape (exact sqMag > bailout^2, mandelbrot.cpp:60) records its step
-// and |Y|^2 and poisons the lane with NaN so later steps cannot escape again.
+// epsilon of sample gid (its pixel's offset from the reference point), from the
+// frame's delta tables, as the series start read it
+__device__ __forceinline__ double2 sample_eps(const SeriesArgs& a, uint32_t gid) {
+  const Common& c = a.c;
+  const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
+  const Loc lc = locate(f, gid - f.sample_base);
+  return make_double2(a.epsre[f.tab_c + lc.c], a.epsim[f.tab_r + lc.r]);
+}
+
+// an orbit entry as one 16-byte vector, and pointers to the orbit in LDS and HBM
 typedef double ld_double2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) const ld_double2* lds_orbit;  // the orbit in LDS
 typedef __attribute__((address_space(1))) const ld_double2* hbm_orbit;  // the orbit in HBM
@@ -1211,11 +1216,13 @@ void k_series_iterate(SeriesArgs a) {
   uint32_t ccur = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   uint32_t ncur = chunk_items(ccur, total), nnext = chunk_items(ccur + W, total);
   uint32_t cur_pix = load_pix(c.active, ccur, total), nxt_pix = load_pix(c.active, ccur + W, total);
-  double4 cur_s = make_double4(0.0, 0.0, 0.0, 0.0);
+  const double2* dstate = (const double2*)c.state;  // the series start's deltas
+  double2 cur_s = make_double2(0.0, 0.0), cur_e = cur_s;
   int2 cur_i = make_int2(0, 0);
   if ((uint32_t)lane_id() < ncur) {
-    cur_s = c.state[cur_pix];
+    cur_s = dstate[cur_pix];
     cur_i = c.state_i[cur_pix];
+    cur_e = sample_eps(a, cur_pix);
   }
   uint32_t pos = 0;
   bool act = false;
@@ -1231,8 +1238,8 @@ void k_series_iterate(SeriesArgs a) {
       uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane_id()) - 1ull));
       int src = (int)min(pos + rank, ncur - 1);
       uint32_t g_pix = __shfl(cur_pix, src);
-      double g0 = __shfl(cur_s.x, src), g1 = __shfl(cur_s.y, src), g2 = __shfl(cur_s.z, src),
-             g3 = __shfl(cur_s.w, src);
+      double g0 = __shfl(cur_s.x, src), g1 = __shfl(cur_s.y, src), g2 = __shfl(cur_e.x, src),
+             g3 = __shfl(cur_e.y, src);
       int g4 = __shfl(cur_i.x, src), g5 = __shfl(cur_i.y, src);
       if (!act && rank < take) {
         pix = g_pix;
@@ -1257,8 +1264,9 @@ void k_series_iterate(SeriesArgs a) {
         ncur = nnext;
         cur_pix = nxt_pix;
         if ((uint32_t)lane_id() < ncur) {
-          cur_s = c.state[cur_pix];
+          cur_s = dstate[cur_pix];
           cur_i = c.state_i[cur_pix];
+          cur_e = sample_eps(a, cur_pix);
         }
         nnext = chunk_items(ccur + W, total);
         nxt_pix = load_pix(c.active, ccur + W, total);
