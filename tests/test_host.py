@@ -157,6 +157,10 @@ def test_bad_arguments_return_status():
     assert L.nm_set_view(h, 10, b"0", b"0", b"-1") == nm.NM_ERR_ARG
     assert L.nm_load_legacy(h, b"/nonexistent/view.dat") == nm.NM_ERR_IO
     assert L.nm_compute_row(h, 99) == nm.NM_ERR_ARG
+    out = (C.c_uint8 * 3)()
+    assert L.nm_beauty_render(h, None, 256, 1, out, None) == nm.NM_ERR_ARG
+    assert L.nm_beauty_render(h, out, 1, 1, None, None) == nm.NM_ERR_ARG
+    assert L.nm_set_bailout(h, 1.0) == nm.NM_ERR_ARG  # the bailout must exceed 1
     L.nm_destroy(h)
 
 
