@@ -60,6 +60,9 @@ typedef struct nm_render_stats {
   double ms_finalize;    /* smoothing / output kernel, ms */
   int32_t graph;         /* 1: ms_total is the mean of HIP graph replays of the pass */
   int32_t tile;          /* sample order used: 0 row-major, T = T x T tiles */
+  int32_t fma2;          /* plain path: 1 when y' = (2x) y + c ran as fma(x y, 2, c), bit-identical for frames
+                            whose coordinates are 0 or at least 2^-200 in magnitude */
+  int32_t reserved;
 } nm_render_stats;
 
 typedef struct nm_mandelbrot nm_mandelbrot; /* class Mandelbrot (mandelbrot.h:7) */

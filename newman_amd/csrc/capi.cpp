@@ -72,6 +72,8 @@ static void copy_stats(const nm::RenderStats& s, nm_render_stats* o) {
   o->ms_finalize = s.ms_finalize;
   o->graph = s.graph;
   o->tile = s.tile;
+  o->fma2 = s.fma2;
+  o->reserved = 0;
 }
 
 extern "C" {

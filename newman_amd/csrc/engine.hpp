@@ -29,6 +29,9 @@ enum DeltaMode : int32_t { DELTA_FP64 = 0, DELTA_FP32X = 1 };
 struct HwFrameHost {
   int nr = 0, nc = 0;
   int card_mode = CARD_TEST_DD;
+  // every coordinate is 0 or at least 2^-200 in magnitude: the device may fold
+  // (2x) y + c into fma(x y, 2, c) bit for bit (see plain_update)
+  bool fma_ok = false;
   std::vector<double> cre, crelo;  // nc
   std::vector<double> cim, cimlo;  // nr
 };
@@ -102,6 +105,7 @@ struct RenderStats {
   double ms_total = 0;       // all of one pass; every time is a mean over repeats
   int graph = 0;             // 1: ms_total is the mean of HIP graph replays
   int tile = 0;              // the sample order used
+  int fma2 = 0;              // plain path: the folded update ran (plain_update)
   int block_k = 0, waves_per_simd = 0;
 };
 

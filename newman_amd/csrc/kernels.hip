@@ -307,32 +307,6 @@ __device__ bool dd_in_cardioid(double xh, double xl, double yh, double yl) {
   return dd_lt(lhs2, dd{0.0625, 0.0});
 }
 
-// The same test in double with a forward error bound: +1 / -1 when the sign of
-// each polynomial is certain (the remainders xl, yl of the coordinates enter
-// through the gradient), 0 when double-double must decide.  With u = 2^-53 and
-// at most 7 roundings per polynomial, |computed - exact| <= 8 u M, where M sums
-// the magnitudes of the terms; the factor 32 leaves a wide margin.
-__device__ __forceinline__ int cardioid_filter(double x, double xl, double y, double yl) {
-  const double u = 1.1102230246251565e-16;
-  const double xm = x - 0.25, y2 = y * y;
-  const double q = xm * xm + y2;
-  const double t = q + xm;
-  const double f = q * t - 0.25 * y2;
-  const double m1 = fabs(q) * (fabs(q) + fabs(xm)) + 0.25 * y2;
-  const double dx = 2.0 * fabs(xm) * (2.0 * fabs(q) + fabs(xm)) + fabs(q);
-  const double dy = 2.0 * fabs(y) * (2.0 * fabs(q) + fabs(xm)) + 0.5 * fabs(y);
-  const double b1 = 32.0 * u * m1 + 2.0 * (fabs(xl) * dx + fabs(yl) * dy);
-  if (f < -b1) return 1;
-  if (!(f > b1)) return 0;
-  const double xp = x + 1.0;
-  const double g = xp * xp + y2 - 0.0625;
-  const double m2 = xp * xp + y2 + 0.0625;
-  const double b2 = 32.0 * u * m2 + 2.0 * (fabs(xl) * 2.0 * fabs(xp) + fabs(yl) * 2.0 * fabs(y));
-  if (g < -b2) return 1;
-  if (g > b2) return -1;
-  return 0;
-}
-
 // descend(X[k] + d) with X[k] = hi + lo: round-to-nearest sum, then one step
 // toward zero when the exact residual has the opposite sign.
 __device__ __forceinline__ double trunc_sum3(double hi, double lo, double d) {
@@ -369,6 +343,91 @@ __device__ __forceinline__ void tile_count_store(uint32_t* tile_count, uint32_t 
 // the persistent kernel with their state (z, c, updates done).
 constexpr int kPreIterDefault = 16;
 
+// One update z <- sq(z) + z0 (complex.h:19-27) with the reference's rounding:
+// x' = (r2 - i2) + cx, y' = (2x) y + cy, r2 = x'^2, i2 = y'^2.  FMA2 computes y'
+// as fma(x y, 2, cy): 2x is exact and fl((2x) y) = 2 fl(x y) whenever x y is 0
+// or a normal double, so the single rounding of fma(p, 2, cy) = fl(2p + cy) is
+// the reference's fl(fl((2x) y) + cy), bit for bit.  x y is 0 or normal before
+// escape when every coordinate of the frame is 0 or at least 2^-200 in
+// magnitude (a nonzero sum of two doubles is at least 2^-54 times the smaller
+// one's binade, so |y| >= 2^-254 unless y = 0 and |x| >= 2^-560 unless x = 0)
+// and the bailout is at most 2^400; the host checks both (fma_ok) and runs the
+// three-operation form otherwise.  Seven FP64 instructions instead of eight for
+// the same eight algorithmic flops.
+template <bool FMA2>
+__device__ __forceinline__ void plain_update(double& x, double& y, double& r2, double& i2, const double cx,
+                                             const double cy) {
+  if (FMA2) {
+    const double p = x * y;
+    y = fma(p, 2.0, cy);
+  } else {
+    const double t = x + x;
+    y = t * y + cy;
+  }
+  x = r2 - i2 + cx;
+  r2 = x * x;
+  i2 = y * y;
+}
+
+// The main cardioid and the period-2 disk lie inside x in [-1.25, 0.375],
+// |y| <= 0.65: the cardioid e^{it}/2 - e^{2it}/4 reaches Re 3/8 at t = pi/3 and
+// |Im| 3 sqrt(3) / 8 = 0.6495 at t = 2 pi/3, the disk spans [-1.25, -0.75].
+// Samples outside the box below skip the test (the HP remainders, below
+// 2^-52 |x|, do not reach the margins).
+__device__ __forceinline__ bool card_box(double x, double y) { return x > -1.3 && x < 0.42 && fabs(y) < 0.7; }
+
+// mandelbrot.cpp:63-71 in double inside that box, with fixed error bounds: +1 in,
+// -1 out, 0 when double-double must decide.  In the box |x - 1/4| <= 1.55,
+// q <= 2.9, |q (q + x - 1/4)| + y^2 / 4 <= 13.2, so the computed cardioid
+// polynomial errs by at most 8 u 13.2 < 1.2e-14, and the HP remainders of the
+// coordinates (below 2.2e-16 for x, 1.1e-16 for y) move it by at most
+// 25.6 |xl| + 10.6 |yl| < 7e-15; the disk polynomial errs by at most 2e-15 in
+// all.  The bounds used, 1e-12 and 1e-13, leave a margin of 50.
+constexpr double kCardBound = 1e-12, kDiskBound = 1e-13;
+__device__ __forceinline__ int cardioid_box_filter(double x, double y) {
+  const double xm = x - 0.25, y2 = y * y;
+  const double q = xm * xm + y2;
+  const double f = q * (q + xm) - 0.25 * y2;
+  if (f < -kCardBound) return 1;
+  if (!(f > kCardBound)) return 0;
+  const double xp = x + 1.0;
+  const double g = xp * xp + y2 - 0.0625;
+  if (g < -kDiskBound) return 1;
+  if (g > kDiskBound) return -1;
+  return 0;
+}
+
+// Pre-iteration of the start kernel: a lane's state, and the wave's scalar
+// mask of lanes still iterating.
+struct PreLane {
+  double &x, &y, &r2, &i2;
+  const double cx, cy;
+  unsigned long long& live;
+  int& kesc;      // update whose test first held (-1: none yet)
+  double& mgesc;  // |z|^2 there
+};
+
+// Updates k0 + J .. k0 + 3 of a lockstep block of four (unrolled by recursion);
+// a lane leaves the live mask at its first escape, which it records.
+template <int J, bool FMA2>
+__device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int k0) {
+  if constexpr (J < 4) {
+    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+    const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
+    const bool e = mg > b2;
+    const unsigned long long esc = __ballot(e) & L.live;
+    if (esc) {
+      if (e && ((L.live >> lane_id()) & 1ull)) {
+        L.kesc = k0 + J;
+        L.mgesc = mg;
+      }
+      L.live &= ~esc;
+    }
+    pre_block<J + 1, FMA2>(L, b2, k0);
+  }
+}
+
+template <bool FMA2>
 __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   const Common& c = a.c;
   unsigned long long ncard = 0, nres = 0, executed = 0, credited = 0;
@@ -378,56 +437,66 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t mine = 0;
     for (int j = 0; j < kPerThread; j++) {
-      uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
-      if (gid >= c.total) continue;
-      const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
-      const Loc lc = locate(f, gid - f.sample_base);
-      if (!lc.valid) {
-        c.flag[gid] = 0;
-        continue;
-      }
-      const uint32_t r = lc.r, col = lc.c;
+      // every lane of the wave runs this body (no early continue), so the
+      // pre-iteration below is wave-uniform control flow
+      const uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
+      bool valid = gid < c.total;
       bool in = false;
-      const double cx = a.cre[f.tab_c + col], cy = a.cim[f.tab_r + r];
-      if (f.card_mode == CARD_ALL) in = true;
-      else if (f.card_mode == CARD_TEST_DD) {
-        const double xl = a.crelo[f.tab_c + col], yl = a.cimlo[f.tab_r + r];
-        const int fc = cardioid_filter(cx, xl, cy, yl);
-        in = fc > 0 || (fc == 0 && dd_in_cardioid(cx, xl, cy, yl));
+      double cx = 0.0, cy = 0.0;
+      if (valid) {
+        const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
+        const Loc lc = locate(f, gid - f.sample_base);
+        valid = lc.valid;
+        if (valid) {
+          cx = a.cre[f.tab_c + lc.c];
+          cy = a.cim[f.tab_r + lc.r];
+          if (f.card_mode == CARD_ALL) in = true;
+          else if (f.card_mode == CARD_TEST_DD && card_box(cx, cy)) {
+            int fc = cardioid_box_filter(cx, cy);
+            if (fc == 0) {
+              const double xl = a.crelo[f.tab_c + lc.c], yl = a.cimlo[f.tab_r + lc.r];
+              fc = dd_in_cardioid(cx, xl, cy, yl) ? 1 : -1;
+            }
+            in = fc > 0;
+          }
+        }
+      }
+      // z <- sq(z) + z0 in the reference's operation order (complex.h:19-27), in
+      // lockstep blocks of four updates while some lane of the wave iterates
+      double x = cx, y = cy, r2 = x * x, i2 = y * y, mgesc = 0.0;
+      int kesc = -1;
+      unsigned long long live = __ballot(valid && !in);
+      int kend = 0;
+      PreLane L{x, y, r2, i2, cx, cy, live, kesc, mgesc};
+      while (live != 0 && kend < pre) {
+        pre_block<0, FMA2>(L, c.b2, kend);
+        kend += 4;
       }
       bool want = false;
-      if (in) {
+      if (valid && in) {
         c.out[gid] = EscapeValue{N, 0.0f};
         ncard++;
-      } else {
-        // z <- sq(z) + z0 in the reference's operation order (complex.h:19-27)
-        double x = cx, y = cy, r2 = x * x, i2 = y * y, mg = 0.0;
-        int k = 0;
-        for (; k < pre; k++) {
-          double t = x + x;
-          y = t * y + cy;
-          x = r2 - i2 + cx;
-          r2 = x * x;
-          i2 = y * y;
-          mg = r2 + i2;
-          if (mg > c.b2) break;
-        }
-        executed += (unsigned long long)min(k + 1, pre);
-        if (k < pre) {
-          c.out[gid] = EscapeValue{k, smooth_of(mg, c)};
+      } else if (valid) {
+        // the reference tests k >= N before each update: an escape counts
+        // only below N; a lane still iterating after kend updates is handed on
+        if (kesc >= 0 && kesc < N) {
+          executed += (unsigned long long)(kesc + 1);
+          c.out[gid] = EscapeValue{kesc, smooth_of(mgesc, c)};
           nres++;
-          credited += (unsigned long long)k;
-        } else if (k >= N) {
+          credited += (unsigned long long)kesc;
+        } else if (kesc >= N || kend >= N) {
+          executed += (unsigned long long)N;
           c.out[gid] = EscapeValue{N, 0.0f};
           nres++;
           credited += (unsigned long long)N;
         } else {
+          executed += (unsigned long long)kend;
           c.state[gid] = make_double4(x, y, cx, cy);
-          c.state_i[gid] = make_int2(k, 0);
+          c.state_i[gid] = make_int2(kend, 0);
           want = true;
         }
       }
-      c.flag[gid] = want ? 1 : 0;
+      if (gid < c.total) c.flag[gid] = want ? 1 : 0;
       mine += want ? 1 : 0;
     }
     tile_count_store(c.tile_count, tile, mine);
@@ -580,14 +649,67 @@ __device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t ch
   return (uint32_t)lane_id() < n ? active[chunk * kChunk + lane_id()] : 0u;
 }
 
+// idle lanes (escaped or never filled) at which a block ends early to refill them
+constexpr int kPlainExitIdle = 12;
+
+// A lane's iteration state inside a block of the plain iterate kernel; `live`
+// is the wave's scalar mask of lanes still iterating.
+struct PlainLane {
+  double &x, &y, &r2, &i2;
+  const double cx, cy;
+  unsigned long long& live;
+  bool& act;
+  unsigned long long& executed;
+};
+
+// Updates J .. K - 1 of a block (unrolled by recursion): returns the updates run.
+template <int J, int K, bool FMA2>
+__device__ __forceinline__ int plain_block(PlainLane& L, const Common& c, const uint32_t pix, const int k0,
+                                           const bool can_refill) {
+  if constexpr (J == K) {
+    return K;
+  } else {
+    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+    const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
+    const unsigned long long esc = __ballot(mg > c.b2) & L.live;
+    if (esc) {
+      L.live &= ~esc;
+      if (L.act && mg > c.b2) {
+        // the reference tests k >= N before each update: an escape at k0 + J
+        // counts only below N (a sample that reached N earlier in the block
+        // ends with N)
+        const int cnt = k0 + J;
+        if (cnt < c.N) {
+          c.out[pix] = EscapeValue{cnt, 0.0f};
+          c.r2[pix] = mg;  // smoothing: k_finalize
+          L.executed += (unsigned long long)(J + 1);
+        } else {
+          c.out[pix] = EscapeValue{c.N, 0.0f};
+          L.executed += (unsigned long long)(c.N - k0);
+        }
+        L.act = false;
+      }
+    }
+    // every 8 updates: end the block when enough lanes wait for a refill
+    if constexpr ((J & 7) == 7 && J + 1 < K) {
+      if (can_refill && __popcll(~L.live) >= kPlainExitIdle) return J + 1;
+    }
+    return plain_block<J + 1, K, FMA2>(L, c, pix, k0, can_refill);
+  }
+}
+
 // Iterate kernel for the samples that outlive the start kernel: z <- sq(z) + z0
 // with the reference's operation order, count = index of the update that
-// escapes (mandelbrot.cpp:243-246).  A block runs K updates and records the first
-// escape with selects: the update whose test holds while no earlier update's of
-// the block did (a flag in a scalar mask; escape need not be absorbing: |z| can
-// fall back under a bailout below 2 + |c|, and overflow turns into NaN, which
-// fails the test).
-template <int K>
+// escapes (mandelbrot.cpp:243-246).  A block runs up to K updates.  The escape
+// test of each update is one compare into a wave mask; the wave's live lanes are
+// a scalar mask, so an update in which no live lane escapes costs the compare
+// and a scalar branch, nothing else.  When some live lane escapes, a handler
+// (taken on about one update in eight) writes its count and |z|^2 and retires
+// it; every 8 updates the block ends early when kPlainExitIdle lanes are idle
+// and work is left, so they are refilled.  The first escape is recorded because the lane leaves the
+// live mask at once (escape need not be absorbing: |z| can fall back under a
+// bailout below 2 + |c|, and overflow turns into NaN, which fails the test).
+template <int K, bool FMA2>
 __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
   const Common& c = a.c;
   const uint32_t total = *c.nactive;
@@ -652,43 +774,20 @@ __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
       }
       need = __ballot(!act);
     }
-    if (__ballot(act) == 0) break;
+    unsigned long long live = __ballot(act);  // wave-uniform: the lanes iterating a sample
+    if (live == 0) break;
     const int k0 = k;
-    int escj = -1;  // the block's first escaping update
-    double esc_mg = 0.0;
-    bool seen = false;  // an escape earlier in the block
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-      double t = x + x;
-      y = t * y + cy;
-      x = r2 - i2 + cx;
-      r2 = x * x;
-      i2 = y * y;
-      const double mg = r2 + i2;  // sqMag (mandelbrot.cpp:245)
-      const bool e = mg > c.b2;
-      const bool first = e && !seen;
-      seen = seen || e;
-      escj = first ? j : escj;
-      esc_mg = first ? mg : esc_mg;
-      // the record is complete at the end of each update (no per-update values
-      // kept live to the end of the block)
-      __asm__ volatile("" ::"v"(escj), "v"(esc_mg));
-    }
+    const bool can_refill = pos < ncur || nnext > 0;  // wave-uniform
+    PlainLane L{x, y, r2, i2, cx, cy, live, act, executed};
+    const int kdone = plain_block<0, K, FMA2>(L, c, pix, k0, can_refill);
     if (act) {
-      // the reference tests k >= N before each update: an escape at k0 + j
-      // counts only below N, and a block that reaches N ends the sample there
-      if (escj >= 0 && k0 + escj < N) {
-        c.out[pix] = EscapeValue{k0 + escj, 0.0f};
-        c.r2[pix] = esc_mg;  // smoothing: k_finalize
-        executed += (unsigned long long)(escj + 1);
-        act = false;
-      } else if (k0 + K >= N) {
+      if (k0 + kdone >= N) {  // reached N inside the block
         c.out[pix] = EscapeValue{N, 0.0f};
         executed += (unsigned long long)(N - k0);
         act = false;
       } else {
-        executed += K;
-        k = k0 + K;
+        executed += (unsigned long long)kdone;
+        k = k0 + kdone;
       }
     }
   }
@@ -1900,13 +1999,23 @@ class EngineImpl : public Engine {
       stats->waves_per_simd = waves;
     }
     // 256-thread workgroups = 4 waves = one per SIMD: num_cu * waves workgroups
+    // the folded update (plain_update) needs every frame's coordinates clear of
+    // tiny nonzero values and a bailout of at most 2^400
+    bool fma2 = opt.bailout <= 0x1p400;
+    for (auto* f : frames) fma2 = fma2 && f->fma_ok;
+    if (stats) stats->fma2 = fma2 ? 1 : 0;
+#define NM_PLAIN(KK)                                                                                      \
+  return fma2 ? run(a, k_plain_start<true>, k_plain_iterate<KK, true>, waves, opt, stats, nout, host_out, \
+                    emit_to)                                                                              \
+              : run(a, k_plain_start<false>, k_plain_iterate<KK, false>, waves, opt, stats, nout, host_out, emit_to)
     switch (K) {
-      case 8: return run(a, k_plain_start, k_plain_iterate<8>, waves, opt, stats, nout, host_out, emit_to);
-      case 16: return run(a, k_plain_start, k_plain_iterate<16>, waves, opt, stats, nout, host_out, emit_to);
-      case 32: return run(a, k_plain_start, k_plain_iterate<32>, waves, opt, stats, nout, host_out, emit_to);
-      case 64: return run(a, k_plain_start, k_plain_iterate<64>, waves, opt, stats, nout, host_out, emit_to);
+      case 8: NM_PLAIN(8);
+      case 16: NM_PLAIN(16);
+      case 32: NM_PLAIN(32);
+      case 64: NM_PLAIN(64);
       default: return NM_ERR_ARG;
     }
+#undef NM_PLAIN
   }
 
   int render_sa(const std::vector<const SaFrameHost*>& frames, const RenderOpts& opt, EscapeValue* host_out,

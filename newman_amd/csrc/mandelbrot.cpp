@@ -660,6 +660,8 @@ int Mandelbrot::buildHwFrame(HwFrameHost& f, const HPComplex& cen, const HPCompl
   }
   int m = classify_frame(cen, s, f.nr, f.nc);
   f.card_mode = m < 0 ? CARD_TEST_DD : m;
+  auto clear = [](double v) { return v == 0.0 || std::fabs(v) >= 0x1p-200; };
+  f.fma_ok = std::all_of(f.cre.begin(), f.cre.end(), clear) && std::all_of(f.cim.begin(), f.cim.end(), clear);
   return NM_OK;
 }
 

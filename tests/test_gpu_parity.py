@@ -82,7 +82,9 @@ _BOUNDARY = [("0.36552730612364814", "0.29715374784582538", "1e-3"),  # t = 1.2
              ("-1.0", "0.25", "1e-13"), ("-1.1767766952966369", "0.1767766952966369", "1e-11"),
              # the cardioid's rightmost (t = pi/3) and topmost (t = 2 pi / 3) points,
              # nearest the device's bounding box of the two components
-             ("0.375", "0.21650635094610965", "1e-6"), ("-0.125", "0.649519052838329", "1e-6")]
+             ("0.375", "0.21650635094610965", "1e-6"), ("-0.125", "0.649519052838329", "1e-6"),
+             # wide frames across the edges of that box (x = 0.42, |y| = 0.7, x = -1.3)
+             ("0.40", "0.2", "0.1"), ("-0.125", "0.68", "0.1"), ("-1.27", "0.0", "0.1")]
 
 
 @pytest.mark.parametrize("cre,cim,width", _BOUNDARY)
