@@ -1,21 +1,28 @@
 #!/usr/bin/env python
-"""bench.py - pixel-iterations/s of newman's per-pixel deep-zoom iteration on one MI355X.
+"""bench.py - pixel-iterations/s of newman's per-pixel iteration on one MI355X.
 
-Workload (default): the deep-zoom FP64-delta configuration - 1920x1080 output with
-4x4 multisampling (7680x4320 samples), view width 1e-30, N = 20000, bailout 1024,
-series tolerance 1e-10, centre from the seed-1 zoom walk (tests/golden/views.json).
-One step = one pass of the hot path over the frame: the series-start kernel plus
-the persistent FP64 perturbation kernel, with the reference orbit and series
-already resident in HBM (the host HP precompute runs once, before timing).
+Workload (default line): the plain-double configuration - 640x480, centre
+-0.75+0i, width 3.0, N = 1024, no multisampling, smoothing on, bailout 1024 (the
+reference's constant, mandelbrot.cpp:58; the radius-2 figure is reported beside
+it), the reference's plain path (getIterationsHW, mandelbrot.cpp:231-254, as
+useHardware() selects at this zoom).  One frame is ~2 us of GPU work, so one
+step = one pass of the hot path over 512 copies of the frame in one launch set
+(start kernel: cardioid test and the first updates; persistent iterate kernel;
+smoothing), the coordinate tables already resident in HBM.
 
 value = credited pixel-iterations per second: the sum of reported escape counts
-over samples not resolved by the cardioid/bulb test, series-skipped iterations
-included, divided by wall time per step.  roofline = algorithmic FP64 FLOPs of the
-iteration kernel (15 per executed perturbation step) over its HIP-event duration,
-against the FP64 vector peak.  cpu_baseline = the CPU oracle (the reference method:
-series scan + HP continuation) on a row subsample of the same frame.
+over samples not resolved by the cardioid/bulb test, divided by wall time per
+step.  roofline = algorithmic FP64 FLOPs (8 per plain update: 3 mul + 5 add) of
+the dominant kernel's executed updates over its HIP-event duration, against the
+FP64 vector peak; both plain kernels are listed.  cpu_baseline = the CPU oracle
+(the reference's plain path with its HP cardioid test) on the same frame.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5]
+deep_zoom (extra key) = the deep-zoom FP64-delta configuration: 1920x1080 output
+with 4x4 multisampling (7680x4320 samples), width 1e-30, N = 20000, centre from
+the seed-1 zoom walk, with the FP32X-delta and learned-skip variants on the same
+frame, the colouring and zoom-video legs and its own CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c4|c5]
 """
 import argparse
 import json
@@ -286,6 +293,20 @@ def video_leg(m, rate):
         "note": "VALU-bound: the exact float bilinear taps cost ~190 VALU instructions per output pixel"}
 
 
+def bailout_leg(m, step, k, w, args):
+    """The same 512-frame set at bailout radius 2, as the configuration names it
+    (the reference's constant is 1024): credited px-it/s and the kernels' times."""
+    m.set_bailout(2.0)
+    step(k, w, 1)
+    t0 = time.perf_counter()
+    s = step(k, w, args.steps)
+    wall = (time.perf_counter() - t0) / args.steps
+    m.set_bailout(1024.0)
+    return {"bailout": 2.0, "value": s["credited"] / wall, "ms_per_step": wall * 1e3,
+            "credited_px_it": s["credited"], "ms_start": s["ms_start"], "ms_iterate": s["ms_iterate"],
+            "roofline": roofline("c1_b2", FLOP_PER_PLAIN_STEP, s)}
+
+
 def tune(step, ks, waves):
     """Sweep block size and persistent waves per SIMD; kernel time by HIP events."""
     best = None
@@ -467,7 +488,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--deep", type=int, default=1,
+                    help="c1: also run the deep-zoom configuration (c2) as the deep_zoom key")
     ap.add_argument("--block-k", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--pre-iter", type=int, default=0, help="plain path: updates in the start kernel")
@@ -487,20 +510,36 @@ def main():
               "rows/tiles with the reference orbit broadcast once; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    line = measure(args.config, args)
+    line = dict({"metric": METRIC}, **line)
+    if args.config == "c1" and args.deep:
+        # the deep-zoom configuration under its own key, with its variants, the
+        # colouring and video legs and its CPU baseline
+        print("# deep_zoom (config 2) ...", file=sys.stderr, flush=True)
+        line["deep_zoom"] = measure("c2", args, block_k=0, waves=0)
+    print(json.dumps(line), flush=True)
+
+
+def measure(config, args, block_k=None, waves=None):
+    """One workload: warm-up, `steps` timed passes in one engine call, and its
+    side legs; returns the bench line's fields (without the metric name)."""
     import newman_amd as nm  # noqa: F401
-    m, desc, flop, step = make_workload(args.config)
+    m, desc, flop, step = make_workload(config)
     t0 = time.perf_counter()
     if not m.useHardware():
         m.precompute()
     t_pre = time.perf_counter() - t0
     n_hp = m.probe_candidates() if not m.useHardware() else 0
-    print("# precompute %.2f s (findProbe screened on the GPU, %d probe orbits in HP; series in HP), len(X) = %s"
-          % (t_pre, n_hp, m.reference_length()), file=sys.stderr, flush=True)
-    k, w = args.block_k, args.waves
-    if args.config != "c4":
-        m.set_launch(tile=args.tile, pre_iter=args.pre_iter)
+    if not m.useHardware():
+        print("# %s: precompute %.2f s (findProbe screened on the GPU, %d probe orbits in HP; series in HP), "
+              "len(X) = %s" % (config, t_pre, n_hp, m.reference_length()), file=sys.stderr, flush=True)
+    k = args.block_k if block_k is None else block_k
+    w = args.waves if waves is None else waves
+    if config != "c4":
+        m.set_launch(tile=args.tile if config == args.config else 0,
+                     pre_iter=args.pre_iter if config == args.config else 0)
     tuned = None
-    if args.config == "c4":
+    if config == "c4":
         # the project's launch autotuner (sample order x waves x block size), then
         # graph replay of the repeated passes
         t0 = time.perf_counter()
@@ -509,8 +548,8 @@ def main():
         k, w = best["block_k"], best["waves_per_simd"]
         tuned = {"best": best, "sweep": table, "seconds": time.perf_counter() - t0}
         print("# autotune: %s (%.1f s)" % (best, tuned["seconds"]), file=sys.stderr, flush=True)
-    elif args.tune:
-        best, rows = tune(step, [8, 16, 32, 64] if args.config == "c1" else [12, 16, 32, 64], [2, 4, 6, 8])
+    elif args.tune and config == args.config:
+        best, rows = tune(step, [8, 16, 32, 64] if config == "c1" else [12, 16, 32, 64], [2, 4, 6, 8])
         k, w = best[0], best[1]
         tuned = {"best": {"block_k": k, "waves_per_simd": w, "ms": best[2]},
                  "sweep": [{"block_k": a, "waves_per_simd": b, "ms": c} for a, b, c in rows]}
@@ -525,7 +564,6 @@ def main():
     credited = s["credited"]
     value = credited / wall
     line = {
-        "metric": METRIC,
         "value": value,
         "unit": "pixel-iterations/s",
         "n_gpus": 1,
@@ -536,38 +574,39 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (fixed view)" if args.config == "c1" else "synthetic (seeded zoom-walk view)",
+        "data": "synthetic (fixed view)" if config == "c1" else "synthetic (seeded zoom-walk view)",
         "config": dict(desc, parallelism="single GPU", block_k=s["block_k"], waves_per_simd=s["waves_per_simd"],
                        tile=s["tile"], graph=s["graph"], pre_iter=m.get_launch()["pre_iter"]),
-        "roofline": roofline(args.config, flop, s),
+        "roofline": roofline(config, flop, s),
         "detail": {"credited_px_it": credited, "executed_px_it": s["executed"],
                    "executed_in_start_kernel": s["executed_start"], "skipped": s["skipped"],
                    "n_iterated": s["n_iterated"], "n_sa_resolved": s["n_sa_resolved"],
                    "n_cardioid": s["n_cardioid"], "n_rebased_blocks": s["n_rebased"],
                    "ms_start": s["ms_start"], "ms_compact": s["ms_compact"], "ms_iterate": s["ms_iterate"],
                    "ms_finalize": s["ms_finalize"], "ms_kernels": s["ms_total"], "precompute_s": t_pre,
-                   "probe_orbits_hp": n_hp,
+                   "probe_orbits_hp": n_hp, "folded_update": bool(s.get("fma2", 0)),
                    "kernel_px_it_per_s": credited / (s["ms_total"] * 1e-3)},
         "cpu_baseline": None,
     }
     if tuned:
         line["autotune"] = tuned
-    if args.legs and args.config == "c1":
+    if args.legs and config == "c1":
         line["single_frame"] = single_frame_leg()
-    if args.legs and args.config == "c2":
+        line["bailout_2"] = bailout_leg(m, step, k, w, args)
+    if args.legs and config == "c2":
         # the viewer's colouring of the rendered grid, on the device copy (no host
         # round trip): 4 x 4 average into 1920x1080 RGB with smoothing; then the
         # zoom video's in-between frames from that grid's 2 x 2 colouring
         line["colour"] = colour_leg(m, 4)
         line["video"] = video_leg(m, 15)
-    if args.variants and args.config in ("c2", "c5"):
+    if args.variants and config in ("c2", "c5"):
         line["variants"] = variants(m, step, args, value, wall)
-    if args.variants and args.config == "c3":
+    if args.variants and config == "c3":
         line["variants"] = fp32x_variant(m, step, args, value)
     if not args.no_cpu_baseline:
-        print("# cpu baseline ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(desc, args.config, args.cpu_threads, args.cpu_rows_every)
-    print(json.dumps(line), flush=True)
+        print("# cpu baseline (%s) ..." % config, file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(desc, config, args.cpu_threads, args.cpu_rows_every)
+    return line
 
 
 if __name__ == "__main__":

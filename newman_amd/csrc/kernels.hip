@@ -229,15 +229,43 @@ __device__ __forceinline__ double log2_near(double q) {
   return fma(2.8853900817779268147 * s, p, (double)E);  // 2 / ln 2
 }
 
+// log2 q for a positive normal q by a table: q = m 2^E with m in [1, 2); the top
+// seven fraction bits of m pick r_i, a double near 1 / c_i for the centre c_i of
+// m's 1/128-wide interval, and l_i = -log2 r_i (rounded from a long double
+// evaluation on the host, d_log2tab); then f = m r_i - 1 (one rounding by the
+// fma, |f| <= 2^-8) and log2 m = l_i + log2(1 + f), the series through f^7 (the
+// next term is below 2^-66).  The error is that of l_i and of the final
+// additions: below 2^-52 + ulp(E + l_i), 3.6e-15 over [2^-20, 2^20] (measured
+// against a long-double log2 on 2e7 points).  About 18 VALU instructions where
+// log2_near's division costs about 30.
+constexpr int kLog2Tab = 128;
+__constant__ double2 d_log2tab[kLog2Tab];  // (r_i, l_i), filled at engine start
+__device__ __forceinline__ double log2_tab(double q) {
+  const long long b = __double_as_longlong(q);
+  const int E = (int)((b >> 52) & 0x7ff) - 1023;
+  const double m = __longlong_as_double((b & 0x000fffffffffffffLL) | 0x3ff0000000000000LL);
+  const double2 t = d_log2tab[(int)((b >> 45) & (kLog2Tab - 1))];
+  const double f = fma(m, t.x, -1.0);
+  // (-1)^(k+1) / (k ln 2), k = 7 .. 1
+  double p = 0.2060992915555662;
+  p = fma(p, f, -0.2404491734814939);
+  p = fma(p, f, 0.28853900817779266);
+  p = fma(p, f, -0.36067376022224085);
+  p = fma(p, f, 0.4808983469629878);
+  p = fma(p, f, -0.7213475204444817);
+  p = fma(p, f, 1.4426950408889634);
+  return ((double)E + t.y) + f * p;
+}
+
 // mandelbrot.cpp:133-136: 1 - log2(0.5 ln r2 / ln b), with 0.5 ln r2 / ln b =
 // log2(r2) * (0.5 ln 2 / ln b) (smooth_k, from the host) and both base-2
-// logarithms by log2_near: within 1e-14 of the formula evaluated in double
-// precision (log2 r2 >= 2 log2 b >= 2, so its relative error is below 2e-15),
-// far below the float the value is stored in.  An infinite r2, or a bailout
-// below 2 (smooth_k = 0), takes the library logarithms.
+// logarithms by log2_tab: within 1e-15 of the formula evaluated in long double
+// (measured on 2e7 radii at bailout 1024; the stored float differed from the
+// correctly rounded one once), far below the float the value is stored in.  An
+// infinite r2, or a bailout below 2 (smooth_k = 0), takes the library logarithms.
 __device__ __forceinline__ float smooth_of(double r2, const Common& c) {
   if (c.smooth_k == 0.0 || !(r2 < INFINITY)) return (float)(1.0 - log2(0.5 * log(r2) / c.log_b));
-  return (float)(1.0 - log2_near(log2_near(r2) * c.smooth_k));
+  return (float)(1.0 - log2_tab(log2_tab(r2) * c.smooth_k));
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -397,12 +425,13 @@ __device__ __forceinline__ int cardioid_box_filter(double x, double y) {
   return 0;
 }
 
-// Pre-iteration of the start kernel: a lane's state, and the wave's scalar
-// mask of lanes still iterating.
+// Pre-iteration of the start kernel: a lane's state, its own flag and the
+// wave's scalar mask of lanes still iterating.
 struct PreLane {
   double &x, &y, &r2, &i2;
   const double cx, cy;
   unsigned long long& live;
+  bool& alive;
   int& kesc;      // update whose test first held (-1: none yet)
   double& mgesc;  // |z|^2 there
 };
@@ -416,12 +445,13 @@ __device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int
     const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
     const bool e = mg > b2;
     const unsigned long long esc = __ballot(e) & L.live;
-    if (esc) {
-      if (e && ((L.live >> lane_id()) & 1ull)) {
+    if (__builtin_expect(esc != 0, 0)) {
+      L.live &= ~esc;
+      if (L.alive && e) {
         L.kesc = k0 + J;
         L.mgesc = mg;
+        L.alive = false;
       }
-      L.live &= ~esc;
     }
     pre_block<J + 1, FMA2>(L, b2, k0);
   }
@@ -465,9 +495,10 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
       // lockstep blocks of four updates while some lane of the wave iterates
       double x = cx, y = cy, r2 = x * x, i2 = y * y, mgesc = 0.0;
       int kesc = -1;
-      unsigned long long live = __ballot(valid && !in);
+      bool alive = valid && !in;
+      unsigned long long live = __ballot(alive);
       int kend = 0;
-      PreLane L{x, y, r2, i2, cx, cy, live, kesc, mgesc};
+      PreLane L{x, y, r2, i2, cx, cy, live, alive, kesc, mgesc};
       while (live != 0 && kend < pre) {
         pre_block<0, FMA2>(L, c.b2, kend);
         kend += 4;
@@ -649,68 +680,43 @@ __device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t ch
   return (uint32_t)lane_id() < n ? active[chunk * kChunk + lane_id()] : 0u;
 }
 
-// idle lanes (escaped or never filled) at which a block ends early to refill them
-constexpr int kPlainExitIdle = 12;
-
 // A lane's iteration state inside a block of the plain iterate kernel; `live`
-// is the wave's scalar mask of lanes still iterating.
+// is the wave's scalar mask of lanes still iterating, `act` the lane's own flag.
 struct PlainLane {
   double &x, &y, &r2, &i2;
   const double cx, cy;
   unsigned long long& live;
   bool& act;
-  unsigned long long& executed;
+  int& jesc;  // update of the block at which the lane escaped (-1: none)
 };
 
-// Updates J .. K - 1 of a block (unrolled by recursion): returns the updates run.
+// Updates J .. K - 1 of a block (unrolled by recursion).  The handler of an
+// escape writes the count and |z|^2 (finalize turns them into smoothing; the
+// block's end replaces a count at or beyond N by N) and leaves the live mask;
+// the no-escape case falls through.
 template <int J, int K, bool FMA2>
-__device__ __forceinline__ int plain_block(PlainLane& L, const Common& c, const uint32_t pix, const int k0,
-                                           const bool can_refill) {
-  if constexpr (J == K) {
-    return K;
-  } else {
+__device__ __forceinline__ void plain_block(PlainLane& L, const Common& c, const uint32_t pix, const int k0) {
+  if constexpr (J < K) {
     plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
     const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
-    const unsigned long long esc = __ballot(mg > c.b2) & L.live;
-    if (esc) {
+    const bool e = mg > c.b2;
+    const unsigned long long esc = __ballot(e) & L.live;
+    if (__builtin_expect(esc != 0, 0)) {
       L.live &= ~esc;
-      if (L.act && mg > c.b2) {
-        // the reference tests k >= N before each update: an escape at k0 + J
-        // counts only below N (a sample that reached N earlier in the block
-        // ends with N)
-        const int cnt = k0 + J;
-        if (cnt < c.N) {
-          c.out[pix] = EscapeValue{cnt, 0.0f};
-          c.r2[pix] = mg;  // smoothing: k_finalize
-          L.executed += (unsigned long long)(J + 1);
-        } else {
-          c.out[pix] = EscapeValue{c.N, 0.0f};
-          L.executed += (unsigned long long)(c.N - k0);
-        }
+      if (L.act && e) {
+        c.out[pix] = EscapeValue{k0 + J, 0.0f};
+        c.r2[pix] = mg;  // smoothing: k_finalize
+        L.jesc = J;
         L.act = false;
       }
     }
-    // every 8 updates: end the block when enough lanes wait for a refill
-    if constexpr ((J & 7) == 7 && J + 1 < K) {
-      if (can_refill && __popcll(~L.live) >= kPlainExitIdle) return J + 1;
-    }
-    return plain_block<J + 1, K, FMA2>(L, c, pix, k0, can_refill);
+    plain_block<J + 1, K, FMA2>(L, c, pix, k0);
   }
 }
 
-// Iterate kernel for the samples that outlive the start kernel: z <- sq(z) + z0
-// with the reference's operation order, count = index of the update that
-// escapes (mandelbrot.cpp:243-246).  A block runs up to K updates.  The escape
-// test of each update is one compare into a wave mask; the wave's live lanes are
-// a scalar mask, so an update in which no live lane escapes costs the compare
-// and a scalar branch, nothing else.  When some live lane escapes, a handler
-// (taken on about one update in eight) writes its count and |z|^2 and retires
-// it; every 8 updates the block ends early when kPlainExitIdle lanes are idle
-// and work is left, so they are refilled.  The first escape is recorded because the lane leaves the
-// live mask at once (escape need not be absorbing: |z| can fall back under a
-// bailout below 2 + |c|, and overflow turns into NaN, which fails the test).
 template <int K, bool FMA2>
-__global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_plain_iterate(
+    PlainArgs a) {
   const Common& c = a.c;
   const uint32_t total = *c.nactive;
   const uint32_t W = gridDim.x * (blockDim.x >> 6);
@@ -777,18 +783,20 @@ __global__ __launch_bounds__(kThreads) void k_plain_iterate(PlainArgs a) {
     unsigned long long live = __ballot(act);  // wave-uniform: the lanes iterating a sample
     if (live == 0) break;
     const int k0 = k;
-    const bool can_refill = pos < ncur || nnext > 0;  // wave-uniform
-    PlainLane L{x, y, r2, i2, cx, cy, live, act, executed};
-    const int kdone = plain_block<0, K, FMA2>(L, c, pix, k0, can_refill);
-    if (act) {
-      if (k0 + kdone >= N) {  // reached N inside the block
+    const bool was = act;
+    int jesc = -1;
+    PlainLane L{x, y, r2, i2, cx, cy, live, act, jesc};
+    plain_block<0, K, FMA2>(L, c, pix, k0);
+    if (was) {
+      // the reference tests k >= N before each update: an escape at k0 + j
+      // counts only below N, and a sample that reaches N in the block ends there
+      const int ran = jesc >= 0 ? jesc + 1 : K;
+      if (k0 + ran > N || (jesc < 0 && k0 + K == N)) {
         c.out[pix] = EscapeValue{N, 0.0f};
-        executed += (unsigned long long)(N - k0);
         act = false;
-      } else {
-        executed += (unsigned long long)kdone;
-        k = k0 + kdone;
       }
+      executed += (unsigned long long)min(ran, N - k0);
+      k = k0 + K;
     }
   }
   const int idx[1] = {C_EXECUTED};
@@ -1624,6 +1632,15 @@ class EngineImpl : public Engine {
     NM_HIP(hipGetDeviceProperties(&prop, device));
     num_cu = prop.multiProcessorCount;
     NM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    {  // log2_tab's table: r_i = nearest(1 / c_i), l_i = -log2 r_i in long double
+      double2 tab[kLog2Tab];
+      for (int i = 0; i < kLog2Tab; i++) {
+        const long double ci = 1.0L + ((long double)i + 0.5L) / kLog2Tab;
+        const double r = (double)(1.0L / ci);
+        tab[i] = make_double2(r, (double)(-log2l((long double)r)));
+      }
+      NM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(d_log2tab), tab, sizeof tab));
+    }
     NM_HIP(hipMalloc((void**)&d_totals, sizeof(unsigned long long) * C_COUNT));
     NM_HIP(hipMalloc((void**)&d_part, sizeof(unsigned long long) * kPartSlots * C_COUNT));
     NM_HIP(hipMalloc((void**)&d_nactive, 256));
