@@ -143,7 +143,7 @@ int nm_probe_candidates(const nm_mandelbrot* h);
 int nm_get_reference(const nm_mandelbrot* h, double* X, double* A, double* B, double* C);
 /* render every row with explicit engine settings and `repeats` timed launches;
  * copy_back = 0 leaves the grid on the device (benchmarks).  block_k: 0 for the
- * default (plain 32; series FP64 64, FP32X 8), else plain 8/16/32/64, series
+ * default (plain 64; series FP64 64, FP32X 8), else plain 8/16/32/64, series
  * FP64 8/12/16/32/64, FP32X 8/12/16/32; other values give NM_ERR_ARG. */
 int nm_render(nm_mandelbrot* h, int block_k, int waves_per_simd, int repeats, int copy_back,
               nm_render_stats* stats);

@@ -112,7 +112,7 @@ struct Common {
   EscapeValue* out;
   double* r2;  // |z|^2 at the escape of iterated samples (per sample)
   double4* state;  // per sample: plain (x, y, cx, cy); series: (dr, di) as double2[]
-  int2* state_i;   // per sample: plain (k, -); series (it, m)
+  int2* state_i;   // per sample, series path: (it, m)
   uint8_t* flag;
   uint32_t* tile_count;
   uint32_t* active;
@@ -126,6 +126,7 @@ struct Common {
 struct PlainArgs {
   Common c;
   int pre_iter;  // updates tried in the start kernel before a sample is handed on
+  int k_start;   // updates every handed-on sample has done: pre_iter rounded up to the start kernel's blocks of 4
   const double* cre;
   const double* crelo;
   const double* cim;
@@ -522,8 +523,9 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
           credited += (unsigned long long)N;
         } else {
           executed += (unsigned long long)kend;
+          // kend == k_start for every lane handed on (the lockstep loop ran to
+          // the first multiple of 4 at or past pre_iter)
           c.state[gid] = make_double4(x, y, cx, cy);
-          c.state_i[gid] = make_int2(kend, 0);
           want = true;
         }
       }
@@ -670,6 +672,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint8_t* flag, const
 // lane holds one entry of three chunks: CUR (decoded: handed to lanes that need
 // work through cross-lane shuffles), NEXT (its decode loads in flight) and the
 // chunk after (its sample-id load in flight), so a refill reads registers only.
+// entries of consecutive chunk `chunk` (0 for kNoChunk)
 __device__ __forceinline__ uint32_t chunk_items(uint32_t chunk, uint32_t total) {
   uint64_t b = (uint64_t)chunk * kChunk;
   return b >= total ? 0u : (uint32_t)min((uint64_t)kChunk, (uint64_t)total - b);
@@ -679,6 +682,81 @@ __device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t ch
   uint32_t n = chunk_items(chunk, total);
   return (uint32_t)lane_id() < n ? active[chunk * kChunk + lane_id()] : 0u;
 }
+
+// Chunk claims: persistent waves take chunks of the active list dynamically.
+// The chunks are split into kQueues equal ranges with one claim counter each
+// (each counter on its own 256-byte line; zeroed with the partial counters at
+// the start of a pass).  A wave claims from its home range (its workgroup's
+// index mod kQueues: consecutive workgroups land on different XCDs) and, once
+// that is empty, from the others in turn.  Waves on a SIMD do not progress at
+// one rate (the issue arbiter prefers older waves), so with a static chunk list
+// per wave the waves finished anywhere between 30% and 100% of the kernel's
+// time and the last ones ran nearly alone on their SIMDs; claiming lets fast
+// waves take more.  One atomic per 64 samples; counters sharing one cache line
+// had serialised the claims (~100 ns each).
+constexpr int kQueues = 8;
+constexpr int kQueueStride = 64;  // u32 between counters: 256 bytes
+constexpr int kPartWords = kPartSlots * C_COUNT + kQueues * kQueueStride / 2;  // u64 words of d_part
+constexpr uint32_t kNoChunk = 0xffffffffu;
+struct ChunkClaim {
+  uint32_t* ctr;  // kQueues claim counters, kQueueStride apart
+  uint32_t nch;   // chunks
+  uint32_t home;  // wave-uniform: the range claimed from
+  uint32_t done;  // wave-uniform: ranges found empty
+  __device__ void init(unsigned long long* part, uint32_t chunks) {
+    ctr = (uint32_t*)(part + (size_t)kPartSlots * C_COUNT);
+    nch = chunks;
+    home = blockIdx.x & (kQueues - 1);
+    done = 0;
+  }
+  // the next chunk, or kNoChunk when every range is empty
+  __device__ uint32_t next() {
+    for (int t = 0; t < kQueues; t++) {
+      const uint32_t h = (home + (uint32_t)t) & (kQueues - 1);
+      if (done & (1u << h)) continue;
+      const uint32_t lo = (uint32_t)((uint64_t)nch * h / kQueues);
+      const uint32_t hi = (uint32_t)((uint64_t)nch * (h + 1) / kQueues);
+      uint32_t v = 0;
+      if (lane_id() == 0) v = atomicAdd(&ctr[h * kQueueStride], 1u);
+      v = __builtin_amdgcn_readfirstlane(v);
+      if (v < hi - lo) {
+        home = h;
+        return lo + v;
+      }
+      done |= 1u << h;
+    }
+    return kNoChunk;
+  }
+};
+
+// The plain iterate kernel's chunks interleave the active list: lane l of chunk
+// c takes entry 4 ((l / 4) nch + c) + l % 4 (16 runs of 4 consecutive entries,
+// nch chunks apart), so every chunk is a systematic sample of the whole list and
+// the per-sample work (from a few updates to N) averages out within a chunk.
+// Runs of 4 adjacent entries keep the 32-byte state loads in 128-byte pieces.
+struct ChunkMap {
+  uint32_t n;    // entries of the active list
+  uint32_t nch;  // chunks: ceil(n / 64)
+  __device__ void init(uint32_t total) {
+    n = total;
+    nch = (total + kChunk - 1) / kChunk;
+  }
+  // below 4 (16 nch) <= n + 63 < 2^32 for a real chunk (prepare_common caps
+  // the sample count); kNoChunk is tested first
+  __device__ uint32_t pos(uint32_t c) const {
+    const uint32_t l = (uint32_t)lane_id();
+    return 4u * ((l >> 2) * nch + c) + (l & 3u);
+  }
+  // the chunk's entries are the lanes below the returned count (pos grows with
+  // the lane)
+  __device__ uint32_t items(uint32_t c) const {
+    const bool ok = c < nch && pos(c) < n;
+    return (uint32_t)__popcll(__ballot(ok));
+  }
+  __device__ uint32_t pix(const uint32_t* active, uint32_t c, uint32_t cnt) const {
+    return (uint32_t)lane_id() < cnt ? active[pos(c)] : 0u;
+  }
+};
 
 // A lane's iteration state inside a block of the plain iterate kernel; `live`
 // is the wave's scalar mask of lanes still iterating, `act` the lane's own flag.
@@ -719,21 +797,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     PlainArgs a) {
   const Common& c = a.c;
   const uint32_t total = *c.nactive;
-  const uint32_t W = gridDim.x * (blockDim.x >> 6);
-  uint32_t ccur = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  uint32_t ncur = chunk_items(ccur, total), nnext = chunk_items(ccur + W, total);
-  uint32_t cur_pix = load_pix(c.active, ccur, total), nxt_pix = load_pix(c.active, ccur + W, total);
-  uint32_t far_pix = load_pix(c.active, ccur + 2 * W, total);
+  ChunkMap cm;
+  cm.init(total);
+  ChunkClaim q;
+  q.init(c.part, cm.nch);
+  const uint32_t c0 = q.next(), c1 = q.next(), c2 = q.next();
+  uint32_t ncur = cm.items(c0), nnext = cm.items(c1), nfar = cm.items(c2);
+  uint32_t cur_pix = cm.pix(c.active, c0, ncur), nxt_pix = cm.pix(c.active, c1, nnext);
+  uint32_t far_pix = cm.pix(c.active, c2, nfar);
   double4 cur_s = make_double4(0.0, 0.0, 0.0, 0.0), nxt_s = cur_s;
-  int cur_k = 0, nxt_k = 0;
-  if ((uint32_t)lane_id() < ncur) {
-    cur_s = c.state[cur_pix];
-    cur_k = c.state_i[cur_pix].x;
-  }
-  if ((uint32_t)lane_id() < nnext) {
-    nxt_s = c.state[nxt_pix];
-    nxt_k = c.state_i[nxt_pix].x;
-  }
+  if ((uint32_t)lane_id() < ncur) cur_s = c.state[cur_pix];
+  if ((uint32_t)lane_id() < nnext) nxt_s = c.state[nxt_pix];
   uint32_t pos = 0;
   bool act = false;
   uint32_t pix = 0;
@@ -750,32 +824,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
       uint32_t g_pix = __shfl(cur_pix, src);
       double g0 = __shfl(cur_s.x, src), g1 = __shfl(cur_s.y, src), g2 = __shfl(cur_s.z, src),
              g3 = __shfl(cur_s.w, src);
-      int g4 = __shfl(cur_k, src);
       if (!act && rank < take) {
         pix = g_pix;
         x = g0;
         y = g1;
         cx = g2;
         cy = g3;
-        k = g4;
+        k = a.k_start;
         r2 = x * x;
         i2 = y * y;
         act = true;
       }
       pos += take;
       if (pos == ncur) {  // rotate: NEXT becomes CUR, the far chunk is decoded
-        ccur += W;
         ncur = nnext;
         cur_pix = nxt_pix;
         cur_s = nxt_s;
-        cur_k = nxt_k;
-        nnext = chunk_items(ccur + W, total);
+        nnext = nfar;
         nxt_pix = far_pix;
-        if ((uint32_t)lane_id() < nnext) {
-          nxt_s = c.state[nxt_pix];
-          nxt_k = c.state_i[nxt_pix].x;
-        }
-        far_pix = load_pix(c.active, ccur + 2 * W, total);
+        if ((uint32_t)lane_id() < nnext) nxt_s = c.state[nxt_pix];
+        const uint32_t cf = q.next();
+        nfar = cm.items(cf);
+        far_pix = cm.pix(c.active, cf, nfar);
         pos = 0;
       }
       need = __ballot(!act);
@@ -1304,7 +1374,6 @@ __global__ __launch_bounds__(kSeriesIterThreads) __attribute__((amdgpu_waves_per
 void k_series_iterate(SeriesArgs a) {
   const Common& c = a.c;
   const uint32_t total = *c.nactive;
-  const uint32_t W = gridDim.x * (blockDim.x >> 6);
   uint32_t executed = 0, rebased = 0;  // per lane (< 2^32 steps per lane per launch)
   const int N = c.N, mend = a.mend;
   // the orbit's tail and head windows (zlds[t] = Zd[tail_lo + t],
@@ -1320,9 +1389,11 @@ void k_series_iterate(SeriesArgs a) {
   }
   __syncthreads();
   const double2* zhead = zlds + kLdsTail;  // rebased lanes read the orbit from its start
-  uint32_t ccur = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  uint32_t ncur = chunk_items(ccur, total), nnext = chunk_items(ccur + W, total);
-  uint32_t cur_pix = load_pix(c.active, ccur, total), nxt_pix = load_pix(c.active, ccur + W, total);
+  ChunkClaim q;
+  q.init(c.part, (total + kChunk - 1) / kChunk);
+  const uint32_t c0 = q.next(), c1 = q.next();
+  uint32_t ncur = chunk_items(c0, total), nnext = chunk_items(c1, total);
+  uint32_t cur_pix = load_pix(c.active, c0, total), nxt_pix = load_pix(c.active, c1, total);
   const double2* dstate = (const double2*)c.state;  // the series start's deltas
   double2 cur_s = make_double2(0.0, 0.0), cur_e = cur_s;
   int2 cur_i = make_int2(0, 0);
@@ -1367,7 +1438,6 @@ void k_series_iterate(SeriesArgs a) {
       }
       pos += take;
       if (pos == ncur) {  // rotate: NEXT becomes CUR, its states loaded now
-        ccur += W;
         ncur = nnext;
         cur_pix = nxt_pix;
         if ((uint32_t)lane_id() < ncur) {
@@ -1375,8 +1445,9 @@ void k_series_iterate(SeriesArgs a) {
           cur_i = c.state_i[cur_pix];
           cur_e = sample_eps(a, cur_pix);
         }
-        nnext = chunk_items(ccur + W, total);
-        nxt_pix = load_pix(c.active, ccur + W, total);
+        const uint32_t cn = q.next();
+        nnext = chunk_items(cn, total);
+        nxt_pix = load_pix(c.active, cn, total);
         pos = 0;
       }
       need = __ballot(!act);
@@ -1642,7 +1713,7 @@ class EngineImpl : public Engine {
       NM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(d_log2tab), tab, sizeof tab));
     }
     NM_HIP(hipMalloc((void**)&d_totals, sizeof(unsigned long long) * C_COUNT));
-    NM_HIP(hipMalloc((void**)&d_part, sizeof(unsigned long long) * kPartSlots * C_COUNT));
+    NM_HIP(hipMalloc((void**)&d_part, sizeof(unsigned long long) * kPartWords));  // slots + claim counters
     NM_HIP(hipMalloc((void**)&d_nactive, 256));
     return NM_OK;
   }
@@ -1742,6 +1813,7 @@ class EngineImpl : public Engine {
 
   // buffers and the common kernel arguments for `total` samples
   int prepare_common(Common& c, uint64_t total, const RenderOpts& opt) {
+    // sample ids and the plain iterate kernel's chunk positions (ChunkMap) stay below 2^32
     if (total == 0 || total >= 0xFFFFF000ull) return NM_ERR_RANGE;
     uint64_t ntiles = (total + kTile - 1) / kTile;
     int st;
@@ -1783,7 +1855,7 @@ class EngineImpl : public Engine {
     // waves = persistent waves per SIMD: 4 per CU for each 256 threads
     const int ti = iter_threads<Args>();
     const int giter = std::max(1, std::min(num_cu * waves * 256 / ti, kSlotFinal - kSlotIter));
-    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartSlots * C_COUNT);
+    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartWords);
     NM_HIP(hipGetLastError());
     if (evs) NM_HIP(hipEventRecord(evs[0], stream));
     hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args);
@@ -2005,11 +2077,12 @@ class EngineImpl : public Engine {
       tables_uploaded(opt);
     }
     a.pre_iter = opt.pre_iter > 0 ? opt.pre_iter : kPreIterDefault;
+    a.k_start = (std::min(a.pre_iter, opt.N) + 3) / 4 * 4;
     a.cre = t;
     a.crelo = t + ncol;
     a.cim = t + 2 * ncol;
     a.cimlo = t + 2 * ncol + nrow;
-    const int K = opt.block_k > 0 ? opt.block_k : 32;
+    const int K = opt.block_k > 0 ? opt.block_k : 64;
     const int waves = opt.waves_per_simd > 0 ? opt.waves_per_simd : 8;
     if (stats) {
       stats->block_k = K;
