@@ -99,6 +99,17 @@ int nm_compute_row(nm_mandelbrot* h, int r);  /* computeRow(int), mandelbrot.cpp
 /* computeRow for every row, one launch (viewer.cpp:166-171 loops computeRow) */
 int nm_compute_all(nm_mandelbrot* h);
 
+/* No counterpart in the reference: nm_compute_all for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Renders h's view on the plain path
+ * (useHardware() true; deep views give NM_ERR_STATE) into out (n >= rows x cols
+ * escape values, row-major) with every device buffer in the library's static
+ * device scratch array: no hipMalloc/hipFree, no stream or event created.  At
+ * most 2^19 samples and 8192 rows or columns (NM_ERR_RANGE beyond).  Results
+ * equal nm_compute_all's.  All calls on one GPU share the scratch array, which
+ * keeps the last call's contents: calls that overlap on one GPU race. */
+int nm_compute_all_noalloc(nm_mandelbrot* h, nm_escape_value* out, size_t n);
+
 int nm_at(nm_mandelbrot* h, int r, int c, nm_escape_value* out);         /* at(r,c), mandelbrot.cpp:318 */
 int nm_at_avg(nm_mandelbrot* h, int r, int c, int sc, nm_escape_value* out); /* at(r,c,sc), :320-332 */
 /* the whole grid, row-major (RenderGrid::values, grid.h:18) */

@@ -204,6 +204,7 @@ _SIGS = [
     ("nm_precompute", C.c_int, [_H]),
     ("nm_compute_row", C.c_int, [_H, C.c_int]),
     ("nm_compute_all", C.c_int, [_H]),
+    ("nm_compute_all_noalloc", C.c_int, [_H, C.c_void_p, C.c_size_t]),
     ("nm_at", C.c_int, [_H, C.c_int, C.c_int, C.POINTER(EscapeValue)]),
     ("nm_at_avg", C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.POINTER(EscapeValue)]),
     ("nm_get_grid", C.c_int, [_H, C.c_void_p, C.c_size_t]),
@@ -401,6 +402,15 @@ class Mandelbrot:
 
     def computeAll(self):
         _check(lib().nm_compute_all(self._h), "nm_compute_all")
+
+    def compute_all_noalloc(self, out=None):
+        """nm_compute_all_noalloc: the plain-path grid into `out` (an ESCAPE_DTYPE
+        array of rows x cols, allocated here when None) without device allocation."""
+        if out is None:
+            out = np.zeros((self.rows(), self.cols()), dtype=ESCAPE_DTYPE)
+        assert out.dtype == ESCAPE_DTYPE and out.flags.c_contiguous
+        _check(lib().nm_compute_all_noalloc(self._h, out.ctypes.data, out.size), "nm_compute_all_noalloc")
+        return out
 
     def at(self, r, c, sc=None):
         e = EscapeValue()

@@ -178,6 +178,15 @@ int nm_compute_all(nm_mandelbrot* h) {
   return guard([&] { return engine_status(h->m.computeAll()); });
 }
 
+int nm_compute_all_noalloc(nm_mandelbrot* h, nm_escape_value* out, size_t n) {
+  if (!h || !out) return NM_ERR_ARG;
+  return guard([&] {
+    int st = h->m.computeAllNoAlloc((nm::EscapeValue*)out, n);
+    if (st != NM_OK) g_err = h->m.noalloc_err_.empty() ? "nm_compute_all_noalloc: argument, view or size" : h->m.noalloc_err_;
+    return st;
+  });
+}
+
 int nm_at(nm_mandelbrot* h, int r, int c, nm_escape_value* out) {
   if (!h || !out || r < 0 || c < 0 || r >= h->m.rows() || c >= h->m.cols()) return NM_ERR_ARG;
   const nm::EscapeValue& e = h->m.at(r, c);

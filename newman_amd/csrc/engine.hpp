@@ -3,6 +3,7 @@
 #pragma once
 #include <cstdint>
 #include <cstddef>
+#include <string>
 #include <vector>
 
 #include "../../include/newman_amd.h"
@@ -108,6 +109,22 @@ struct RenderStats {
   int fma2 = 0;              // plain path: the folded update ran (plain_update)
   int block_k = 0, waves_per_simd = 0;
 };
+
+// A plain-path frame as tables the caller keeps (render_hw_noalloc).
+struct HwFrameView {
+  int nr = 0, nc = 0;
+  int card_mode = CARD_TEST_DD;
+  bool fma_ok = false;
+  const double *cre = nullptr, *crelo = nullptr;  // nc each
+  const double *cim = nullptr, *cimlo = nullptr;  // nr each
+};
+
+// The plain path of one frame (at most 2^19 samples, rows and columns at most
+// 8192) with every device buffer in a file-scope device array: no device
+// allocation, no stream or event created (the null stream), no engine buffer
+// touched.  Results in host_out (nr x nc, row-major).  Calls overlapping on one
+// device share that array and race.
+int render_hw_noalloc(const HwFrameView& f, int N, double bailout, EscapeValue* host_out, std::string& err);
 
 // Device engine: one per process per device.  All calls are synchronous.
 class Engine {

@@ -1639,6 +1639,62 @@ constexpr int iter_threads() { return kThreads; }
 template <>
 constexpr int iter_threads<SeriesArgs>() { return kSeriesIterThreads; }
 
+// workgroups of the tile kernels (start, scatter, finalize, emit)
+static int tile_grid(uint64_t total, int num_cu) {
+  const uint64_t ntiles = (total + kTile - 1) / kTile;
+  const uint64_t cap = std::min<uint64_t>((uint64_t)num_cu * 16, (uint64_t)(kSlotIter - kSlotStart));
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, cap));
+}
+
+// The kernels of one pass on `stream`: zero the counters (part: kPartWords
+// words), start, scan, scatter, iterate, finalize (and emit for tiled sample
+// orders).  evs != nullptr records an event before each stage (eager timing);
+// under stream capture it is null.  Returns the first launch error.
+template <class Args, class StartFn, class IterFn>
+static hipError_t launch_pass_on(hipStream_t stream, int num_cu, unsigned long long* part, uint32_t* nactive,
+                                 Args& args, StartFn start, IterFn iter, int waves, EscapeValue* emit_to,
+                                 hipEvent_t* evs) {
+  const Common& c = args.c;
+  const uint32_t ntiles = (uint32_t)((c.total + kTile - 1) / kTile);
+  const int gtile = tile_grid(c.total, num_cu);
+  // waves = persistent waves per SIMD: 4 per CU for each 256 threads
+  const int ti = iter_threads<Args>();
+  const int giter = std::max(1, std::min(num_cu * waves * 256 / ti, kSlotFinal - kSlotIter));
+  hipError_t e;
+#define NM_LAUNCH_OK(expr)                   \
+  do {                                       \
+    expr;                                    \
+    if ((e = hipGetLastError()) != hipSuccess) return e; \
+  } while (0)
+  NM_LAUNCH_OK(hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, part, kPartWords));
+  if (evs && (e = hipEventRecord(evs[0], stream)) != hipSuccess) return e;
+  NM_LAUNCH_OK(hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args));
+  if (evs && (e = hipEventRecord(evs[1], stream)) != hipSuccess) return e;
+  NM_LAUNCH_OK(hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, c.tile_count, ntiles, nactive));
+  NM_LAUNCH_OK(hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint8_t*)c.flag,
+                                  (const uint32_t*)c.tile_count, c.total, c.active));
+  if (evs && (e = hipEventRecord(evs[2], stream)) != hipSuccess) return e;
+  NM_LAUNCH_OK(hipLaunchKernelGGL(iter, dim3(giter), dim3(ti), 0, stream, args));
+  if (evs && (e = hipEventRecord(evs[3], stream)) != hipSuccess) return e;
+  static_assert(kPartSlots - kSlotFinal >= kSlotIter - kSlotStart, "finalize runs on the start grid");
+  NM_LAUNCH_OK(hipLaunchKernelGGL(k_finalize, dim3(gtile), dim3(kThreads), 0, stream, c));
+  if (emit_to) NM_LAUNCH_OK(hipLaunchKernelGGL(k_emit, dim3(gtile), dim3(kThreads), 0, stream, c, emit_to));
+  if (evs && (e = hipEventRecord(evs[4], stream)) != hipSuccess) return e;
+#undef NM_LAUNCH_OK
+  return hipSuccess;
+}
+
+// log2_tab's table: r_i = nearest(1 / c_i), l_i = -log2 r_i in long double
+static hipError_t fill_log2_table() {
+  double2 tab[kLog2Tab];
+  for (int i = 0; i < kLog2Tab; i++) {
+    const long double ci = 1.0L + ((long double)i + 0.5L) / kLog2Tab;
+    const double r = (double)(1.0L / ci);
+    tab[i] = make_double2(r, (double)(-log2l((long double)r)));
+  }
+  return hipMemcpyToSymbol(HIP_SYMBOL(d_log2tab), tab, sizeof tab);
+}
+
 class EngineImpl : public Engine {
  public:
   hipStream_t stream = nullptr;
@@ -1703,15 +1759,7 @@ class EngineImpl : public Engine {
     NM_HIP(hipGetDeviceProperties(&prop, device));
     num_cu = prop.multiProcessorCount;
     NM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    {  // log2_tab's table: r_i = nearest(1 / c_i), l_i = -log2 r_i in long double
-      double2 tab[kLog2Tab];
-      for (int i = 0; i < kLog2Tab; i++) {
-        const long double ci = 1.0L + ((long double)i + 0.5L) / kLog2Tab;
-        const double r = (double)(1.0L / ci);
-        tab[i] = make_double2(r, (double)(-log2l((long double)r)));
-      }
-      NM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(d_log2tab), tab, sizeof tab));
-    }
+    NM_HIP(fill_log2_table());
     NM_HIP(hipMalloc((void**)&d_totals, sizeof(unsigned long long) * C_COUNT));
     NM_HIP(hipMalloc((void**)&d_part, sizeof(unsigned long long) * kPartWords));  // slots + claim counters
     NM_HIP(hipMalloc((void**)&d_nactive, 256));
@@ -1805,12 +1853,6 @@ class EngineImpl : public Engine {
     tab_tile = opt.tile;
   }
 
-  int grid_for_tiles(uint64_t total) const {
-    uint64_t ntiles = (total + kTile - 1) / kTile;
-    const uint64_t cap = std::min<uint64_t>((uint64_t)num_cu * 16, (uint64_t)(kSlotIter - kSlotStart));
-    return (int)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, cap));
-  }
-
   // buffers and the common kernel arguments for `total` samples
   int prepare_common(Common& c, uint64_t total, const RenderOpts& opt) {
     // sample ids and the plain iterate kernel's chunk positions (ChunkMap) stay below 2^32
@@ -1843,41 +1885,9 @@ class EngineImpl : public Engine {
     return NM_OK;
   }
 
-  // The kernels of one pass: zero the counters, start, scan, scatter,
-  // iterate, finalize (and emit for tiled sample orders).  ev != nullptr
-  // records an event before each stage (eager timing); under stream capture
-  // it is null.
   template <class Args, class StartFn, class IterFn>
   int launch_pass(Args& args, StartFn start, IterFn iter, int waves, EscapeValue* emit_to, hipEvent_t* evs) {
-    const Common& c = args.c;
-    const uint32_t ntiles = (uint32_t)((c.total + kTile - 1) / kTile);
-    const int gtile = grid_for_tiles(c.total);
-    // waves = persistent waves per SIMD: 4 per CU for each 256 threads
-    const int ti = iter_threads<Args>();
-    const int giter = std::max(1, std::min(num_cu * waves * 256 / ti, kSlotFinal - kSlotIter));
-    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, d_part, kPartWords);
-    NM_HIP(hipGetLastError());
-    if (evs) NM_HIP(hipEventRecord(evs[0], stream));
-    hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args);
-    NM_HIP(hipGetLastError());
-    if (evs) NM_HIP(hipEventRecord(evs[1], stream));
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, c.tile_count, ntiles, d_nactive);
-    NM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint8_t*)c.flag,
-                       (const uint32_t*)c.tile_count, c.total, c.active);
-    NM_HIP(hipGetLastError());
-    if (evs) NM_HIP(hipEventRecord(evs[2], stream));
-    hipLaunchKernelGGL(iter, dim3(giter), dim3(ti), 0, stream, args);
-    NM_HIP(hipGetLastError());
-    if (evs) NM_HIP(hipEventRecord(evs[3], stream));
-    static_assert(kPartSlots - kSlotFinal >= kSlotIter - kSlotStart, "finalize runs on the start grid");
-    hipLaunchKernelGGL(k_finalize, dim3(gtile), dim3(kThreads), 0, stream, c);
-    NM_HIP(hipGetLastError());
-    if (emit_to) {
-      hipLaunchKernelGGL(k_emit, dim3(gtile), dim3(kThreads), 0, stream, c, emit_to);
-      NM_HIP(hipGetLastError());
-    }
-    if (evs) NM_HIP(hipEventRecord(evs[4], stream));
+    NM_HIP(launch_pass_on(stream, num_cu, d_part, d_nactive, args, start, iter, waves, emit_to, evs));
     return NM_OK;
   }
 
@@ -2422,6 +2432,178 @@ int Engine::instance(Engine** out) {
   }
   *out = g_engine;
   return g_engine_status;
+}
+
+
+// ================================================ allocation-free plain render
+// render_hw_noalloc: the plain path (getIterationsHW, mandelbrot.cpp:231-254) of
+// one frame with every device buffer carved from `scratch`, a file-scope device
+// array: coordinate tables, the frame record, per-sample results, flags,
+// states, |z|^2, the active list, tile counts and the counters.  It calls no
+// hipMalloc/hipFree, creates no stream or event (the null stream) and touches no
+// engine buffer, for callers that cannot allocate device memory.  Each device
+// holds its own copy of the array; its address comes from hipGetSymbolAddress
+// once per device (noalloc_setup) and only that address reaches kernels and
+// copies.  Every call on a device shares the array: calls that overlap on one
+// device race (the C boundary serialises its own calls).
+constexpr uint64_t kNoAllocMaxSamples = 1ull << 19;
+constexpr int kNoAllocMaxSide = 8192;
+constexpr int kNoAllocMaxDevices = 64;
+constexpr size_t na_align(size_t b) { return (b + 255) & ~(size_t)255; }
+// bytes of the layout for n samples on an nr x nc frame (noalloc_layout)
+constexpr size_t na_bytes(uint64_t n, uint64_t nr, uint64_t nc) {
+  return na_align((2 * nc + 2 * nr) * sizeof(double)) + na_align(sizeof(DevFrame)) +
+         na_align(n * sizeof(EscapeValue)) + na_align(n * sizeof(double)) + na_align(n * sizeof(double4)) +
+         na_align(n) + na_align((n + kTile - 1) / kTile * 4) + na_align(n * 4) +
+         na_align((size_t)kPartWords * sizeof(unsigned long long)) + na_align(4);
+}
+constexpr size_t kNoAllocScratchBytes =
+    na_bytes(kNoAllocMaxSamples, kNoAllocMaxSide, kNoAllocMaxSide);  // about 29 MiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocDevice {
+  void* ptr = nullptr;  // device address of this device's scratch
+  int num_cu = 0;
+};
+static std::mutex g_noalloc_mu;
+static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+// once per device, with that device current: the scratch address and the
+// log2 table of the smoothing
+static int noalloc_setup(int dev, NoAllocDevice** out, std::string& err) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    err = "device index beyond the allocation-free path's table";
+    return NM_ERR_RANGE;
+  }
+  NoAllocDevice& d = g_noalloc[dev];
+  if (!d.ptr) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return NM_ERR_HIP;
+    }
+    int cu = 0;
+    e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = fill_log2_table();
+    if (e != hipSuccess) {
+      err = std::string("allocation-free set-up: ") + hipGetErrorString(e);
+      return NM_ERR_HIP;
+    }
+    d.num_cu = cu > 0 ? cu : 256;
+    d.ptr = p;
+  }
+  *out = &d;
+  return NM_OK;
+}
+
+int render_hw_noalloc(const HwFrameView& f, int N, double bailout, EscapeValue* host_out, std::string& err) {
+  if (f.nr <= 0 || f.nc <= 0 || N <= 0 || !(bailout > 0.0) || !host_out || !f.cre || !f.crelo || !f.cim ||
+      !f.cimlo) {
+    err = "bad argument";
+    return NM_ERR_ARG;
+  }
+  const uint64_t n = (uint64_t)f.nr * f.nc;
+  if (n > kNoAllocMaxSamples || f.nr > kNoAllocMaxSide || f.nc > kNoAllocMaxSide) {
+    err = "frame larger than the allocation-free path's scratch (at most 2^19 samples, 8192 rows or columns)";
+    return NM_ERR_RANGE;
+  }
+  int ndev = 0, dev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    err = "no HIP device";
+    return NM_ERR_NODEVICE;
+  }
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+    return NM_ERR_HIP;
+  }
+  NoAllocDevice* d = nullptr;
+  int st = noalloc_setup(dev, &d, err);
+  if (st != NM_OK) return st;
+  // the layout: tables, frame, out, r2, state, flag, tile counts, active,
+  // counters, active count
+  char* b = (char*)d->ptr;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* q = b + off;
+    off += na_align(bytes);
+    return q;
+  };
+  double* tabs = (double*)take((2 * (size_t)f.nc + 2 * (size_t)f.nr) * sizeof(double));
+  DevFrame* frame = (DevFrame*)take(sizeof(DevFrame));
+  EscapeValue* out = (EscapeValue*)take(n * sizeof(EscapeValue));
+  double* r2 = (double*)take(n * sizeof(double));
+  double4* state = (double4*)take(n * sizeof(double4));
+  uint8_t* flag = (uint8_t*)take(n);
+  uint32_t* tiles = (uint32_t*)take((n + kTile - 1) / kTile * 4);
+  uint32_t* active = (uint32_t*)take(n * 4);
+  unsigned long long* part = (unsigned long long*)take((size_t)kPartWords * sizeof(unsigned long long));
+  uint32_t* nactive = (uint32_t*)take(4);
+  if (off > kNoAllocScratchBytes) {  // cannot happen for frames within the limits above
+    err = "allocation-free layout overflow";
+    return NM_ERR_RANGE;
+  }
+  DevFrame df{};
+  df.sample_base = 0;
+  df.nr = (uint32_t)f.nr;
+  df.nc = (uint32_t)f.nc;
+  df.tab_c = 0;
+  df.tab_r = 0;
+  df.card_mode = f.card_mode;
+  df.out_base = 0;
+  df.inv_nc = 1.0 / (double)f.nc;
+  hipStream_t s = nullptr;  // the null stream: nothing is created
+#define NA_HIP(expr)                                               \
+  do {                                                             \
+    hipError_t e_ = (expr);                                        \
+    if (e_ != hipSuccess) {                                        \
+      err = std::string(#expr) + ": " + hipGetErrorString(e_);     \
+      return NM_ERR_HIP;                                           \
+    }                                                              \
+  } while (0)
+  NA_HIP(hipMemcpyAsync(frame, &df, sizeof df, hipMemcpyHostToDevice, s));
+  NA_HIP(hipMemcpyAsync(tabs, f.cre, (size_t)f.nc * 8, hipMemcpyHostToDevice, s));
+  NA_HIP(hipMemcpyAsync(tabs + f.nc, f.crelo, (size_t)f.nc * 8, hipMemcpyHostToDevice, s));
+  NA_HIP(hipMemcpyAsync(tabs + 2 * f.nc, f.cim, (size_t)f.nr * 8, hipMemcpyHostToDevice, s));
+  NA_HIP(hipMemcpyAsync(tabs + 2 * f.nc + f.nr, f.cimlo, (size_t)f.nr * 8, hipMemcpyHostToDevice, s));
+  PlainArgs a{};
+  Common& c = a.c;
+  c.frames = frame;
+  c.nframes = 1;
+  c.frame_size = (uint32_t)n;
+  c.inv_frame = 1.0 / (double)n;
+  c.total = (uint32_t)n;
+  c.out = out;
+  c.r2 = r2;
+  c.state = state;
+  c.state_i = nullptr;  // the plain path keeps no per-sample count
+  c.flag = flag;
+  c.tile_count = tiles;
+  c.active = active;
+  c.nactive = nactive;
+  c.part = part;
+  c.N = N;
+  c.b2 = bailout * bailout;
+  c.b2m = c.b2 * (1.0 - 1.0 / 1099511627776.0);
+  c.log_b = std::log(bailout);
+  c.smooth_k = bailout >= 2.0 ? 0.5 * std::log(2.0) / c.log_b : 0.0;
+  a.pre_iter = kPreIterDefault;
+  a.k_start = (std::min(a.pre_iter, N) + 3) / 4 * 4;
+  a.cre = tabs;
+  a.crelo = tabs + f.nc;
+  a.cim = tabs + 2 * f.nc;
+  a.cimlo = tabs + 2 * f.nc + f.nr;
+  const bool fma2 = f.fma_ok && bailout <= 0x1p400;
+  NA_HIP(fma2 ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<true>, k_plain_iterate<64, true>, 8,
+                               nullptr, nullptr)
+              : launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<false>, k_plain_iterate<64, false>, 8,
+                               nullptr, nullptr));
+  NA_HIP(hipMemcpyAsync(host_out, out, n * sizeof(EscapeValue), hipMemcpyDeviceToHost, s));
+  NA_HIP(hipStreamSynchronize(s));
+#undef NA_HIP
+  return NM_OK;
 }
 
 }  // namespace nm

@@ -3,6 +3,7 @@
 #include "mandelbrot.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -636,33 +637,65 @@ static int classify_frame(const HPComplex& center, const HPComplex& sz, int nr, 
 // (descend(Y0) in getIterationsHW, mandelbrot.cpp:234) plus the double-double
 // remainder used by the device cardioid test.
 int Mandelbrot::buildHwFrame(HwFrameHost& f, const HPComplex& cen, const HPComplex& s) {
-  const int L = hp_limbs_for_bits(hp_default_bits());
   f.nr = rows();
   f.nc = cols();
   f.cre.resize(f.nc); f.crelo.resize(f.nc);
   f.cim.resize(f.nr); f.cimlo.resize(f.nr);
-  hpf p(L), rem(L), td(2);
-  for (int c = 0; c < f.nc; c++) {
-    pixel_re(p, cen.re, s.re, (long)(c - cols() / 2));
-    double t = p.get_d();
-    hp_set_d(td, t);
-    hp_sub(rem, p, td);
-    f.cre[c] = t;
-    f.crelo[c] = get_d_nearest(rem);
-  }
-  for (int r = 0; r < f.nr; r++) {
-    pixel_re(p, cen.im, s.im, (long)(rows() / 2 - r - 1));
-    double t = p.get_d();
-    hp_set_d(td, t);
-    hp_sub(rem, p, td);
-    f.cim[r] = t;
-    f.cimlo[r] = get_d_nearest(rem);
-  }
-  int m = classify_frame(cen, s, f.nr, f.nc);
-  f.card_mode = m < 0 ? CARD_TEST_DD : m;
-  auto clear = [](double v) { return v == 0.0 || std::fabs(v) >= 0x1p-200; };
-  f.fma_ok = std::all_of(f.cre.begin(), f.cre.end(), clear) && std::all_of(f.cim.begin(), f.cim.end(), clear);
+  fillHwTables(cen, s, f.cre.data(), f.crelo.data(), f.cim.data(), f.cimlo.data(), &f.card_mode, &f.fma_ok);
   return NM_OK;
+}
+
+// The tables of buildHwFrame into caller-owned arrays (cols() and rows()
+// entries); allocates nothing.
+void Mandelbrot::fillHwTables(const HPComplex& cen, const HPComplex& s, double* cre, double* crelo, double* cim,
+                              double* cimlo, int* card_mode, bool* fma_ok) const {
+  const int L = hp_limbs_for_bits(hp_default_bits());
+  const int nr = rows(), nc = cols();
+  hpf p(L), rem(L), td(2);
+  for (int c = 0; c < nc; c++) {
+    pixel_re(p, cen.re, s.re, (long)(c - nc / 2));
+    double t = p.get_d();
+    hp_set_d(td, t);
+    hp_sub(rem, p, td);
+    cre[c] = t;
+    crelo[c] = get_d_nearest(rem);
+  }
+  for (int r = 0; r < nr; r++) {
+    pixel_re(p, cen.im, s.im, (long)(nr / 2 - r - 1));
+    double t = p.get_d();
+    hp_set_d(td, t);
+    hp_sub(rem, p, td);
+    cim[r] = t;
+    cimlo[r] = get_d_nearest(rem);
+  }
+  int m = classify_frame(cen, s, nr, nc);
+  *card_mode = m < 0 ? CARD_TEST_DD : m;
+  auto clear = [](double v) { return v == 0.0 || std::fabs(v) >= 0x1p-200; };
+  *fma_ok = std::all_of(cre, cre + nc, clear) && std::all_of(cim, cim + nr, clear);
+}
+
+// computeAll without device allocation (render_hw_noalloc): the plain path
+// only; the coordinate tables go through a static host buffer.
+int Mandelbrot::computeAllNoAlloc(EscapeValue* out, size_t n) {
+  if (!out || N <= 0) return NM_ERR_ARG;
+  if (!useHardware()) return NM_ERR_STATE;  // deep views need the device reference orbit
+  const int nr = rows(), nc = cols();
+  if ((uint64_t)nr * nc > n) return NM_ERR_ARG;
+  constexpr int kSide = 8192;
+  if (nr > kSide || nc > kSide || (uint64_t)nr * nc > (1ull << 19)) return NM_ERR_RANGE;
+  static std::mutex mu;
+  static double tab[4 * kSide];
+  std::lock_guard<std::mutex> lk(mu);
+  HwFrameView v;
+  v.nr = nr;
+  v.nc = nc;
+  double *cre = tab, *crelo = tab + kSide, *cim = tab + 2 * kSide, *cimlo = tab + 3 * kSide;
+  fillHwTables(center, sz, cre, crelo, cim, cimlo, &v.card_mode, &v.fma_ok);
+  v.cre = cre;
+  v.crelo = crelo;
+  v.cim = cim;
+  v.cimlo = cimlo;
+  return render_hw_noalloc(v, N, bailout, out, noalloc_err_);
 }
 
 // Per-column / per-row epsilon = descend(Y0 - X[0]) (mandelbrot.cpp:155-159).
