@@ -8,8 +8,9 @@
  *
  * Conventions: plain C types only; every function returns an int status (0 = ok,
  * < 0 = error, see NM_ERR_*) and never throws; all calls are synchronous; output
- * buffers are owned by the caller.  A handle is not thread-safe: use one handle per
- * thread.  HP values cross the boundary as decimal strings ("-0.75",
+ * buffers are owned by the caller.  Calls are serialised by a process-wide lock,
+ * so different handles may be used from different threads; one handle must not be
+ * used from two threads at once.  HP values cross the boundary as decimal strings ("-0.75",
  * "0.123e-30"), as the reference's view files carry them (viewer.cpp:12-22).
  */
 #ifndef NEWMAN_AMD_H
@@ -107,7 +108,8 @@ int nm_compute_all(nm_mandelbrot* h);
  * device scratch array: no hipMalloc/hipFree, no stream or event created.  At
  * most 2^19 samples and 8192 rows or columns (NM_ERR_RANGE beyond).  Results
  * equal nm_compute_all's.  All calls on one GPU share the scratch array, which
- * keeps the last call's contents: calls that overlap on one GPU race. */
+ * keeps the last call's contents; the library's process-wide lock serialises
+ * them (calls that overlapped on one GPU would race). */
 int nm_compute_all_noalloc(nm_mandelbrot* h, nm_escape_value* out, size_t n);
 
 int nm_at(nm_mandelbrot* h, int r, int c, nm_escape_value* out);         /* at(r,c), mandelbrot.cpp:318 */

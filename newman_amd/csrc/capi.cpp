@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -28,8 +29,15 @@ static int put_str(const std::string& s, char* out, size_t cap) {
   return NM_OK;
 }
 
+// Every call that reaches the device engine runs under this process-wide lock:
+// the engine (one per process) keeps one stream and grow-only buffers, so two
+// threads rendering through different handles would otherwise overwrite each
+// other's tables and results, or free a buffer the other's kernels still use.
+static std::recursive_mutex g_call_mu;
+
 template <class F>
 static int guard(F&& f) {
+  std::lock_guard<std::recursive_mutex> lk(g_call_mu);
   try {
     return f();
   } catch (const std::bad_alloc&) {
