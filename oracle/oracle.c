@@ -158,6 +158,19 @@ int orc_set_size(orc_view* v, const char* re, const char* im) {
   return a || b ? -1 : 0;
 }
 
+/* sz = width / cols (square samples) at the view's current precision, without
+ * setPrecision: the frame sizes of a zoom-video batch, which share one reference
+ * and the deepest frame's precision (the batch builder parses each frame's width
+ * at that precision and divides by the column count) */
+int orc_set_width(orc_view* v, const char* width) {
+  ohp w;
+  w.n = v->szre.n;
+  if (ohp_set_str(&w, width)) return -1;
+  ohp_div_ui(&v->szre, &w, (unsigned long)v->nc);
+  ohp_copy(&v->szim, &v->szre);
+  return 0;
+}
+
 int orc_set_view(orc_view* v, int N, const char* cre, const char* cim, const char* width) {
   /* parse the width generously, derive the precision the view needs, then parse
    * the centre at that precision */

@@ -42,6 +42,7 @@ void orc_set_tol(orc_view* v, double tol);
 void orc_set_bailout(orc_view* v, double bailout);
 int orc_set_center(orc_view* v, const char* re, const char* im); /* at the current precision */
 int orc_set_size(orc_view* v, const char* re, const char* im);
+int orc_set_width(orc_view* v, const char* width); /* sz = width / cols at the current precision */
 /* view by width: parses at the precision the target needs, sz.re = sz.im = width/nc,
  * then setPrecision */
 int orc_set_view(orc_view* v, int N, const char* cre, const char* cim, const char* width);

@@ -25,6 +25,7 @@ _SIGS = [
     ("orc_set_bailout", None, [_V, C.c_double]),
     ("orc_set_center", C.c_int, [_V, C.c_char_p, C.c_char_p]),
     ("orc_set_size", C.c_int, [_V, C.c_char_p, C.c_char_p]),
+    ("orc_set_width", C.c_int, [_V, C.c_char_p]),
     ("orc_set_view", C.c_int, [_V, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p]),
     ("orc_set_precision", None, [_V]),
     ("orc_precision_bits", C.c_long, [_V]),
@@ -134,6 +135,10 @@ class OView:
 
     def set_center(self, re, im):
         assert self.L.orc_set_center(self.v, _b(re), _b(im)) == 0
+
+    def set_width(self, width):
+        """sz = width / cols at the current precision (a batch frame's size)"""
+        assert self.L.orc_set_width(self.v, _b(width)) == 0
 
     def set_size(self, re, im):
         assert self.L.orc_set_size(self.v, _b(re), _b(im)) == 0
