@@ -301,3 +301,226 @@ def test_tile_probe_is_the_worst_sample_of_its_tile():
                 for c in range(tc * 16, min(56, tc * 16 + 16), 5):
                     u = o.sa_start(r, c)[0]
                     assert u == 0 or (ua[tr, tc] != 0 and u >= ua[tr, tc])
+
+
+# ------------------------------------- the series skip, pinned independently
+def _orbit_mp(c, n):
+    x = [c]
+    for _ in range(n):
+        x.append(x[-1] * x[-1] + c)
+    return x
+
+
+def test_series_B_C_as_sums_over_the_orbit_at_a_nonzero_reference():
+    """computeSeries (mandelbrot.cpp:112-131) written as the solution of its linear
+    recurrences: with G(k, n) = prod_{j=k}^{n-1} 2 X_j,
+      A_n = sum_{k=0}^{n} G(k, n),  B_n = sum_{k=1}^{n} G(k, n) A_k^2,
+      C_n = sum_{k=1}^{n} G(k, n) 2 A_k B_k,
+    evaluated from an mpmath orbit at 300 bits; A_n also equals the Taylor
+    coefficient of the orbit in c, taken from power-series iteration (truncated
+    at degree 3), and B_n, C_n differ from the Taylor b_n, c_n (the reference's
+    new-index quirk)."""
+    mp.mp.prec = 300
+    c = mp.mpc("-0.7391", "0.1837")
+    n_max = 6
+    o = O.OView(8, 8)
+    o.set_view(40, "-0.7391", "0.1837", "1e-20")
+    o.precompute_at("-0.7391", "0.1837")
+    Xt, Xn, A, B, Cc = o.series()
+    X = _orbit_mp(c, n_max)
+
+    def G(k, n):
+        p = mp.mpc(1)
+        for j in range(k, n):
+            p *= 2 * X[j]
+        return p
+
+    # Taylor coefficients of Y_n(e) = orbit of c + e: Y_0 = c + e, Y_{n+1} = Y_n^2 + c + e
+    poly = [c, mp.mpc(1), mp.mpc(0), mp.mpc(0)]
+    taylor = [poly]
+    for _ in range(n_max):
+        p = taylor[-1]
+        sq = [sum(p[i] * p[k - i] for i in range(k + 1)) for k in range(4)]
+        taylor.append([sq[0] + c, sq[1] + 1, sq[2], sq[3]])
+    Am = [sum(G(k, n) for k in range(n + 1)) for n in range(n_max + 1)]
+    Bm = [sum((G(k, n) * Am[k] ** 2 for k in range(1, n + 1)), mp.mpc(0)) for n in range(n_max + 1)]
+    Cm = [sum((G(k, n) * 2 * Am[k] * Bm[k] for k in range(1, n + 1)), mp.mpc(0)) for n in range(n_max + 1)]
+
+    def close(arr, n, want, rel=1e-14):
+        got = complex(arr[2 * n], arr[2 * n + 1])
+        return abs(got - complex(want)) <= rel * max(abs(complex(want)), 1e-300)
+
+    for n in range(1, n_max + 1):
+        assert abs(Am[n] - taylor[n][1]) <= mp.mpf(10) ** -60 * abs(Am[n])  # A is the derivative
+        assert close(A, n, Am[n]) and close(B, n, Bm[n]) and close(Cc, n, Cm[n]), n
+        assert not close(B, n, taylor[n][2], 1e-3) and not close(Cc, n, taylor[n][3], 1e-3), n
+
+
+def _direct_count(cr, ci, N, bailout=1024.0):
+    """z_0 = c, z_{k+1} = z_k^2 + c at the current mpmath precision; the first k
+    whose descend()ed |z_k|^2 exceeds bailout^2 (the series path's count)."""
+    z = mp.mpc(cr, ci)
+    c = z
+    for k in range(N):
+        zz = complex(z)
+        if zz.real * zz.real + zz.imag * zz.imag > bailout * bailout:
+            return k
+        z = z * z + c
+    return N
+
+
+# c* on the real axis where the orbit's element X[99] (X[0] = c) equals the
+# bailout 1024: c slightly below escapes at 100, slightly above at 99 (found by
+# bisection of the escape index at 400 bits)
+_C_STAR = "0.25100340768234008000104192638925597691522136096723"
+
+
+def test_escape_inside_the_skip_equals_direct_iteration():
+    """A reference just below c* (its orbit keeps X[99] a hair under the bailout
+    and escapes at 100): samples on its right cross the bailout at X[99] + d[99]
+    with the series still stable at the default tolerance, so the reference's
+    binary search (mandelbrot.cpp:186-206) decides their count inside the skip;
+    samples on its left continue in HP and escape at 100.  Every count equals
+    direct iteration from z = c at 400 bits."""
+    mp.mp.prec = 400
+    ref = mp.nstr(mp.mpf(_C_STAR) - mp.mpf("1e-24"), 60)
+    nr, nc, N, width = 6, 8, 300, "1e-15"
+    o = O.OView(nr, nc)
+    o.set_view(N, ref, "0", width)
+    assert not o.use_hardware()
+    assert o.precompute_at(ref, "0") == 100
+    rc = np.array([(r, c) for r in range(nr) for c in range(nc)], dtype=np.int32)
+    got = o.render_samples(rc)
+    sz = mp.mpf(width) / nc
+    inside = 0
+    for k, (r, c) in enumerate(rc):
+        ua, found, d = o.sa_start(int(r), int(c))
+        assert ua == 0 and found == 99  # stable over the whole reference orbit
+        cr = mp.mpf(ref) + (int(c) - nc // 2) * sz
+        ci = (nr // 2 - int(r) - 1) * sz
+        assert got["iterations"][k] == _direct_count(cr, ci, N), (r, c)
+        inside += int(got["iterations"][k] == 99)
+    assert 0 < inside < len(rc)  # both branches ran
+
+
+class _Trunc:
+    """mpf arithmetic as the reference uses it (GMP mpf: every result truncated
+    toward zero at the working precision), from mpmath's low-level layer with
+    rounding mode 'd' at P = 64 ((bits + 127) // 64 + 1) bits (the precision
+    tests/golden/make_hp_goldens.py pins against both HP implementations)."""
+
+    def __init__(self, bits):
+        from mpmath import libmp
+        self.L = libmp
+        self.P = 64 * ((bits + 127) // 64 + 1)
+
+    def s(self, x):
+        return self.L.from_str(x, self.P, "d")
+
+    def d(self, x):
+        return self.L.from_float(float(x))
+
+    def add(self, a, b):
+        return self.L.mpf_add(a, b, self.P, "d")
+
+    def sub(self, a, b):
+        return self.L.mpf_sub(a, b, self.P, "d")
+
+    def mul(self, a, b):
+        return self.L.mpf_mul(a, b, self.P, "d")
+
+    def mul_i(self, a, k):
+        return self.L.mpf_mul(a, self.L.from_int(k), self.P, "d")
+
+    def div_i(self, a, k):
+        return self.L.mpf_div(a, self.L.from_int(k), self.P, "d")
+
+    def get_d(self, a):  # mpf_get_d: truncation toward zero
+        return self.L.to_float(self.L.mpf_pos(a, 53, "d"))
+
+
+def test_series_stop_index_and_continuation_pinned_independently():
+    """The default tolerance on a view whose samples continue past the skip (t11
+    at width 8e-16, N 3000, reference point at the view's centre):
+    * the first unstable index equals the first i at which isUnstable
+      (mandelbrot.cpp:138-142) holds when evaluated in exact arithmetic on the
+      same doubles (the series terms descend()ed, eps and its powers as the
+      reference forms them), unless the exact test lies within 1e-12 relative of
+      its threshold, and found = max(i - 3, 1) - 1;
+    * the count equals the reference's HP continuation (mandelbrot.cpp:209-224)
+      redone in mpmath with truncation at the same precision: the pixel
+      coordinate, the reference orbit and the continuation from
+      Y = X[found] + d[found], each operation truncated as mpf truncates.
+    Counts are not compared with iteration from z = c: the reference's series
+    (its new-index B, C recurrence) moves the skip point by ~1e-10 relative, and
+    chaotic tails turn that into different counts for some samples (printed);
+    good = max(i - 3, 1) is the reference's convention, which only the reference
+    itself could pin."""
+    v = json.load(open(os.path.join(GOLD, "views.json")))["t11"]
+    nr, nc, width, N = 4, 6, "8e-16", v["N"]
+    o = O.OView(nr, nc)
+    o.set_view(N, v["re"], v["im"], width)
+    assert not o.use_hardware()  # the series path (sample spacing below 1.5e-16)
+    o.precompute_at(v["re"], v["im"])
+    L = o.ref_len()
+    Xt, Xn, A, B, Cc = o.series()
+    rc = np.array([(r, c) for r in range(nr) for c in range(nc)], dtype=np.int32)
+    got = o.render_samples(rc)
+    T = _Trunc(o.bits())
+    cre, cim = T.s(v["re"]), T.s(v["im"])
+    sz = T.div_i(T.s(width), nc)
+    # the reference orbit as computeOrbit forms it (mandelbrot.cpp:97-110)
+    X = [(cre, cim)]
+    for i in range(1, L + 1):
+        pr, pi = X[-1]
+        xr = T.add(T.sub(T.mul(pr, pr), T.mul(pi, pi)), cre)
+        xi = T.add(T.mul_i(T.mul(pr, pi), 2), cim)
+        X.append((xr, xi))
+    assert all(T.get_d(X[i][0]) == Xt[2 * i] for i in range(0, L, 97))
+    tol = mp.mpf(1e-10)
+    mp.mp.prec = 300
+    differ_from_z0 = checked = 0
+    for k, (r, c) in enumerate(rc):
+        ua, found, d = o.sa_start(int(r), int(c))
+        e = o.pixel_eps(int(r), int(c))
+        e0, e1 = float(e[0]), float(e[1])
+        # eps2 = sq(eps), eps3 = eps * eps2 as the reference rounds them (complex.h)
+        e2 = (e0 * e0 - e1 * e1, (2.0 * e0) * e1)
+        e3 = (e0 * e2[0] - e1 * e2[1], e0 * e2[1] + e1 * e2[0])
+        first = 0
+        for i in range(1, L):
+            b = mp.mpc(B[2 * i], B[2 * i + 1]) * mp.mpc(*e2)
+            cc = mp.mpc(Cc[2 * i], Cc[2 * i + 1]) * mp.mpc(*e3)
+            if abs(b) ** 2 * tol < abs(cc) ** 2:
+                first = i
+                break
+        if first != ua:
+            i = min(first or L, ua or L)
+            b = mp.mpc(B[2 * i], B[2 * i + 1]) * mp.mpc(*e2)
+            cc = mp.mpc(Cc[2 * i], Cc[2 * i + 1]) * mp.mpc(*e3)
+            assert abs(abs(b) ** 2 * tol / abs(cc) ** 2 - 1) < 1e-12, (r, c, ua, first)
+        assert found == (max(ua - 3, 1) if ua else L) - 1
+        # the pixel (mandelbrot.cpp:269-283) and the HP continuation, truncated
+        y0r = T.add(cre, T.mul_i(sz, int(c) - nc // 2))
+        y0i = T.add(cim, T.mul_i(sz, nr // 2 - int(r) - 1))
+        yr, yi = T.add(X[found][0], T.d(d[0])), T.add(X[found][1], T.d(d[1]))
+        dr, di = T.get_d(yr), T.get_d(yi)
+        if dr * dr + di * di > 1024.0 ** 2:
+            continue  # escaped inside the skip: the binary search (pinned above)
+        want = N
+        for i in range(found + 1, N):
+            nr_ = T.add(T.sub(T.mul(yr, yr), T.mul(yi, yi)), y0r)
+            ni_ = T.add(T.mul_i(T.mul(yr, yi), 2), y0i)
+            yr, yi = nr_, ni_
+            dr, di = T.get_d(yr), T.get_d(yi)
+            if dr * dr + di * di > 1024.0 ** 2:
+                want = i
+                break
+        assert got["iterations"][k] == want, (r, c, found)
+        checked += 1
+        cr = mp.mpf(v["re"]) + (int(c) - nc // 2) * (mp.mpf(width) / nc)
+        ci = mp.mpf(v["im"]) + (nr // 2 - int(r) - 1) * (mp.mpf(width) / nc)
+        with mp.workprec(400):
+            differ_from_z0 += int(got["iterations"][k] != _direct_count(cr, ci, N))
+    assert checked == len(rc)
+    print("t11 at 8e-16: %d of %d counts differ from 400-bit iteration from z = c" % (differ_from_z0, len(rc)))
