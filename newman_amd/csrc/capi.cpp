@@ -638,6 +638,14 @@ int nm_colour(nm_mandelbrot* h, int frame, const uint8_t* pal, int npal, int smo
 int nm_colour_grid(const nm_escape_value* grid, int nr, int nc, int N, const uint8_t* pal, int npal, int smooth,
                    int sc, uint8_t* out, double* ms) {
   if (!grid || !pal || !out || nr <= 0 || nc <= 0 || sc <= 0 || N <= 0 || npal < N) return NM_ERR_ARG;
+  // getColor reads pal[it] and pal[it - 1] (viewer.cpp:89-95): a negative count
+  // would index before the palette on the device
+  const size_t n = (size_t)nr * nc;
+  for (size_t i = 0; i < n; i++)
+    if (grid[i].iterations < 0) {
+      g_err = "nm_colour_grid: negative iteration count";
+      return NM_ERR_ARG;
+    }
   return guard([&] {
     nm::Engine* eng = nullptr;
     int st = nm::Engine::instance(&eng);

@@ -1253,7 +1253,7 @@ __device__ __forceinline__ int pert_block(double& dr, double& di, const double e
   int kdone = K;
   bool moved = false;
   int escj = -1;  // step of the block's first escape
-  bool seen = false;  // an escape earlier in the block
+  unsigned long long seen = 0;  // wave-uniform: lanes with an escape earlier in the block
   ld_double2 z = zp[0];
 #pragma unroll
   for (int j = 0; j < K; j++) {
@@ -1267,20 +1267,30 @@ __device__ __forceinline__ int pert_block(double& dr, double& di, const double e
     if (END) reb = reb || (moved ? orbit_index(zp, head) + j == mend : j == endj);
     double wr = zr + yr, wi = zi + yi;
     double d0r = dr, d0i = di;
-    if (reb) {  // rebase: d = Y against Zd[0] = 0, so W = Y
-      d0r = yr;
-      d0i = yi;
-      wr = yr;
-      wi = yi;
-      zp = orbit_back(head, j);
-      moved = true;
+    // a rebase (d = Y against Zd[0] = 0, so W = Y) moves four doubles and the
+    // cursor under the lane mask; about half the wave-steps have no rebasing
+    // lane and skip those moves
+    if (__builtin_expect(__ballot(reb) != 0, 0)) {
+      if (reb) {
+        d0r = yr;
+        d0i = yi;
+        wr = yr;
+        wi = yi;
+        zp = orbit_back(head, j);
+        moved = true;
+      }
     }
     if (j + 1 < K) z = zp[j + 1];
-    const bool e = r2 > b2;
-    const bool first = e && !seen;
-    seen = seen || e;
-    escj = first ? j : escj;
-    esc_r2 = first ? r2 : esc_r2;
+    // the first escape's record, only on wave-steps where some lane escapes
+    // for the first time in the block
+    const unsigned long long first = __ballot(r2 > b2) & ~seen;
+    if (__builtin_expect(first != 0, 0)) {
+      seen |= first;
+      if ((first >> lane_id()) & 1ull) {
+        escj = j;
+        esc_r2 = r2;
+      }
+    }
     dr = fma(d0r, wr, fma(-d0i, wi, er));
     di = fma(d0r, wi, fma(d0i, wr, ei));
     // the escape record is complete at the end of each step (no chain of
@@ -1330,26 +1340,35 @@ __device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, flo
     const float dr = ldexpf(wr, ex), di = ldexpf(wi, ex);
     const float yr = zr + dr, yi = zi + di;
     const float r2 = fmaf(yr, yr, yi * yi);
-    if (r2 > b2g) {  // near the bailout: the reference's test in FP64
-      const double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
-      const double r2d = ydr * ydr + ydi * ydi;
-      if (r2d > b2) {  // first escape: NaN in wr reaches every later step
-        esc = it + j;
-        esc_r2 = r2d;
-        wr = qnanf;
+    // near the bailout: the reference's test in FP64 (only on wave-steps where
+    // some lane comes near it)
+    if (__builtin_expect(__ballot(r2 > b2g) != 0, 0)) {
+      if (r2 > b2g) {
+        const double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
+        const double r2d = ydr * ydr + ydi * ydi;
+        if (r2d > b2) {  // first escape: NaN in wr reaches every later step
+          esc = it + j;
+          esc_r2 = r2d;
+          wr = qnanf;
+        }
       }
     }
-    const float dmag = fmaf(dr, dr, di * di);
-    bool reb = r2 < dmag;
+    // the rebase test |Y|^2 < |d|^2 in the mantissa's scale: |Z 2^-ex + w|^2 <
+    // |w|^2 (d = w 2^ex underflows FP32 long before the deltas this form is for;
+    // Z 2^-ex is formed in FP64, then rounded; an overflow to inf means |Y| >> |d|)
+    const float ysr = (float)ldexp(zrd, -ex) + wr, ysi = (float)ldexp(zid, -ex) + wi;
+    bool reb = fmaf(ysr, ysr, ysi * ysi) < fmaf(wr, wr, wi * wi);
     if (END) reb = reb || (moved ? orbit_index(zp, head) + j == mend : j == endj);
     float Wr = zr + yr, Wi = zi + yi;
-    if (reb) {  // rebase in FP64: d = Y against Zd[0] = 0, so W = Y
-      const double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
-      fx_normalise(ydr, ydi, er, ei, wr, wi, ex, esr, esi);
-      Wr = (float)ydr;
-      Wi = (float)ydi;
-      zp = orbit_back(head, j);
-      moved = true;
+    if (__builtin_expect(__ballot(reb) != 0, 0)) {
+      if (reb) {  // rebase in FP64: d = Y against Zd[0] = 0, so W = Y
+        const double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
+        fx_normalise(ydr, ydi, er, ei, wr, wi, ex, esr, esi);
+        Wr = (float)ydr;
+        Wi = (float)ydi;
+        zp = orbit_back(head, j);
+        moved = true;
+      }
     }
     if (j + 1 < K) z = zp[j + 1];
     const float nwr = fmaf(wr, Wr, fmaf(-wi, Wi, esr));

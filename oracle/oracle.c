@@ -907,9 +907,10 @@ static orc_escape pert_from_fx(const pert_ctx* P, double dr, double di, double e
       }
     }
     if (it == N - 1) break;
-    float dmag = fmaf(fdr, fdr, fdi * fdi);
+    /* the rebase test in the mantissa's scale: |Z 2^-ex + w|^2 < |w|^2 */
+    float ysr = (float)ldexp(zrd, -ex) + wr, ysi = (float)ldexp(zid, -ex) + wi;
     float Wr = zr + yr, Wi = zi + yi;
-    if (r2 < dmag || m == P->mend) {
+    if (fmaf(ysr, ysr, ysi * ysi) < fmaf(wr, wr, wi * wi) || m == P->mend) {
       double ydr = zrd + ldexp((double)wr, ex), ydi = zid + ldexp((double)wi, ex);
       fx_normalise(ydr, ydi, er, ei, &wr, &wi, &ex, &esr, &esi);
       Wr = (float)ydr;

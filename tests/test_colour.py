@@ -109,6 +109,19 @@ def test_palette_rejects_bad_input(tmp_path):
         nm.Palette().cache(10)  # nothing loaded
 
 
+def test_colour_grid_rejects_negative_counts():
+    """getColor reads pal[it] and pal[it - 1] (viewer.cpp:89-95): a caller's grid
+    with a negative count is refused on the host, before any device work."""
+    nm = _nm()
+    g = np.zeros((4, 4), dtype=nm.ESCAPE_DTYPE)
+    g["iterations"] = 3
+    g["iterations"][2, 1] = -1
+    pal = np.zeros((16, 3), dtype=np.uint8)
+    with pytest.raises(nm.NewmanError) as e:
+        nm.colour_grid(g, 16, pal, True, 1)
+    assert e.value.code == nm.NM_ERR_ARG
+
+
 def _keyframes(rng, kr, kc):
     return (rng.integers(0, 256, size=(kr, kc, 3), dtype=np.uint8),
             rng.integers(0, 256, size=(kr, kc, 3), dtype=np.uint8))
