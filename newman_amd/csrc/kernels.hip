@@ -241,11 +241,12 @@ __device__ __forceinline__ double log2_near(double q) {
 // log2_near's division costs about 30.
 constexpr int kLog2Tab = 128;
 __constant__ double2 d_log2tab[kLog2Tab];  // (r_i, l_i), filled at engine start
-__device__ __forceinline__ double log2_tab(double q) {
+template <class Tab>
+__device__ __forceinline__ double log2_tab(double q, Tab tab) {
   const long long b = __double_as_longlong(q);
   const int E = (int)((b >> 52) & 0x7ff) - 1023;
   const double m = __longlong_as_double((b & 0x000fffffffffffffLL) | 0x3ff0000000000000LL);
-  const double2 t = d_log2tab[(int)((b >> 45) & (kLog2Tab - 1))];
+  const double2 t = tab[(int)((b >> 45) & (kLog2Tab - 1))];
   const double f = fma(m, t.x, -1.0);
   // (-1)^(k+1) / (k ln 2), k = 7 .. 1
   double p = 0.2060992915555662;
@@ -264,9 +265,19 @@ __device__ __forceinline__ double log2_tab(double q) {
 // (measured on 2e7 radii at bailout 1024; the stored float differed from the
 // correctly rounded one once), far below the float the value is stored in.  An
 // infinite r2, or a bailout below 2 (smooth_k = 0), takes the library logarithms.
-__device__ __forceinline__ float smooth_of(double r2, const Common& c) {
+// The table is d_log2tab itself or a workgroup's LDS copy of it (load_log2tab):
+// a wave waiting for an LDS read does not also wait for its outstanding stores,
+// as a wait for a global load does (one in-order counter covers both).
+template <class Tab = const double2*>
+__device__ __forceinline__ float smooth_of(double r2, const Common& c, Tab tab = d_log2tab) {
   if (c.smooth_k == 0.0 || !(r2 < INFINITY)) return (float)(1.0 - log2(0.5 * log(r2) / c.log_b));
-  return (float)(1.0 - log2_tab(log2_tab(r2) * c.smooth_k));
+  return (float)(1.0 - log2_tab(log2_tab(r2, tab) * c.smooth_k, tab));
+}
+
+// the workgroup's LDS copy of d_log2tab (all threads call it; it ends with a barrier)
+__device__ __forceinline__ void load_log2tab(double2* lds) {
+  for (int i = threadIdx.x; i < kLog2Tab; i += blockDim.x) lds[i] = d_log2tab[i];
+  __syncthreads();
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -461,12 +472,18 @@ __device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int
 template <bool FMA2>
 __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   const Common& c = a.c;
+  __shared__ double2 ltab[kLog2Tab];
+  load_log2tab(ltab);
   unsigned long long ncard = 0, nres = 0, executed = 0, credited = 0;
   const uint32_t ntiles = (c.total + kTile - 1) / kTile;
   const int N = c.N;
   const int pre = min(a.pre_iter, N);
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t mine = 0;
+    // a tile inside one frame reads its frame record through scalar loads
+    const uint32_t f0 = frame_of(tile * kTile, c.inv_frame);
+    const uint32_t f1 = frame_of(min(tile * kTile + kTile, c.total) - 1, c.inv_frame);
+    const bool one_frame = f0 == f1;
     for (int j = 0; j < kPerThread; j++) {
       // every lane of the wave runs this body (no early continue), so the
       // pre-iteration below is wave-uniform control flow
@@ -475,7 +492,7 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
       bool in = false;
       double cx = 0.0, cy = 0.0;
       if (valid) {
-        const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
+        const DevFrame& f = c.frames[one_frame ? __builtin_amdgcn_readfirstlane(f0) : frame_of(gid, c.inv_frame)];
         const Loc lc = locate(f, gid - f.sample_base);
         valid = lc.valid;
         if (valid) {
@@ -513,7 +530,7 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
         // only below N; a lane still iterating after kend updates is handed on
         if (kesc >= 0 && kesc < N) {
           executed += (unsigned long long)(kesc + 1);
-          c.out[gid] = EscapeValue{kesc, smooth_of(mgesc, c)};
+          c.out[gid] = EscapeValue{kesc, smooth_of(mgesc, c, (const double2*)ltab)};
           nres++;
           credited += (unsigned long long)kesc;
         } else if (kesc >= N || kend >= N) {
