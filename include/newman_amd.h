@@ -34,6 +34,8 @@ extern "C" {
 
 #define NM_DELTA_FP64 0  /* perturbation deltas in FP64 */
 #define NM_DELTA_FP32X 1 /* extension: FP32 mantissa + int32 exponent deltas */
+#define NM_SERIES_DOUBLE 0   /* the reference's series test and skip in double */
+#define NM_SERIES_FLOATEXP 1 /* extension: the same in double mantissa + int exponent */
 
 /* grid.h:8-15 RenderGrid::EscapeValue, 8 bytes, same layout */
 typedef struct nm_escape_value {
@@ -133,6 +135,14 @@ int nm_save_view(const nm_mandelbrot* h, const char* path);
 int nm_set_bailout(nm_mandelbrot* h, double bailout);
 /* NM_DELTA_FP64 (default) or NM_DELTA_FP32X */
 int nm_set_delta_mode(nm_mandelbrot* h, int mode);
+/* The series test and skip of the series path (mandelbrot.cpp:138-142, 160-178):
+ * NM_SERIES_DOUBLE (default) in the reference's doubles; NM_SERIES_FLOATEXP
+ * (extension) in the same operation order with each real held as a double
+ * mantissa and an int exponent: the same results wherever the reference's
+ * doubles are normal numbers, and a working test below widths of about 1e-90,
+ * where the reference's |B e^2|^2 and |C e^3|^2 leave the double range and every
+ * index tests stable.  Takes effect on a precomputed reference at once. */
+int nm_set_series_mode(nm_mandelbrot* h, int mode);
 /* a view by width: centre strings and width = cols * sz.re (square samples),
  * parsed at the precision the width needs, then setPrecision */
 int nm_set_view(nm_mandelbrot* h, int N, const char* cre, const char* cim, const char* width);

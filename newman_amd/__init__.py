@@ -23,6 +23,8 @@ NM_ERR_IO = -6
 NM_ERR_NOMEM = -7
 NM_DELTA_FP64 = 0
 NM_DELTA_FP32X = 1
+NM_SERIES_DOUBLE = 0
+NM_SERIES_FLOATEXP = 1
 
 _ERRNAMES = {NM_ERR_ARG: "NM_ERR_ARG", NM_ERR_HIP: "NM_ERR_HIP", NM_ERR_STATE: "NM_ERR_STATE",
              NM_ERR_NODEVICE: "NM_ERR_NODEVICE", NM_ERR_RANGE: "NM_ERR_RANGE", NM_ERR_IO: "NM_ERR_IO",
@@ -218,6 +220,7 @@ _SIGS = [
     ("nm_save_view", C.c_int, [_H, C.c_char_p]),
     ("nm_set_bailout", C.c_int, [_H, C.c_double]),
     ("nm_set_delta_mode", C.c_int, [_H, C.c_int]),
+    ("nm_set_series_mode", C.c_int, [_H, C.c_int]),
     ("nm_set_view", C.c_int, [_H, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p]),
     ("nm_precompute_at", C.c_int, [_H, C.c_char_p, C.c_char_p]),
     ("nm_reference_length", C.c_int, [_H]),
@@ -500,6 +503,10 @@ class Mandelbrot:
 
     def set_delta_mode(self, mode):
         _check(lib().nm_set_delta_mode(self._h, int(mode)), "nm_set_delta_mode")
+
+    def set_series_mode(self, mode):
+        """NM_SERIES_DOUBLE (the reference's doubles) or NM_SERIES_FLOATEXP (extension)."""
+        _check(lib().nm_set_series_mode(self._h, int(mode)), "nm_set_series_mode")
 
     def set_view(self, N, cre, cim, width):
         _check(lib().nm_set_view(self._h, int(N), _b(str(cre)), _b(str(cim)), _b(str(width))), "nm_set_view")

@@ -283,6 +283,11 @@ int nm_set_delta_mode(nm_mandelbrot* h, int mode) {
   return NM_OK;
 }
 
+int nm_set_series_mode(nm_mandelbrot* h, int mode) {
+  if (!h || (mode != NM_SERIES_DOUBLE && mode != NM_SERIES_FLOATEXP)) return NM_ERR_ARG;
+  return guard([&] { return h->m.setSeriesMode(mode); });
+}
+
 int nm_set_view(nm_mandelbrot* h, int N, const char* cre, const char* cim, const char* width) {
   if (!h) return NM_ERR_ARG;
   return guard([&] { return h->m.setView(N, cre, cim, width); });

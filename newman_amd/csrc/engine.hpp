@@ -23,6 +23,11 @@ enum CardMode : int32_t { CARD_NONE = 0, CARD_ALL = 1, CARD_TEST_DD = 2, CARD_MA
 
 // Arithmetic used for the deltas of the perturbation kernel.
 enum DeltaMode : int32_t { DELTA_FP64 = 0, DELTA_FP32X = 1 };
+// the series test and skip in the reference's doubles, or in floatexp (double
+// mantissa + int exponent: the same results where the doubles are normal
+// numbers, and a working test where they under- or overflow)
+enum SeriesMode : int32_t { SERIES_DOUBLE = 0, SERIES_FLOATEXP = 1 };
+constexpr int32_t kFxZeroExp = -0x40000000;  // the exponent of a floatexp zero
 
 // One frame of the plain-double path (mandelbrot.cpp:231-254): per-column and
 // per-row truncated coordinates (descend of the HP pixel map) plus the remainder
@@ -53,6 +58,11 @@ struct SaReferenceHost {
   std::vector<double> Zd;      // 2*(len+2): Zd[0] = 0, Zd[k] = nearest(X[k-1]); Zd[len+1] = X[len]
   std::vector<double> Xlo;     // 2*len: nearest(X[k] - Zd[k+1])
   std::vector<double> A, B, C; // 2*len: descend(A[i]) etc. (mandelbrot.cpp:166-168)
+  // The same coefficients in floatexp: per index and component the truncated
+  // 53-bit mantissa in [1/2, 1) (0 for zero) and its exponent (kFxZeroExp for zero)
+  std::vector<double> Am, Bm, Cm;    // 2*len
+  std::vector<int32_t> Ae, Be, Ce;   // 2*len
+  int fxs = 0;  // 1: the tables below come from the floatexp form, and the kernels test in it
   // Stop-index tables (prefix over j in [1, i]):
   std::vector<double> PM;      // min log2(tol |B_j|^2 / |C_j|^2)
   std::vector<double> BX, BN;  // max / min log2 |B_j|^2

@@ -166,6 +166,11 @@ struct SeriesArgs {
   // every sample searches its own (the reference's per-pixel scan)
   const int* tile_ua;
   uint32_t skip_T, skip_ntc, skip_tpf;  // tile side, tiles per frame row, tiles per frame
+  // floatexp series (fxs != 0): A, B, C as mantissas (double2) and exponents
+  // (int2) per index; the thresholds above are then built from them
+  const double2 *Am, *Bm, *Cm;
+  const int2 *Ae, *Be, *Ce;
+  int fxs;
 };
 
 // ------------------------------------------------------------ device helpers
@@ -1031,6 +1036,98 @@ __device__ int sa_stop_index(const SeriesArgs& a, double L, int2 br, double2 e2,
   return 0;
 }
 
+// ------------------------------------------------ floatexp series (extension)
+// The reference's series test and skip (mandelbrot.cpp:138-142, 160-178) in the
+// same operation order, each real held as m * 2^e with m = 0 (e = kFxZeroE) or
+// 1/2 <= |m| < 1.  A product or sum rounds the mantissa once, to nearest, as the
+// double operation on the scaled operands does, so wherever the reference's
+// doubles are normal numbers the result is the same number; below widths of
+// about 1e-90, where its |B e^2|^2 and |C e^3|^2 leave the double range and the
+// test degenerates, nothing here under- or overflows.
+constexpr int kFxZeroE = -0x40000000;
+struct fxr {
+  double m;
+  int e;
+};
+struct fxc {
+  fxr re, im;
+};
+__device__ __forceinline__ fxr fx_make(double d, int e) {
+  int k;
+  const double m = frexp(d, &k);
+  return d == 0.0 ? fxr{0.0, kFxZeroE} : fxr{m, e + k};
+}
+__device__ __forceinline__ fxr fx_mul(fxr a, fxr b) { return fx_make(a.m * b.m, a.e + b.e); }
+// the operand with the larger exponent keeps its scale (a zero has the least
+// exponent, and the other operand comes through unchanged)
+__device__ __forceinline__ fxr fx_add(fxr a, fxr b) {
+  const bool ab = a.e >= b.e;
+  const fxr h = ab ? a : b, l = ab ? b : a;
+  return fx_make(h.m + ldexp(l.m, max(l.e - h.e, -2100)), h.e);
+}
+__device__ __forceinline__ fxr fx_neg(fxr a) { return fxr{-a.m, a.e}; }
+// a < b for a, b >= 0
+__device__ __forceinline__ bool fx_lt(fxr a, fxr b) { return a.e < b.e || (a.e == b.e && a.m < b.m); }
+__device__ __forceinline__ double fx_get(fxr a) { return ldexp(a.m, max(a.e, -2100)); }
+__device__ __forceinline__ fxc fxc_of(double2 z) { return fxc{fx_make(z.x, 0), fx_make(z.y, 0)}; }
+__device__ __forceinline__ fxc fxc_mul(fxc a, fxc b) {  // complex.h:29-31
+  return fxc{fx_add(fx_mul(a.re, b.re), fx_neg(fx_mul(a.im, b.im))), fx_add(fx_mul(a.re, b.im), fx_mul(a.im, b.re))};
+}
+__device__ __forceinline__ fxc fxc_sq(fxc a) {  // complex.h:19-21: (re^2 - im^2, (2 re) im)
+  return fxc{fx_add(fx_mul(a.re, a.re), fx_neg(fx_mul(a.im, a.im))), fx_mul(fxr{a.re.m, a.re.e + 1}, a.im)};
+}
+__device__ __forceinline__ fxr fxc_sqmag(fxc a) { return fx_add(fx_mul(a.re, a.re), fx_mul(a.im, a.im)); }
+__device__ __forceinline__ fxc fx_coef(const double2* m, const int2* e, int i) {
+  const double2 mm = m[i];
+  const int2 ee = e[i];
+  return fxc{fxr{mm.x, ee.x}, fxr{mm.y, ee.y}};
+}
+
+// e, e^2 and e^3 of a sample in floatexp (mandelbrot.cpp:160-161)
+struct EpsFx {
+  fxc e, e2, e3;
+};
+__device__ __forceinline__ EpsFx eps_fx(double2 e) {
+  EpsFx p;
+  p.e = fxc_of(e);
+  p.e2 = fxc_sq(p.e);
+  p.e3 = fxc_mul(p.e, p.e2);
+  return p;
+}
+
+__device__ __forceinline__ bool unstable_fx(const SeriesArgs& a, int i, const EpsFx& p) {
+  const fxc b = fxc_mul(fx_coef(a.Bm, a.Be, i), p.e2);
+  const fxc c = fxc_mul(fx_coef(a.Cm, a.Ce, i), p.e3);
+  return fx_lt(fx_mul(fxc_sqmag(b), fx_make(a.tol, 0)), fxc_sqmag(c));
+}
+
+__device__ __forceinline__ double2 sa_term_fx(const SeriesArgs& a, int i, double2 e, const EpsFx& p) {
+  if (i == 0) return e;
+  const fxc ta = fxc_mul(fx_coef(a.Am, a.Ae, i), p.e);
+  const fxc tb = fxc_mul(fx_coef(a.Bm, a.Be, i), p.e2);
+  const fxc tc = fxc_mul(fx_coef(a.Cm, a.Ce, i), p.e3);
+  return make_double2(fx_get(fx_add(fx_add(ta.re, tb.re), tc.re)), fx_get(fx_add(fx_add(ta.im, tb.im), tc.im)));
+}
+
+// The first unstable index with the floatexp test.  Its rounding is a few units
+// in the last place of |B e^2|^2 tol and |C e^3|^2 with no range limit, so an
+// index whose threshold T_j = log2(tol |B_j|^2 / |C_j|^2) lies more than kMarg
+// above log2 |e|^2 is stable (T_j = -inf where B_j = 0 and C_j != 0: unstable;
+// +inf where C_j = 0: stable); only the indices at or below that line are tested,
+// in order, found on the prefix minimum and then on the sparse table.
+__device__ int sa_stop_index_fx(const SeriesArgs& a, double L, int2 br, const EpsFx& p) {
+  const int last = a.len - 1;
+  if (last < 1 || L == -INFINITY) return 0;  // e = 0: the test never fires
+  const double v = L + kMarg;
+  int i = first_le(a.PM, br.x, br.y, v);
+  if (i > br.y && br.y < last) i = first_le(a.PM, br.y + 1, last, v);
+  while (i <= last) {
+    if (unstable_fx(a, i, p)) return i;
+    i = first_le_from(a, i + 1, v);
+  }
+  return 0;
+}
+
 // Per-tile series skip: the first index at which the reference's stability test
 // fails at each tile's probe (its sample farthest from the reference point).
 // One wave per tile.  mode 0: the reference's scan over i = 1, 2, ... (64 indices
@@ -1052,21 +1149,24 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
   const double2 e2 = make_double2(e.x * e.x - e.y * e.y, 2.0 * e.x * e.y);
   const double2 e3 = cmul(e, e2);
   const double L = log2_sqmag(e);
+  const EpsFx pf = a.fxs ? eps_fx(e) : EpsFx{};
   const int last = a.len - 1;
   int ua = -1, h = 0, ev = 0;
   if (last < 1 || L == -INFINITY) {  // e = 0: the test never fires
     ua = 0;
     h = 1;
   } else if (mode == 1) {
-    ua = sa_stop_index(a, L, stop_bracket(a, L + kMarg), e2, e3);
+    ua = a.fxs ? sa_stop_index_fx(a, L, stop_bracket(a, L + kMarg), pf)
+               : sa_stop_index(a, L, stop_bracket(a, L + kMarg), e2, e3);
     h = 1;
   } else if (mode == 2) {
     const int p = pred[t];
     const int below = p > 0 ? p - 1 : last;
-    bool ok = p >= 0 && p <= last && below >= 1 && a.PM[below] > L + kMarg && a.SP[below] == 0 &&
-              rounding_certified(a, L, a.BN[below], a.BX[below], a.CX[below]);
+    // the floatexp test has no range limits: the prefix minimum alone certifies
+    bool ok = p >= 0 && p <= last && below >= 1 && a.PM[below] > L + kMarg &&
+              (a.fxs || (a.SP[below] == 0 && rounding_certified(a, L, a.BN[below], a.BX[below], a.CX[below])));
     if (ok && p > 0) {
-      ok = unstable_at(a, p, e2, e3);
+      ok = a.fxs ? unstable_fx(a, p, pf) : unstable_at(a, p, e2, e3);
       ev = 1;
     }
     if (ok) {
@@ -1080,7 +1180,7 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
     ua = 0;
     for (int base = 1; base <= last; base += 64) {
       const int i = base + lane;
-      const bool u = i <= last && unstable_at(a, i, e2, e3);
+      const bool u = i <= last && (a.fxs ? unstable_fx(a, i, pf) : unstable_at(a, i, e2, e3));
       const unsigned long long m = __ballot(u);
       if (m) {
         const int k = __ffsll((unsigned long long)m);  // 1 + the first unstable lane
@@ -1100,9 +1200,10 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
 
 // The series start kernel: per sample the reference's series start
 // (mandelbrot.cpp:144-206) and the first perturbation step; samples that need
-// more are flagged for the iterate kernel.  Held to 80 VGPRs (6 waves per SIMD;
-// one 8-byte value spills): 5% less time than at its natural 89 (5 waves) where
-// the stop search dominates.
+// more are flagged for the iterate kernel.  Held to 80 VGPRs (6 waves per SIMD):
+// 5% less time than at its natural 89 (5 waves) where the stop search dominates.
+// FXS: the series test and skip in floatexp.
+template <bool FXS>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_series_start(SeriesArgs a) {
   const Common& c = a.c;
   unsigned long long ncard = 0, nres = 0, skipped = 0, credited = 0;
@@ -1151,19 +1252,32 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
         ncard++;
       } else {
         // eps2 = sq(eps), eps3 = eps * eps2 (complex.h:19-21, mandelbrot.cpp:160-161)
-        double2 e2 = make_double2(e.x * e.x - e.y * e.y, 2.0 * e.x * e.y);
-        double2 e3 = cmul(e, e2);
-        int ua = a.tile_ua ? a.tile_ua[skip_tile] : sa_stop_index(a, L, br, e2, e3);
+        double2 e2, e3;
+        EpsFx pf;
+        if constexpr (FXS) {
+          pf = eps_fx(e);
+        } else {
+          e2 = make_double2(e.x * e.x - e.y * e.y, 2.0 * e.x * e.y);
+          e3 = cmul(e, e2);
+        }
+        auto term = [&](int i) {
+          if constexpr (FXS) return sa_term_fx(a, i, e, pf);
+          else return sa_term(a, i, e, e2, e3);
+        };
+        int ua;
+        if (a.tile_ua) ua = a.tile_ua[skip_tile];
+        else if constexpr (FXS) ua = sa_stop_index_fx(a, L, br, pf);
+        else ua = sa_stop_index(a, L, br, e2, e3);
         int dsize = ua ? max(ua - 3, 1) : a.len;
         int found = dsize - 1;
-        double2 d = sa_term(a, found, e, e2, e3);
+        double2 d = term(found);
         double2 zf = a.Zd[found + 1], xl = a.Xlo[found];
         double yr = trunc_sum3(zf.x, xl.x, d.x), yi = trunc_sum3(zf.y, xl.y, d.y);
         if (yr * yr + yi * yi > c.b2) {
           // escape inside the skip: the reference's binary search (mandelbrot.cpp:186-206)
           int low = 0, high = dsize - 1, mid = dsize / 2;
           while (low <= high) {
-            double2 dm = sa_term(a, mid, e, e2, e3);
+            double2 dm = term(mid);
             double2 zm = a.Zd[mid + 1], xm = a.Xlo[mid];
             double ym = trunc_sum3(zm.x, xm.x, dm.x), yn = trunc_sum3(zm.y, xm.y, dm.y);
             if (!(ym * ym + yn * yn > c.b2)) low = mid + 1;
@@ -1173,7 +1287,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
             }
             mid = (low + high) / 2;
           }
-          d = sa_term(a, found, e, e2, e3);
+          d = term(found);
           zf = a.Zd[found + 1];
           xl = a.Xlo[found];
           yr = trunc_sum3(zf.x, xl.x, d.x);
@@ -1769,6 +1883,9 @@ class EngineImpl : public Engine {
   // reference layout (offsets into d_ref)
   int ref_len = 0;
   const double2 *rZd = nullptr, *rXlo = nullptr, *rA = nullptr, *rB = nullptr, *rC = nullptr;
+  const double2 *rAm = nullptr, *rBm = nullptr, *rCm = nullptr;
+  const int2 *rAe = nullptr, *rBe = nullptr, *rCe = nullptr;
+  int rfxs = 0;
   const double *rPM = nullptr, *rBX = nullptr, *rBN = nullptr, *rCX = nullptr, *rCN = nullptr;
   const int* rSP = nullptr;
   const double *rST = nullptr, *rBNs = nullptr, *rCNs = nullptr;
@@ -1832,6 +1949,9 @@ class EngineImpl : public Engine {
            oBX = place((size_t)len * 8), oBN = place((size_t)len * 8), oCX = place((size_t)len * 8),
            oCN = place((size_t)len * 8), oSP = place((size_t)len * 4), oST = place(ref.ST.size() * 8),
            oBNs = place(ref.BNs.size() * 8), oCNs = place(ref.CNs.size() * 8), oPMT = place(ref.PMT.size() * 4);
+    const bool fxs = ref.fxs && ref.Am.size() == 2 * (size_t)len;
+    const size_t nfm = fxs ? (size_t)len * 16 : 0, nfe = fxs ? (size_t)len * 8 : 0;
+    size_t oAm = place(nfm), oBm = place(nfm), oCm = place(nfm), oAe = place(nfe), oBe = place(nfe), oCe = place(nfe);
     std::vector<char> h(off, 0);
     std::memcpy(h.data() + oZ, ref.Zd.data(), ref.Zd.size() * 8);
     std::memcpy(h.data() + oXlo, ref.Xlo.data(), ref.Xlo.size() * 8);
@@ -1848,6 +1968,14 @@ class EngineImpl : public Engine {
     std::memcpy(h.data() + oBNs, ref.BNs.data(), ref.BNs.size() * 8);
     std::memcpy(h.data() + oCNs, ref.CNs.data(), ref.CNs.size() * 8);
     if (!ref.PMT.empty()) std::memcpy(h.data() + oPMT, ref.PMT.data(), ref.PMT.size() * 4);
+    if (fxs) {
+      std::memcpy(h.data() + oAm, ref.Am.data(), nfm);
+      std::memcpy(h.data() + oBm, ref.Bm.data(), nfm);
+      std::memcpy(h.data() + oCm, ref.Cm.data(), nfm);
+      std::memcpy(h.data() + oAe, ref.Ae.data(), nfe);
+      std::memcpy(h.data() + oBe, ref.Be.data(), nfe);
+      std::memcpy(h.data() + oCe, ref.Ce.data(), nfe);
+    }
     int st = grow(&d_ref, &n_ref_bytes, off);
     if (st) return st;
     NM_HIP(hipMemcpyAsync(d_ref, h.data(), off, hipMemcpyHostToDevice, stream));
@@ -1872,6 +2000,13 @@ class EngineImpl : public Engine {
     rpmt_n = ref.PMT.empty() ? 0 : ref.pmt_n;
     rpmt_base = ref.pmt_base;
     rpmt_inv_h = ref.pmt_inv_h;
+    rfxs = fxs ? 1 : 0;
+    rAm = fxs ? (const double2*)(b + oAm) : nullptr;
+    rBm = fxs ? (const double2*)(b + oBm) : nullptr;
+    rCm = fxs ? (const double2*)(b + oCm) : nullptr;
+    rAe = fxs ? (const int2*)(b + oAe) : nullptr;
+    rBe = fxs ? (const int2*)(b + oBe) : nullptr;
+    rCe = fxs ? (const int2*)(b + oCe) : nullptr;
     ref_len = len;
     ref_serial = serial;
     return NM_OK;
@@ -2249,22 +2384,18 @@ class EngineImpl : public Engine {
       stats->block_k = K;
       stats->waves_per_simd = waves;
     }
+    auto go = [&](auto iter) {
+      return a.fxs ? run(a, k_series_start<true>, iter, waves, opt, stats, nout, host_out, emit_to)
+                   : run(a, k_series_start<false>, iter, waves, opt, stats, nout, host_out, emit_to);
+    };
     switch (K) {
-      case 8:
-        return fx ? run(a, k_series_start, k_series_iterate<8, true>, waves, opt, stats, nout, host_out, emit_to)
-                  : run(a, k_series_start, k_series_iterate<8, false>, waves, opt, stats, nout, host_out, emit_to);
-      case 12:
-        return fx ? run(a, k_series_start, k_series_iterate<12, true>, waves, opt, stats, nout, host_out, emit_to)
-                  : run(a, k_series_start, k_series_iterate<12, false>, waves, opt, stats, nout, host_out, emit_to);
-      case 16:
-        return fx ? run(a, k_series_start, k_series_iterate<16, true>, waves, opt, stats, nout, host_out, emit_to)
-                  : run(a, k_series_start, k_series_iterate<16, false>, waves, opt, stats, nout, host_out, emit_to);
-      case 32:
-        return fx ? run(a, k_series_start, k_series_iterate<32, true>, waves, opt, stats, nout, host_out, emit_to)
-                  : run(a, k_series_start, k_series_iterate<32, false>, waves, opt, stats, nout, host_out, emit_to);
+      case 8: return fx ? go(k_series_iterate<8, true>) : go(k_series_iterate<8, false>);
+      case 12: return fx ? go(k_series_iterate<12, true>) : go(k_series_iterate<12, false>);
+      case 16: return fx ? go(k_series_iterate<16, true>) : go(k_series_iterate<16, false>);
+      case 32: return fx ? go(k_series_iterate<32, true>) : go(k_series_iterate<32, false>);
       case 64:  // FP64 only: an FP32X mantissa is renormalised at block ends
         if (fx) return NM_ERR_ARG;
-        return run(a, k_series_start, k_series_iterate<64, false>, waves, opt, stats, nout, host_out, emit_to);
+        return go(k_series_iterate<64, false>);
       default: return NM_ERR_ARG;
     }
   }
@@ -2294,6 +2425,13 @@ class EngineImpl : public Engine {
     a.zd_count = ref_zd_count;
     a.tol = opt.tol;
     a.ltol = std::log2(opt.tol);
+    a.fxs = rfxs;
+    a.Am = rAm;
+    a.Bm = rBm;
+    a.Cm = rCm;
+    a.Ae = rAe;
+    a.Be = rBe;
+    a.Ce = rCe;
   }
 
   int tile_skip(const std::vector<const SaFrameHost*>& frames, const RenderOpts& opt, TileSearch& ts) override {

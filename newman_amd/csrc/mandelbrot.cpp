@@ -293,6 +293,14 @@ void Mandelbrot::computeSeries() {
   ref_.A.assign(2 * (size_t)len, 0.0);
   ref_.B.assign(2 * (size_t)len, 0.0);
   ref_.C.assign(2 * (size_t)len, 0.0);
+  ref_.Am.assign(2 * (size_t)len, 0.0);
+  ref_.Bm.assign(2 * (size_t)len, 0.0);
+  ref_.Cm.assign(2 * (size_t)len, 0.0);
+  ref_.Ae.assign(2 * (size_t)len, kFxZeroExp);
+  ref_.Be.assign(2 * (size_t)len, kFxZeroExp);
+  ref_.Ce.assign(2 * (size_t)len, kFxZeroExp);
+  ref_.Am[0] = 0.5;
+  ref_.Ae[0] = 1;
   hpf Ar(L), Ai(L), Br(L), Bi(L), Cr(L), Ci(L), nAr(L), nAi(L), nBr(L), nBi(L), nCr(L), nCi(L), t(L);
   hpf one(2);
   hp_set_d(one, 1.0);
@@ -320,6 +328,15 @@ void Mandelbrot::computeSeries() {
     ref_.A[2 * i] = Ar.get_d(); ref_.A[2 * i + 1] = Ai.get_d();
     ref_.B[2 * i] = Br.get_d(); ref_.B[2 * i + 1] = Bi.get_d();
     ref_.C[2 * i] = Cr.get_d(); ref_.C[2 * i + 1] = Ci.get_d();
+    const hpf* hv[6] = {&Ar, &Ai, &Br, &Bi, &Cr, &Ci};
+    std::vector<double>* fm[3] = {&ref_.Am, &ref_.Bm, &ref_.Cm};
+    std::vector<int32_t>* fe[3] = {&ref_.Ae, &ref_.Be, &ref_.Ce};
+    for (int k = 0; k < 6; k++) {
+      long e = 0;
+      const double m = hv[k]->get_d_2exp(&e);
+      (*fm[k / 2])[2 * i + k % 2] = m;
+      (*fe[k / 2])[2 * i + k % 2] = m == 0.0 ? kFxZeroExp : (int32_t)e;
+    }
   }
 }
 
@@ -329,6 +346,14 @@ static double log2_sqmag(double re, double im) {
   long double m = r * r + i * i;
   if (m == 0) return -INFINITY;
   return (double)log2l(m);
+}
+
+// log2 |z|^2 of a floatexp pair (mantissas mr, mi with exponents er, ei)
+static double log2_sqmag_fx(double mr, int er, double mi, int ei) {
+  if (mr == 0.0 && mi == 0.0) return -INFINITY;
+  const int E = mr == 0.0 ? ei : mi == 0.0 ? er : std::max(er, ei);
+  const double r = std::ldexp(mr, std::max(er - E, -2100)), i = std::ldexp(mi, std::max(ei - E, -2100));
+  return std::log2(r * r + i * i) + 2.0 * E;
 }
 
 // Tables for the O(log N) per-sample stop-index search that reproduces the
@@ -342,6 +367,11 @@ static double log2_sqmag(double re, double im) {
 void Mandelbrot::buildThresholds() {
   const int len = ref_.len;
   const double ltol = std::log2(error_tolerance);
+  ref_.fxs = series_mode == SERIES_FLOATEXP ? 1 : 0;
+  if (ref_.fxs) {
+    buildThresholdsFx();
+    return;
+  }
   ref_.PM.assign(len, INFINITY);
   ref_.BX.assign(len, -INFINITY);
   ref_.BN.assign(len, INFINITY);
@@ -382,30 +412,66 @@ void Mandelbrot::buildThresholds() {
     ref_.BNs[i] = std::min(ref_.BNs[i + 1], special ? INFINITY : log2_sqmag(br, bi));
     ref_.CNs[i] = std::min(ref_.CNs[i + 1], special ? INFINITY : log2_sqmag(cr, ci));
   }
-  // the stop-search lookup: cells of the finite PM range, each holding the first
-  // index whose prefix minimum reaches the cell's lower edge
-  {
-    double pmax = -INFINITY, pmin = INFINITY;
-    for (int i = 1; i < len; i++)
-      if (std::isfinite(ref_.PM[i])) {
-        pmax = std::max(pmax, ref_.PM[i]);
-        pmin = std::min(pmin, ref_.PM[i]);
-      }
-    ref_.PMT.clear();
-    ref_.pmt_n = 0;
-    if (pmax > pmin) {
-      const int n = (int)std::min<long>(65536, 8L * len);
-      const double h = (pmax - pmin) / n;
-      ref_.pmt_base = pmin;
-      ref_.pmt_inv_h = 1.0 / h;
-      ref_.pmt_n = n;
-      ref_.PMT.assign((size_t)n + 1, len);
-      int i = len;  // first index with PM[i] <= v; non-increasing in v
-      for (int k = 0; k <= n; k++) {
-        const double v = pmin + k * h;
-        while (i > 1 && ref_.PM[i - 1] <= v) i--;
-        ref_.PMT[k] = i;
-      }
+  buildStopLookup();
+}
+
+// The tables of the floatexp series test.  Its rounding has no range limits,
+// so only the thresholds matter: T_i = log2 tol + log2 |B_i|^2 - log2 |C_i|^2
+// from the floatexp coefficients, -inf where B_i = 0 and C_i != 0 (the test
+// fires for every e != 0), +inf where C_i = 0 (it never fires); PM is their
+// prefix minimum, ST their sparse table of range minima and PMT the lookup over
+// the finite range of PM.  The magnitude tables (BX, BN, CX, CN, BNs, CNs) and
+// the special counts (SP) belong to the double test's certificate and are not
+// read in this mode.
+void Mandelbrot::buildThresholdsFx() {
+  const int len = ref_.len;
+  const double ltol = std::log2(error_tolerance);
+  ref_.T.assign(len, INFINITY);
+  ref_.PM.assign(len, INFINITY);
+  ref_.BX.assign(len, -INFINITY);
+  ref_.BN.assign(len, INFINITY);
+  ref_.CX.assign(len, -INFINITY);
+  ref_.CN.assign(len, INFINITY);
+  ref_.SP.assign(len, 0);
+  ref_.BNs.assign(len + 1, INFINITY);
+  ref_.CNs.assign(len + 1, INFINITY);
+  double pm = INFINITY;
+  for (int i = 1; i < len; i++) {
+    const double lb = log2_sqmag_fx(ref_.Bm[2 * i], ref_.Be[2 * i], ref_.Bm[2 * i + 1], ref_.Be[2 * i + 1]);
+    const double lc = log2_sqmag_fx(ref_.Cm[2 * i], ref_.Ce[2 * i], ref_.Cm[2 * i + 1], ref_.Ce[2 * i + 1]);
+    const double T = lc == -INFINITY ? INFINITY : lb == -INFINITY ? -INFINITY : ltol + lb - lc;
+    ref_.T[i] = T;
+    pm = std::min(pm, T);
+    ref_.PM[i] = pm;
+  }
+  buildStopLookup();
+}
+
+// the stop-search lookup (cells of the finite PM range, each holding the first
+// index whose prefix minimum reaches the cell's lower edge) and the sparse table
+// of range minima of T
+void Mandelbrot::buildStopLookup() {
+  const int len = ref_.len;
+  double pmax = -INFINITY, pmin = INFINITY;
+  for (int i = 1; i < len; i++)
+    if (std::isfinite(ref_.PM[i])) {
+      pmax = std::max(pmax, ref_.PM[i]);
+      pmin = std::min(pmin, ref_.PM[i]);
+    }
+  ref_.PMT.clear();
+  ref_.pmt_n = 0;
+  if (pmax > pmin) {
+    const int n = (int)std::min<long>(65536, 8L * len);
+    const double h = (pmax - pmin) / n;
+    ref_.pmt_base = pmin;
+    ref_.pmt_inv_h = 1.0 / h;
+    ref_.pmt_n = n;
+    ref_.PMT.assign((size_t)n + 1, len);
+    int i = len;  // first index with PM[i] <= v; non-increasing in v
+    for (int k = 0; k <= n; k++) {
+      const double v = pmin + k * h;
+      while (i > 1 && ref_.PM[i - 1] <= v) i--;
+      ref_.PMT[k] = i;
     }
   }
   int K = 1;
@@ -417,6 +483,18 @@ void Mandelbrot::buildThresholds() {
     for (int i = 0; i + (1 << k) <= len; i++)
       ref_.ST[(size_t)k * len + i] =
           std::min(ref_.ST[(size_t)(k - 1) * len + i], ref_.ST[(size_t)(k - 1) * len + i + (1 << (k - 1))]);
+}
+
+int Mandelbrot::setSeriesMode(int mode) {
+  if (mode != SERIES_DOUBLE && mode != SERIES_FLOATEXP) return NM_ERR_ARG;
+  if (mode == series_mode) return NM_OK;
+  series_mode = mode;
+  if (have_ref_) {  // the coefficients are kept in both forms: only the tables change
+    buildThresholds();
+    rendered_ = false;
+    ref_serial_ = next_serial();
+  }
+  return NM_OK;
 }
 
 // mandelbrot.cpp:261-267

@@ -39,6 +39,10 @@ class Mandelbrot {
   // reference's constant 1024, mandelbrot.cpp:58) and the delta arithmetic.
   double bailout = 1024.0;
   int delta_mode = DELTA_FP64;
+  // the series test and skip in the reference's doubles or in floatexp
+  // (SeriesMode); setSeriesMode rebuilds the tables of a precomputed reference
+  int series_mode = SERIES_DOUBLE;
+  int setSeriesMode(int mode);
   // Launch settings for later renders (0 = the engine's defaults): iterations
   // per block, persistent waves per SIMD, sample order (0 row-major, T = T x T
   // tiles) and HIP-graph replay of repeated passes.  autotuneFrames sets them.
@@ -201,6 +205,8 @@ class Mandelbrot {
   int computeOrbit(const HPComplex& X0, bool store);
   void computeSeries();
   void buildThresholds();
+  void buildThresholdsFx();
+  void buildStopLookup();
   std::string snapshot() const;
   int uploadReference();
 };

@@ -16,6 +16,7 @@
 #include <unistd.h>
 
 /* ------------------------------------------------------------------ state */
+#define FX_ZERO_E (-0x40000000) /* the exponent of a floatexp zero */
 struct orc_view {
   int nr, nc, N;
   double tol;     /* error_tolerance, mandelbrot.cpp:9 */
@@ -30,6 +31,11 @@ struct orc_view {
   double* Xn;         /* 2*(len+1) nearest doubles of X */
   double* Xlo;        /* 2*(len+1) nearest(X - Xn) */
   double *At, *Bt, *Ct; /* 2*len truncated A, B, C (mandelbrot.cpp:166-168) */
+  /* floatexp series (an extension, not the reference's arithmetic): per index and
+   * component the truncated 53-bit mantissa in [1/2, 1) and its exponent */
+  double *Am, *Bm, *Cm;
+  int *Ae, *Be, *Ce;
+  int fxs; /* 1: the series test and skip in floatexp */
   int winner;
 };
 
@@ -88,8 +94,12 @@ static void free_series(orc_view* v) {
   free(v->At);
   free(v->Bt);
   free(v->Ct);
+  free(v->Am); free(v->Bm); free(v->Cm);
+  free(v->Ae); free(v->Be); free(v->Ce);
   v->X = NULL;
   v->Xt = v->Xn = v->Xlo = v->At = v->Bt = v->Ct = NULL;
+  v->Am = v->Bm = v->Cm = NULL;
+  v->Ae = v->Be = v->Ce = NULL;
   v->len = 0;
   v->winner = -1;
 }
@@ -145,6 +155,7 @@ void orc_destroy(orc_view* v) {
 void orc_set_N(orc_view* v, int N) { v->N = N; }
 void orc_set_tol(orc_view* v, double tol) { v->tol = tol; }
 void orc_set_bailout(orc_view* v, double b) { v->bailout = b; }
+void orc_set_series_mode(orc_view* v, int fxs) { v->fxs = fxs != 0; }
 
 int orc_set_center(orc_view* v, const char* re, const char* im) {
   int a = ohp_set_str(&v->cre, re);
@@ -344,6 +355,13 @@ void orc_pixel_coord_d(orc_view* v, int r, int c, double* z) {
 typedef struct {
   double re, im;
 } lpc;
+typedef struct {
+  double m;
+  int e;
+} fxr;
+typedef struct {
+  fxr re, im;
+} fxc;
 static inline lpc lp_sq(lpc a) {
   lpc r = {a.re * a.re - a.im * a.im, 2.0 * a.re * a.im};
   return r;
@@ -502,6 +520,15 @@ static void compute_series(orc_view* v) {
   v->At = (double*)malloc(sizeof(double) * 2 * (size_t)len);
   v->Bt = (double*)malloc(sizeof(double) * 2 * (size_t)len);
   v->Ct = (double*)malloc(sizeof(double) * 2 * (size_t)len);
+  double** fm[3] = {&v->Am, &v->Bm, &v->Cm};
+  int** fe[3] = {&v->Ae, &v->Be, &v->Ce};
+  for (int k = 0; k < 3; k++) {
+    *fm[k] = (double*)calloc(2 * (size_t)len, sizeof(double));
+    *fe[k] = (int*)malloc(sizeof(int) * 2 * (size_t)len);
+    for (size_t j = 0; j < 2 * (size_t)len; j++) (*fe[k])[j] = FX_ZERO_E;
+  }
+  v->Am[0] = 0.5;
+  v->Ae[0] = 1;
   ohp Ar, Ai, Br, Bi, Cr, Ci, nAr, nAi, nBr, nBi, nCr, nCi, t, u;
   ohp* all[] = {&Ar, &Ai, &Br, &Bi, &Cr, &Ci, &nAr, &nAi, &nBr, &nBi, &nCr, &nCi, &t, &u};
   for (int k = 0; k < 14; k++) ohp_zero(all[k], n);
@@ -541,6 +568,13 @@ static void compute_series(orc_view* v) {
     v->At[2 * i] = ohp_get_d(&Ar); v->At[2 * i + 1] = ohp_get_d(&Ai);
     v->Bt[2 * i] = ohp_get_d(&Br); v->Bt[2 * i + 1] = ohp_get_d(&Bi);
     v->Ct[2 * i] = ohp_get_d(&Cr); v->Ct[2 * i + 1] = ohp_get_d(&Ci);
+    const ohp* hv[6] = {&Ar, &Ai, &Br, &Bi, &Cr, &Ci};
+    for (int k = 0; k < 6; k++) {
+      long e = 0;
+      double m = ohp_get_d_2exp(&e, hv[k]);
+      (*fm[k / 2])[2 * i + k % 2] = m;
+      (*fe[k / 2])[2 * i + k % 2] = m == 0.0 ? FX_ZERO_E : (int)e;
+    }
   }
   (void)u;
 }
@@ -627,7 +661,78 @@ void orc_get_series(const orc_view* v, double* Xt, double* Xn, double* A, double
   if (C) memcpy(C, v->Ct, sizeof(double) * ns);
 }
 
+/* the floatexp coefficients (mantissas and exponents, 2*len each) */
+void orc_get_series_fx(const orc_view* v, double* Am, int* Ae, double* Bm, int* Be, double* Cm, int* Ce) {
+  size_t ns = 2 * (size_t)v->len;
+  memcpy(Am, v->Am, sizeof(double) * ns); memcpy(Ae, v->Ae, sizeof(int) * ns);
+  memcpy(Bm, v->Bm, sizeof(double) * ns); memcpy(Be, v->Be, sizeof(int) * ns);
+  memcpy(Cm, v->Cm, sizeof(double) * ns); memcpy(Ce, v->Ce, sizeof(int) * ns);
+}
+
 /* --------------------------------------------------------- SA + HP path */
+/* Floatexp series (an extension): the reference's series test and skip
+ * (mandelbrot.cpp:138-142, 160-178) in the same operation order, each real held
+ * as m * 2^e with m = 0 (e = FX_ZERO_E) or 1/2 <= |m| < 1.  A product or sum
+ * rounds the mantissa once, to nearest, as a double operation on the scaled
+ * operands does: where the reference's doubles are normal numbers the results
+ * are the same, and nothing under- or overflows where they are not. */
+static fxr fxr_make(double d, int e) {
+  fxr r;
+  if (d == 0.0) {
+    r.m = 0.0;
+    r.e = FX_ZERO_E;
+  } else {
+    int k = 0;
+    r.m = frexp(d, &k);
+    r.e = e + k;
+  }
+  return r;
+}
+static fxr fxr_mul(fxr a, fxr b) {
+  if (a.m == 0.0 || b.m == 0.0) return fxr_make(0.0, 0);
+  return fxr_make(a.m * b.m, a.e + b.e);
+}
+static fxr fxr_add(fxr a, fxr b) {
+  if (a.m == 0.0) return b;
+  if (b.m == 0.0) return a;
+  if (a.e >= b.e) return fxr_make(a.m + ldexp(b.m, b.e - a.e), a.e);
+  return fxr_make(ldexp(a.m, a.e - b.e) + b.m, b.e);
+}
+static fxr fxr_neg(fxr a) {
+  a.m = -a.m;
+  return a;
+}
+/* a < b for a, b >= 0 */
+static int fxr_lt(fxr a, fxr b) {
+  if (b.m == 0.0) return 0;
+  if (a.m == 0.0) return 1;
+  return a.e < b.e || (a.e == b.e && a.m < b.m);
+}
+static double fxr_get(fxr a) { return a.m == 0.0 ? 0.0 : ldexp(a.m, a.e); }
+static fxc fxc_of(lpc z) {
+  fxc r = {fxr_make(z.re, 0), fxr_make(z.im, 0)};
+  return r;
+}
+static fxc fxc_mul(fxc a, fxc b) { /* complex.h:29-31 */
+  fxc r = {fxr_add(fxr_mul(a.re, b.re), fxr_neg(fxr_mul(a.im, b.im))),
+           fxr_add(fxr_mul(a.re, b.im), fxr_mul(a.im, b.re))};
+  return r;
+}
+static fxc fxc_sq(fxc a) { /* complex.h:19-21: (re^2 - im^2, 2 re im) */
+  fxr two = {0.5, 2};
+  fxc r = {fxr_add(fxr_mul(a.re, a.re), fxr_neg(fxr_mul(a.im, a.im))), fxr_mul(fxr_mul(two, a.re), a.im)};
+  return r;
+}
+static fxr fxc_sqmag(fxc a) { return fxr_add(fxr_mul(a.re, a.re), fxr_mul(a.im, a.im)); }
+static fxc fx_coef(const double* m, const int* e, int i) {
+  fxc r = {{m[2 * i], e[2 * i]}, {m[2 * i + 1], e[2 * i + 1]}};
+  return r;
+}
+/* isUnstable(B e^2, C e^3) in floatexp */
+static int is_unstable_fx(fxc b, fxc c, double tol) {
+  return fxr_lt(fxr_mul(fxc_sqmag(b), fxr_make(tol, 0)), fxc_sqmag(c));
+}
+
 /* mandelbrot.cpp:138-142 */
 static int is_unstable(lpc b, lpc c, double tol) {
   double bmag = b.re * b.re + b.im * b.im;
@@ -637,6 +742,15 @@ static int is_unstable(lpc b, lpc c, double tol) {
 
 static lpc sa_term(const orc_view* v, int i, lpc eps, lpc eps2, lpc eps3) {
   if (i == 0) return eps;
+  if (v->fxs) {
+    fxc e = fxc_of(eps), e2 = fxc_sq(e), e3 = fxc_mul(e, e2);
+    fxc a = fxc_mul(fx_coef(v->Am, v->Ae, i), e);
+    fxc b = fxc_mul(fx_coef(v->Bm, v->Be, i), e2);
+    fxc c = fxc_mul(fx_coef(v->Cm, v->Ce, i), e3);
+    fxc s = {fxr_add(fxr_add(a.re, b.re), c.re), fxr_add(fxr_add(a.im, b.im), c.im)};
+    lpc d = {fxr_get(s.re), fxr_get(s.im)};
+    return d;
+  }
   lpc a = {v->At[2 * i], v->At[2 * i + 1]};
   lpc b = {v->Bt[2 * i], v->Bt[2 * i + 1]};
   lpc c = {v->Ct[2 * i], v->Ct[2 * i + 1]};
@@ -651,6 +765,15 @@ static lpc sa_term(const orc_view* v, int i, lpc eps, lpc eps2, lpc eps3) {
 static int sa_scan(const orc_view* v, lpc eps, int* unstable_at) {
   lpc eps2 = lp_sq(eps), eps3 = lp_mul(eps, eps2);
   *unstable_at = 0;
+  if (v->fxs) {
+    fxc e = fxc_of(eps), e2 = fxc_sq(e), e3 = fxc_mul(e, e2);
+    for (int i = 1; i < v->len; i++)
+      if (is_unstable_fx(fxc_mul(fx_coef(v->Bm, v->Be, i), e2), fxc_mul(fx_coef(v->Cm, v->Ce, i), e3), v->tol)) {
+        *unstable_at = i;
+        return i - 3 < 1 ? 1 : i - 3;
+      }
+    return v->len;
+  }
   for (int i = 1; i < v->len; i++) {
     lpc a = {v->At[2 * i], v->At[2 * i + 1]};
     lpc b = {v->Bt[2 * i], v->Bt[2 * i + 1]};

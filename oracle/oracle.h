@@ -40,6 +40,10 @@ void orc_destroy(orc_view* v);
 void orc_set_N(orc_view* v, int N);
 void orc_set_tol(orc_view* v, double tol);
 void orc_set_bailout(orc_view* v, double bailout);
+/* 1: the series test and skip in floatexp (an extension; set before precompute
+ * or after it, the coefficients are kept in both forms) */
+void orc_set_series_mode(orc_view* v, int fxs);
+void orc_get_series_fx(const orc_view* v, double* Am, int* Ae, double* Bm, int* Be, double* Cm, int* Ce);
 int orc_set_center(orc_view* v, const char* re, const char* im); /* at the current precision */
 int orc_set_size(orc_view* v, const char* re, const char* im);
 int orc_set_width(orc_view* v, const char* width); /* sz = width / cols at the current precision */

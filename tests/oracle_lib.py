@@ -23,6 +23,8 @@ _SIGS = [
     ("orc_set_N", None, [_V, C.c_int]),
     ("orc_set_tol", None, [_V, C.c_double]),
     ("orc_set_bailout", None, [_V, C.c_double]),
+    ("orc_set_series_mode", None, [_V, C.c_int]),
+    ("orc_get_series_fx", None, [_V, _V, _V, _V, _V, _V, _V]),
     ("orc_set_center", C.c_int, [_V, C.c_char_p, C.c_char_p]),
     ("orc_set_size", C.c_int, [_V, C.c_char_p, C.c_char_p]),
     ("orc_set_width", C.c_int, [_V, C.c_char_p]),
@@ -133,6 +135,10 @@ class OView:
     def set_bailout(self, b):
         self.L.orc_set_bailout(self.v, b)
 
+    def set_series_mode(self, fxs):
+        """1: the series test and skip in floatexp (an extension of the reference)"""
+        self.L.orc_set_series_mode(self.v, int(fxs))
+
     def set_center(self, re, im):
         assert self.L.orc_set_center(self.v, _b(re), _b(im)) == 0
 
@@ -173,6 +179,15 @@ class OView:
         Cc = np.zeros(2 * n)
         self.L.orc_get_series(self.v, Xt.ctypes.data, Xn.ctypes.data, A.ctypes.data, B.ctypes.data, Cc.ctypes.data)
         return Xt, Xn, A, B, Cc
+
+    def series_fx(self):
+        """The floatexp series: (mantissas, exponents) of A, B and C, each 2 per index."""
+        n = self.ref_len()
+        out = []
+        for _ in range(3):
+            out += [np.zeros(2 * n), np.zeros(2 * n, dtype=np.int32)]
+        self.L.orc_get_series_fx(self.v, *[a.ctypes.data for a in out])
+        return [(out[2 * k], out[2 * k + 1]) for k in range(3)]
 
     def render_rows(self, rows, nthreads=0):
         rows = np.ascontiguousarray(rows, dtype=np.int32)
