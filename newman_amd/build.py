@@ -15,6 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnewman_amd.so")
+# test support: the host-side walk of the kernels' indexing (csrc/index_check.cpp)
+INDEX_CHECK_LIB = os.path.join(HERE, "libnm_index_check.so")
 ARCH = os.environ.get("NEWMAN_AMD_ARCH", "gfx950")
 
 HOST_SRCS = ["hpfloat.cpp", "mandelbrot.cpp", "palette.cpp", "video.cpp", "capi.cpp"]
@@ -53,6 +55,16 @@ def build_library(verbose=True):
     return LIB
 
 
+def build_index_check():
+    """The indexing checker: host code over csrc/sample_index.hpp, which the kernels
+    share; the HIP headers supply its host/device qualifiers."""
+    tmp = INDEX_CHECK_LIB + ".tmp"
+    _run(["g++"] + CXXFLAGS + ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-shared", "-o", tmp,
+                               os.path.join(CSRC, "index_check.cpp")])
+    os.replace(tmp, INDEX_CHECK_LIB)
+    return INDEX_CHECK_LIB
+
+
 def build_oracle():
     """The CPU oracle is test infrastructure (oracle/); built here for convenience."""
     _run(["make", "-B", "-C", os.path.join(ROOT, "oracle")])  # always recompiled, like the library
@@ -61,5 +73,6 @@ def build_oracle():
 
 if __name__ == "__main__":
     build_library()
+    build_index_check()
     if "--oracle" in sys.argv:
         build_oracle()

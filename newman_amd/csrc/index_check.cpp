@@ -1,0 +1,207 @@
+// index_check.cpp - a host-side walk of every thread of the render kernels'
+// launches through the same index code the kernels run (sample_index.hpp), with
+// every index checked against the length of the buffer it addresses (the
+// lengths the engine allocates: prepare_common, render_hw, render_sa).
+//
+// For a frame set (nframes frames of nr x nc in a sample order) it checks:
+//  * the start kernels (plain and series): each (workgroup, tile, j, thread) of
+//    the launch grid: the tile's frame span, the sample's frame (and that the
+//    one-frame shortcut names the sample's own frame), its position, the column /
+//    row table entries, the cardioid-mask word, the tile-skip entry, and the
+//    per-sample and per-tile stores;
+//  * k_emit: every sample's output element of the row-major grids, and that the
+//    valid samples cover every pixel of every frame exactly once;
+//  * the active-list chunks of both iterate kernels for a range of list
+//    lengths: every entry read below the length, each entry taken once, and the
+//    claim ranges partitioning the chunks.
+// Test support, not part of the render path: built as its own library and called
+// from the CPU tests.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "sample_index.hpp"
+
+namespace {
+
+struct Report {
+  unsigned long long violations = 0, checked = 0;
+  char* msg;
+  int cap;
+  void fail(const char* fmt, ...) {
+    if (violations++ == 0 && msg && cap > 0) {
+      va_list ap;
+      va_start(ap, fmt);
+      std::vsnprintf(msg, (size_t)cap, fmt, ap);
+      va_end(ap);
+    }
+  }
+  bool in(uint64_t idx, uint64_t len, const char* what, uint64_t gid) {
+    checked++;
+    if (idx < len) return true;
+    fail("%s index %llu >= length %llu (sample %llu)", what, (unsigned long long)idx, (unsigned long long)len,
+         (unsigned long long)gid);
+    return false;
+  }
+};
+
+// the slot caps of the engine's start and iterate launches
+constexpr uint64_t kStartSlots = 4096;
+
+}  // namespace
+
+extern "C" {
+
+// Returns the number of violations (0: every index in bounds, every pixel
+// covered once); msg gets the first.  mask: frames carry cardioid masks;
+// skip_T: tile side of per-tile series skips (0: none).  plant (the checker's
+// own negative controls, 0 for none): 1 moves the last frame's column tables
+// one entry on, 2 gives the frames one sample more than the sample order has,
+// 3 interleaves the plain chunks with one chunk too few.
+long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int mask, int skip_T, int plant,
+                         char* msg, int cap, unsigned long long* checked) {
+  using namespace nm;
+  Report R;
+  R.msg = msg;
+  R.cap = cap;
+  if (msg && cap > 0) msg[0] = 0;
+  if (nr <= 0 || nc <= 0 || nframes <= 0 || num_cu <= 0) return -1;
+  const uint64_t fs = frame_samples(nr, nc, tile);
+  if (fs == 0) return -1;
+  const uint64_t total64 = fs * (uint64_t)nframes;
+  if (total64 >= 0xFFFFF000ull) return -2;  // the engine refuses it (prepare_common)
+  const uint32_t total = (uint32_t)total64;
+  // the frame records and buffer lengths as render_hw / render_sa build them
+  const uint64_t mask_words = ((uint64_t)nr * nc + 31) / 32;
+  std::vector<DevFrame> df;
+  for (int k = 0; k < nframes; k++) {
+    DevFrame d = make_frame((uint64_t)k, nr, nc, tile, (uint32_t)fs);
+    d.tab_c = (uint32_t)((uint64_t)k * nc);
+    d.tab_r = (uint32_t)((uint64_t)k * nr);
+    d.mask_base = mask ? (uint64_t)k * mask_words * 32 : 0;
+    if (plant == 1 && k == nframes - 1) d.tab_c += 1;
+    if (plant == 2) d.sample_base = (uint32_t)((uint64_t)k * (fs + 1));
+    df.push_back(d);
+  }
+  const double inv_frame = 1.0 / (double)(plant == 2 ? fs + 1 : fs);
+  const uint64_t ncol = (uint64_t)nframes * nc, nrow = (uint64_t)nframes * nr;
+  const uint64_t nmask = mask ? (uint64_t)nframes * mask_words : 1;
+  const uint64_t nout = (uint64_t)nframes * nr * nc;
+  const uint32_t ntiles = start_tiles(total);
+  const uint32_t sk_ntc = skip_T ? (uint32_t)((nc + skip_T - 1) / skip_T) : 0;
+  const uint32_t sk_tpf = skip_T ? sk_ntc * (uint32_t)((nr + skip_T - 1) / skip_T) : 0;
+  const uint64_t nskip = (uint64_t)sk_tpf * nframes;
+  std::vector<uint8_t> seen(nout, 0);
+
+  // ---- the start kernels
+  const int grid = start_grid(total, num_cu, kStartSlots);
+  for (int b = 0; b < grid; b++)
+    for (uint32_t t = (uint32_t)b; t < ntiles; t += (uint32_t)grid) {
+      R.in(t, ntiles, "tile_count", t);
+      uint32_t f0, f1;
+      tile_frame_span(t, total, inv_frame, &f0, &f1);
+      R.in(f0, (uint64_t)nframes, "frames (tile span)", (uint64_t)t * kTile);
+      R.in(f1, (uint64_t)nframes, "frames (tile span)", (uint64_t)t * kTile);
+      for (int j = 0; j < kPerThread; j++)
+        for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+          const uint32_t gid = start_gid(t, j, tid);
+          if (gid >= total) continue;  // an idle lane: the kernels load and store nothing for it
+          const uint32_t fi = f0 == f1 ? f0 : frame_of(gid, inv_frame);
+          if (fi != gid / fs) R.fail("sample %u: frame %u, want %llu", gid, fi, (unsigned long long)(gid / fs));
+          if (frame_of(gid, inv_frame) != gid / fs) R.fail("sample %u: frame_of is not the quotient", gid);
+          if (!R.in(fi, (uint64_t)nframes, "frames", gid)) continue;
+          const DevFrame& f = df[fi];
+          if (gid < f.sample_base) {
+            R.fail("sample %u below its frame's base %u", gid, f.sample_base);
+            continue;
+          }
+          const uint32_t local = gid - f.sample_base;
+          R.in(local, fs, "frame-local sample", gid);
+          R.in(gid, total, "out/flag/state/state_i/r2", gid);
+          const Loc lc = locate(f, local);
+          if (!lc.valid) continue;  // tile padding: only its flag is written
+          R.in(lc.r, f.nr, "row", gid);
+          R.in(lc.c, f.nc, "column", gid);
+          R.in((uint64_t)f.tab_c + lc.c, ncol, "column tables (cre, crelo, epsre)", gid);
+          R.in((uint64_t)f.tab_r + lc.r, nrow, "row tables (cim, cimlo, epsim)", gid);
+          if (mask) R.in((f.mask_base + (uint64_t)lc.r * f.nc + lc.c) >> 5, nmask, "cardioid mask", gid);
+          if (skip_T)
+            R.in((uint64_t)fi * sk_tpf + (lc.r / (uint32_t)skip_T) * sk_ntc + lc.c / (uint32_t)skip_T, nskip,
+                 "tile skips", gid);
+        }
+    }
+
+  // ---- k_emit: sample order -> the row-major grids
+  for (uint32_t gid = 0; gid < total; gid++) {
+    const uint32_t fi = frame_of(gid, inv_frame);
+    if (!R.in(fi, (uint64_t)nframes, "frames (emit)", gid)) continue;
+    const DevFrame& f = df[fi];
+    const Loc lc = locate(f, gid - f.sample_base);
+    if (!lc.valid) continue;
+    const uint64_t o = f.out_base + (uint64_t)lc.r * f.nc + lc.c;
+    if (!R.in(o, nout, "grid", gid)) continue;
+    if (seen[o]++) R.fail("grid element %llu written twice (sample %u)", (unsigned long long)o, gid);
+  }
+  for (uint64_t o = 0; o < nout; o++)
+    if (!seen[o]) {
+      R.fail("grid element %llu written by no sample", (unsigned long long)o);
+      break;
+    }
+
+  // ---- the iterate kernels' chunks of an active list of n entries
+  std::vector<uint64_t> lens = {0, 1, 3, 4, 63, 64, 65, 255, 256, 257, 1000, 4095, 4097};
+  lens.push_back(total / 3);
+  lens.push_back(total > 0 ? total - 1 : 0);
+  lens.push_back(total);
+  std::vector<uint8_t> took;
+  for (uint64_t n64 : lens) {
+    if (n64 > total) continue;
+    const uint32_t n = (uint32_t)n64, nch = chunk_count(n);
+    // claim ranges: a partition of [0, nch)
+    uint32_t next = 0;
+    for (uint32_t h = 0; h < (uint32_t)kQueues; h++) {
+      const uint32_t lo = claim_lo(nch, h), hi = claim_lo(nch, h + 1);
+      if (lo != next || hi < lo) R.fail("claim range %u of %u chunks: [%u, %u) after %u", h, nch, lo, hi, next);
+      next = hi;
+    }
+    if (next != nch) R.fail("claim ranges end at %u of %u chunks", next, nch);
+    // plain: interleaved chunks
+    took.assign(n, 0);
+    const uint32_t nchp = plant == 3 && nch > 1 ? nch - 1 : nch;
+    for (uint32_t c = 0; c < nch; c++) {
+      uint32_t items = 0;
+      for (uint32_t l = 0; l < (uint32_t)kChunk; l++) items += chunk_pos(l, c, nchp) < n;
+      for (uint32_t l = 0; l < (uint32_t)kChunk; l++) {
+        const uint32_t p = chunk_pos(l, c, nchp);
+        if ((l < items) != (p < n)) R.fail("chunk %u of %u entries: the lanes with entries are not a prefix", c, n);
+        if (l < items && R.in(p, n, "active (plain chunks)", c) && took[p]++)
+          R.fail("active entry %u taken twice", p);
+      }
+    }
+    for (uint32_t p = 0; p < n; p++)
+      if (!took[p]) {
+        R.fail("active entry %u of %u taken by no chunk (plain)", p, n);
+        break;
+      }
+    // series: contiguous chunks
+    took.assign(n, 0);
+    for (uint32_t c = 0; c < nch; c++) {
+      const uint32_t len = chunk_len(c, n);
+      for (uint32_t l = 0; l < len; l++) {
+        const uint64_t p = (uint64_t)c * kChunk + l;
+        if (R.in(p, n, "active (series chunks)", c) && took[p]++) R.fail("active entry %llu taken twice", p);
+      }
+    }
+    for (uint32_t p = 0; p < n; p++)
+      if (!took[p]) {
+        R.fail("active entry %u of %u taken by no chunk (series)", p, n);
+        break;
+      }
+    if (chunk_len(0xffffffffu, n) != 0) R.fail("the no-chunk marker has entries");
+  }
+  if (checked) *checked = R.checked;
+  return (long long)R.violations;
+}
+
+}  // extern "C"

@@ -51,24 +51,13 @@
 
 #include "colour.hpp"
 #include "engine.hpp"
+#include "sample_index.hpp"
 #include "video.hpp"
 
 namespace nm {
 
 // ------------------------------------------------------------ device structs
-struct DevFrame {
-  uint32_t sample_base;  // first global sample id of this frame
-  uint32_t nr, nc;
-  uint32_t tab_c, tab_r;  // offsets into the column / row tables
-  int32_t card_mode;
-  uint64_t out_base;   // first output element (row-major grid of this frame)
-  uint64_t mask_base;  // first bit in the cardioid mask (row-major)
-  // sample order: tile == 0 is row-major; otherwise T x T tiles (row-major tile
-  // grid, edge tiles padded) of 8 x 8 micro-blocks, one wave per micro-block
-  uint32_t tile, ntc;     // T, tiles per frame row
-  uint32_t tshift, mshift;  // log2(T*T), log2(T/8)
-  double inv_nc, inv_ntc;
-};
+// (DevFrame, the sample orders and the start tiling: sample_index.hpp)
 
 enum CounterIdx {
   C_CARDIOID = 0,
@@ -81,10 +70,6 @@ enum CounterIdx {
   C_COUNT = 7
 };
 
-constexpr int kThreads = 256;           // workgroup size of every kernel
-constexpr int kPerThread = 4;           // samples per thread per start tile
-constexpr int kTile = kThreads * kPerThread;
-constexpr int kChunk = 64;              // active samples per chunk (one per lane)
 constexpr int kHead = 64;               // orbit padding past its end (>= max K)
 // orbit windows each iterate workgroup keeps in LDS: the head (where rebased
 // lanes live) and the tail (where samples that never rebased run out the
@@ -176,36 +161,6 @@ struct SeriesArgs {
 // ------------------------------------------------------------ device helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-// x / d for x < 2^32 with inv = 1.0 / d: (x + 1/2) / d is at least 1/(2d) away
-// from an integer and the product errs by at most (x / d) 2^-52 < 1/(2d), so the
-// truncated product is the exact quotient
-__device__ __forceinline__ uint32_t udiv(uint32_t x, double inv) { return (uint32_t)(((double)x + 0.5) * inv); }
-
-// every frame of a frame set has the same size: frame = sample id / frame size
-__device__ __forceinline__ uint32_t frame_of(uint32_t gid, double inv_frame) { return udiv(gid, inv_frame); }
-
-// the (row, column) of a frame-local sample id; padding samples of edge tiles
-// are not valid
-struct Loc {
-  uint32_t r, c;
-  bool valid;
-};
-__device__ __forceinline__ Loc locate(const DevFrame& f, uint32_t local) {
-  Loc l;
-  if (f.tile == 0) {
-    l.r = udiv(local, f.inv_nc);
-    l.c = local - l.r * f.nc;
-    l.valid = true;
-    return l;
-  }
-  const uint32_t t = local >> f.tshift, w = local & ((1u << f.tshift) - 1u);
-  const uint32_t tr = udiv(t, f.inv_ntc), tc = t - tr * f.ntc;
-  const uint32_t mb = w >> 6, in = w & 63u;
-  l.r = tr * f.tile + ((mb >> f.mshift) << 3) + (in >> 3);
-  l.c = tc * f.tile + ((mb & ((1u << f.mshift) - 1u)) << 3) + (in & 7u);
-  l.valid = l.r < f.nr && l.c < f.nc;
-  return l;
-}
 
 // log2 q for a positive normal q, within 3e-15 absolute (measured against a
 // long-double log2 on 5e7 points of [2^-20, 2^20]): q = m 2^E with m in
@@ -480,19 +435,19 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   __shared__ double2 ltab[kLog2Tab];
   load_log2tab(ltab);
   unsigned long long ncard = 0, nres = 0, executed = 0, credited = 0;
-  const uint32_t ntiles = (c.total + kTile - 1) / kTile;
+  const uint32_t ntiles = start_tiles(c.total);
   const int N = c.N;
   const int pre = min(a.pre_iter, N);
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t mine = 0;
     // a tile inside one frame reads its frame record through scalar loads
-    const uint32_t f0 = frame_of(tile * kTile, c.inv_frame);
-    const uint32_t f1 = frame_of(min(tile * kTile + kTile, c.total) - 1, c.inv_frame);
+    uint32_t f0, f1;
+    tile_frame_span(tile, c.total, c.inv_frame, &f0, &f1);
     const bool one_frame = f0 == f1;
     for (int j = 0; j < kPerThread; j++) {
       // every lane of the wave runs this body (no early continue), so the
       // pre-iteration below is wave-uniform control flow
-      const uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
+      const uint32_t gid = start_gid(tile, j, threadIdx.x);
       bool valid = gid < c.total;
       bool in = false;
       double cx = 0.0, cy = 0.0;
@@ -695,10 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint8_t* flag, const
 // work through cross-lane shuffles), NEXT (its decode loads in flight) and the
 // chunk after (its sample-id load in flight), so a refill reads registers only.
 // entries of consecutive chunk `chunk` (0 for kNoChunk)
-__device__ __forceinline__ uint32_t chunk_items(uint32_t chunk, uint32_t total) {
-  uint64_t b = (uint64_t)chunk * kChunk;
-  return b >= total ? 0u : (uint32_t)min((uint64_t)kChunk, (uint64_t)total - b);
-}
+__device__ __forceinline__ uint32_t chunk_items(uint32_t chunk, uint32_t total) { return chunk_len(chunk, total); }
 
 __device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t chunk, uint32_t total) {
   uint32_t n = chunk_items(chunk, total);
@@ -716,7 +668,6 @@ __device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t ch
 // time and the last ones ran nearly alone on their SIMDs; claiming lets fast
 // waves take more.  One atomic per 64 samples; counters sharing one cache line
 // had serialised the claims (~100 ns each).
-constexpr int kQueues = 8;
 constexpr int kQueueStride = 64;  // u32 between counters: 256 bytes
 constexpr int kPartWords = kPartSlots * C_COUNT + kQueues * kQueueStride / 2;  // u64 words of d_part
 constexpr uint32_t kNoChunk = 0xffffffffu;
@@ -736,8 +687,7 @@ struct ChunkClaim {
     for (int t = 0; t < kQueues; t++) {
       const uint32_t h = (home + (uint32_t)t) & (kQueues - 1);
       if (done & (1u << h)) continue;
-      const uint32_t lo = (uint32_t)((uint64_t)nch * h / kQueues);
-      const uint32_t hi = (uint32_t)((uint64_t)nch * (h + 1) / kQueues);
+      const uint32_t lo = claim_lo(nch, h), hi = claim_lo(nch, h + 1);
       uint32_t v = 0;
       if (lane_id() == 0) v = atomicAdd(&ctr[h * kQueueStride], 1u);
       v = __builtin_amdgcn_readfirstlane(v);
@@ -761,14 +711,11 @@ struct ChunkMap {
   uint32_t nch;  // chunks: ceil(n / 64)
   __device__ void init(uint32_t total) {
     n = total;
-    nch = (total + kChunk - 1) / kChunk;
+    nch = chunk_count(total);
   }
   // below 4 (16 nch) <= n + 63 < 2^32 for a real chunk (prepare_common caps
   // the sample count); kNoChunk is tested first
-  __device__ uint32_t pos(uint32_t c) const {
-    const uint32_t l = (uint32_t)lane_id();
-    return 4u * ((l >> 2) * nch + c) + (l & 3u);
-  }
+  __device__ uint32_t pos(uint32_t c) const { return chunk_pos((uint32_t)lane_id(), c, nch); }
   // the chunk's entries are the lanes below the returned count (pos grows with
   // the lane)
   __device__ uint32_t items(uint32_t c) const {
@@ -1207,11 +1154,11 @@ template <bool FXS>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_series_start(SeriesArgs a) {
   const Common& c = a.c;
   unsigned long long ncard = 0, nres = 0, skipped = 0, credited = 0;
-  const uint32_t ntiles = (c.total + kTile - 1) / kTile;
+  const uint32_t ntiles = start_tiles(c.total);
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t mine = 0;
     for (int j = 0; j < kPerThread; j++) {
-      uint32_t gid = tile * kTile + j * kThreads + threadIdx.x;
+      uint32_t gid = start_gid(tile, j, threadIdx.x);
       bool valid = gid < c.total;
       // phase 1: locate the sample, cardioid class, epsilon and log2 |e|^2
       bool in = false;
@@ -1540,7 +1487,7 @@ void k_series_iterate(SeriesArgs a) {
   __syncthreads();
   const double2* zhead = zlds + kLdsTail;  // rebased lanes read the orbit from its start
   ChunkClaim q;
-  q.init(c.part, (total + kChunk - 1) / kChunk);
+  q.init(c.part, chunk_count(total));
   const uint32_t c0 = q.next(), c1 = q.next();
   uint32_t ncur = chunk_items(c0, total), nnext = chunk_items(c1, total);
   uint32_t cur_pix = load_pix(c.active, c0, total), nxt_pix = load_pix(c.active, c1, total);
@@ -1791,9 +1738,7 @@ constexpr int iter_threads<SeriesArgs>() { return kSeriesIterThreads; }
 
 // workgroups of the tile kernels (start, scatter, finalize, emit)
 static int tile_grid(uint64_t total, int num_cu) {
-  const uint64_t ntiles = (total + kTile - 1) / kTile;
-  const uint64_t cap = std::min<uint64_t>((uint64_t)num_cu * 16, (uint64_t)(kSlotIter - kSlotStart));
-  return (int)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, cap));
+  return start_grid(total, num_cu, (uint64_t)(kSlotIter - kSlotStart));
 }
 
 // The kernels of one pass on `stream`: zero the counters (part: kPartWords
@@ -2165,34 +2110,6 @@ class EngineImpl : public Engine {
       stats->tile = emit_to ? opt.tile : 0;
     }
     return NM_OK;
-  }
-
-  // DevFrame for frame k of a set: sample order, table offsets, bases
-  static DevFrame make_frame(uint64_t k, int nr, int nc, int tile, uint32_t frame_size) {
-    DevFrame d{};
-    d.sample_base = (uint32_t)(k * frame_size);
-    d.nr = (uint32_t)nr;
-    d.nc = (uint32_t)nc;
-    d.out_base = k * (uint64_t)nr * nc;
-    d.inv_nc = 1.0 / (double)nc;
-    d.tile = (uint32_t)tile;
-    if (tile) {
-      d.ntc = (uint32_t)((nc + tile - 1) / tile);
-      d.inv_ntc = 1.0 / (double)d.ntc;
-      int lt = 0;
-      while ((1 << lt) < tile) lt++;
-      d.tshift = (uint32_t)(2 * lt);
-      d.mshift = (uint32_t)(lt - 3);
-    }
-    return d;
-  }
-
-  // samples per frame in the sample order (tiles pad the frame); 0 = bad tile
-  static uint64_t frame_samples(int nr, int nc, int tile) {
-    if (tile == 0) return (uint64_t)nr * nc;
-    if (tile != 8 && tile != 16 && tile != 32 && tile != 64 && tile != 128 && tile != 256) return 0;
-    uint64_t ntr = (uint64_t)(nr + tile - 1) / tile, ntc = (uint64_t)(nc + tile - 1) / tile;
-    return ntr * ntc * (uint64_t)tile * tile;
   }
 
   int render_hw(const std::vector<const HwFrameHost*>& frames, const RenderOpts& opt, EscapeValue* host_out,
@@ -2719,15 +2636,8 @@ int render_hw_noalloc(const HwFrameView& f, int N, double bailout, EscapeValue* 
     err = "allocation-free layout overflow";
     return NM_ERR_RANGE;
   }
-  DevFrame df{};
-  df.sample_base = 0;
-  df.nr = (uint32_t)f.nr;
-  df.nc = (uint32_t)f.nc;
-  df.tab_c = 0;
-  df.tab_r = 0;
+  DevFrame df = make_frame(0, f.nr, f.nc, 0, (uint32_t)n);
   df.card_mode = f.card_mode;
-  df.out_base = 0;
-  df.inv_nc = 1.0 / (double)f.nc;
   hipStream_t s = nullptr;  // the null stream: nothing is created
 #define NA_HIP(expr)                                               \
   do {                                                             \

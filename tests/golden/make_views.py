@@ -42,6 +42,7 @@ SPECS = {
     "c3_seed1": (1, 50000, "1e-200", 256, "boundary"),
     "c4_seed1": (1, 60000, "1e-40", 256, "boundary"),
     "c5_seed1": (1, 100000, "1e-100", 256, "boundary"),
+    "c3_detail1": (1, 50000, "1e-200", 256, "detail"),
     "c5_detail1": (1, 100000, "1e-100", 256, "detail"),
 }
 # the learned skip's training centres: the detail walk of seeds 2 ... 13
