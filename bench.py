@@ -77,26 +77,30 @@ def make_workload(name):
             return m.render_frames(widths, k, w, reps, copy_back=False)[1]
         return m, desc, FLOP_PER_PLAIN_STEP, step
     if name == "c3":
-        v = views["c3_seed1"]
+        v = views["c3_detail1"]
         m = nm.Mandelbrot(1440, 2560)
         m.set_view(v["N"], v["re"], v["im"], v["width"])
-        m.set_skip_mode(nm.SKIP_NONE, 64)
-        desc = {"workload": "precision variants: 2560x1440, width 1e-200, N 50000, bailout 1024, centre = zoom walk "
-                            "seed 1; no series (the reference's double-precision series test degenerates at this "
-                            "depth), FP64 deltas (variant: FP32X deltas)",
-                "samples": 2560 * 1440, "N": 50000, "width": "1e-200", "precision_bits": m.precision_bits()}
+        m.set_series_mode(nm.NM_SERIES_FLOATEXP)
+        desc = {"workload": "precision variants: 2560x1440, width 1e-200, N 50000, bailout 1024, tol 1e-10, centre = "
+                            "detail zoom walk seed 1; series test and skip in floatexp (the reference's double "
+                            "test degenerates at this depth: e^2 underflows), FP64 deltas (variant: FP32X deltas)",
+                "samples": 2560 * 1440, "N": 50000, "width": "1e-200", "precision_bits": m.precision_bits(),
+                "series": "floatexp"}
 
         def step(k, w, reps=1):
             return m.render(k, w, reps, copy_back=False)
         return m, desc, FLOP_PER_PERT_STEP, step
     if name == "c5":
-        v = views["c5_seed1"]
+        v = views["c5_detail1"]
         m = nm.Mandelbrot(2160, 3840)
         m.set_view(v["N"], v["re"], v["im"], v["width"])
+        m.set_series_mode(nm.NM_SERIES_FLOATEXP)
         desc = {"workload": "learned-skip configuration: 3840x2160, width 1e-100, N 100000, bailout 1024, tol 1e-10, "
-                            "centre = zoom walk seed 1; the FP64 per-sample path (variants: FP32X deltas, 64x64 "
-                            "tile skips from the probe search and from the learned predictor)",
-                "samples": 3840 * 2160, "N": 100000, "width": "1e-100", "precision_bits": m.precision_bits()}
+                            "centre = detail zoom walk seed 1; series test and skip in floatexp; the FP64 "
+                            "per-sample path (variants: FP32X deltas, 64x64 tile skips from the probe search and "
+                            "from the learned predictor trained on the detail walks of seeds 2-11)",
+                "samples": 3840 * 2160, "N": 100000, "width": "1e-100", "precision_bits": m.precision_bits(),
+                "series": "floatexp"}
 
         def step(k, w, reps=1):
             return m.render(k, w, reps, copy_back=False)
@@ -197,7 +201,56 @@ def train_skipnet(nm, m, T, factors, epochs=120):
                  "loss_last": float(curve[-1]), "train_s": time.perf_counter() - t0}
 
 
-def variants(m, step, args, value64, wall64):
+def seeded_views(nm, m, names, T):
+    """Per named zoom-walk view (this frame size, N and series mode, its own
+    findProbe reference): the tile features and the probe searches' targets."""
+    views = load_views()
+    out = []
+    for name in names:
+        v = views[name]
+        t = nm.Mandelbrot(m.rows(), m.cols())
+        t.set_view(int(v["N"]), v["re"], v["im"], v["width"])
+        t.set_series_mode(nm.NM_SERIES_FLOATEXP)
+        t.precompute()
+        ua, _, _ = t.tile_skips(T, mode=0)
+        out.append((name, t, ua, t.tile_features(T), t.tile_targets(ua)))
+    return out
+
+
+TRAIN_SEEDS = list(range(2, 12))   # detail walks c5_train2 .. c5_train11
+HELD_OUT_SEEDS = [12, 13]
+
+
+def train_skipnet_seeded(nm, m, T, epochs=120):
+    """The learned tile skip's network trained on the GPU on the probe searches of
+    the detail zoom walks of seeds 2-11 (3840x2160 at width 1e-100, 2040 tiles
+    each), and its acceptance on held-out views: seeds 12 and 13 and this view
+    (seed 1), none of them seen in training."""
+    t0 = time.perf_counter()
+    train = seeded_views(nm, m, ["c5_train%d" % s for s in TRAIN_SEEDS], T)
+    t_data = time.perf_counter() - t0
+    x = np.concatenate([r[3] for r in train])
+    t = np.concatenate([r[4] for r in train])
+    net = nm.SkipNet()
+    t0 = time.perf_counter()
+    curve = net.train(x, t, epochs=epochs, batch=256, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    held = [(name, tv, ua) for name, tv, ua, _, _ in seeded_views(nm, m, ["c5_train%d" % s for s in HELD_OUT_SEEDS],
+                                                                    T)]
+    held.append(("c5_detail1", m, m.tile_skips(T, mode=0)[0]))
+    acc = {}
+    for name, tv, ua in held:
+        ul, how, st = tv.tile_skips(T, mode=2, net=net)
+        scan = tv.tile_skips(T, mode=0)[2]
+        acc[name] = {"tiles": int(len(how)), "accepted": float((how == 2).mean()),
+                     "equal_to_scan": bool(np.array_equal(ul, ua)), "exact_tests_learned": int(st["evals"]),
+                     "exact_tests_scan": int(scan["evals"])}
+    return net, {"samples": int(len(t)), "views": len(train), "seeds": TRAIN_SEEDS, "epochs": epochs,
+                 "loss_first": float(curve[0]), "loss_last": float(curve[-1]), "train_s": t_train,
+                 "data_s": t_data, "held_out": acc}
+
+
+def variants(m, step, args, value64, wall64, seeded=False):
     """The same frame with FP32X deltas, with per-tile skips (64 x 64 tiles) from
     the plain probe search (the reference's scan at each tile's probe) and from
     the learned predictor, and with FP32X + learned skips: credited px-it/s, its
@@ -207,7 +260,10 @@ def variants(m, step, args, value64, wall64):
     k, w = args.block_k, args.waves
     m.render(k, w, 1, copy_back=True)
     base = m.grid()["iterations"].copy()
-    net, train = train_skipnet(nm, m, T, [0.3, 0.5, 0.7, 1.5, 2.2, 3.3])
+    if seeded:
+        net, train = train_skipnet_seeded(nm, m, T)
+    else:
+        net, train = train_skipnet(nm, m, T, [0.3, 0.5, 0.7, 1.5, 2.2, 3.3])
     out = {"training": train}
     for name, delta, skip in (("fp32x", 1, 0), ("fp64_tile_scan", 0, 1), ("fp64_learned_skip", 0, 3),
                               ("fp32x_learned_skip", 1, 3)):
@@ -383,38 +439,40 @@ def cpu_baseline(desc, name, nthreads, rows_every):
                 "single_thread": _one_thread(one, "the full frame 4 times")}
     views = load_views()
     if name == "c3":
-        v = views["c3_seed1"]
+        v = views["c3_detail1"]
         nr, nc = 1440, 2560
         o = O.OView(nr, nc)
         o.set_view(v["N"], v["re"], v["im"], v["width"])
+        o.set_series_mode(1)
         t0 = time.perf_counter()
         o.precompute(nthreads)
         t_pre = time.perf_counter() - t0
         rows = np.arange(rows_every // 2, nr, rows_every * 5, dtype=np.int32)
         rc = np.array([(r, c) for r in rows for c in range(nc)], dtype=np.int32)
         t0 = time.perf_counter()
-        out = o.pert_samples_ua(rc, np.ones(len(rc), dtype=np.int32), nthreads)
+        out = o.render_samples(rc, nthreads, pert=True)
         t_run = time.perf_counter() - t0
         credited = int(out["iterations"].astype(np.int64).sum())
         rc1 = np.array([(r, c) for r in rows[::4] for c in range(nc)], dtype=np.int32)
 
         def one():
-            o1 = o.pert_samples_ua(rc1, np.ones(len(rc1), dtype=np.int32), 1)
+            o1 = o.render_samples(rc1, 1, pert=True)
             return int(o1["iterations"].astype(np.int64).sum()), o1.size
         return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
-                "sample": "oracle FP64 perturbation without series (the method the GPU runs here) on rows %d::%d of "
-                          "the 2560x1440 frame (%d samples, %.3g px-it, %.1f s); precompute %.1f s, not included"
-                          % (rows_every // 2, rows_every * 5, out.size, credited, t_run, t_pre),
+                "sample": "oracle floatexp series start + FP64 perturbation (the method the GPU runs here) on rows "
+                          "%d::%d of the 2560x1440 frame (%d samples, %.3g credited px-it, %.1f s); precompute "
+                          "%.1f s, not included" % (rows_every // 2, rows_every * 5, out.size, credited, t_run, t_pre),
                 "single_thread": _one_thread(one, "rows %d::%d" % (rows_every // 2, rows_every * 20))}
     if name == "c5":
-        v = views["c5_seed1"]
+        v = views["c5_detail1"]
         nr, nc = 2160, 3840
         o = O.OView(nr, nc)
         o.set_view(v["N"], v["re"], v["im"], v["width"])
+        o.set_series_mode(1)
         t0 = time.perf_counter()
         o.precompute(nthreads)
         t_pre = time.perf_counter() - t0
-        rows = np.arange(rows_every // 2, nr, rows_every * 4, dtype=np.int32)
+        rows = np.arange(rows_every // 2, nr, rows_every * 15, dtype=np.int32)
         t0 = time.perf_counter()
         out = o.render_rows(rows, nthreads)
         t_run = time.perf_counter() - t0
@@ -424,10 +482,11 @@ def cpu_baseline(desc, name, nthreads, rows_every):
             o1 = o.render_rows(rows[::2], 1)
             return int(o1["iterations"].astype(np.int64).sum()), o1.size
         return {"value": credited / t_run, "unit": "pixel-iterations/s", "cores": nthreads, "kind": "port",
-                "sample": "oracle reference method (series scan + HP continuation, %d-bit) on rows %d::%d of the "
-                          "3840x2160 frame (%d samples, %.3g credited px-it, %.1f s); precompute %.1f s, not "
-                          "included" % (o.bits(), rows_every // 2, rows_every * 4, out.size, credited, t_run, t_pre),
-                "single_thread": _one_thread(one, "rows %d::%d" % (rows_every // 2, rows_every * 8))}
+                "sample": "oracle reference method (series scan in floatexp + HP continuation, %d-bit) on rows "
+                          "%d::%d of the 3840x2160 frame (%d samples, %.3g credited px-it, %.1f s); precompute "
+                          "%.1f s, not included" % (o.bits(), rows_every // 2, rows_every * 15, out.size, credited,
+                                                    t_run, t_pre),
+                "single_thread": _one_thread(one, "rows %d::%d" % (rows_every // 2, rows_every * 30))}
     if name == "c4":
         v = views["c4_seed1"]
         nr, nc = 720, 1280
@@ -600,7 +659,7 @@ def measure(config, args, block_k=None, waves=None):
         line["colour"] = colour_leg(m, 4)
         line["video"] = video_leg(m, 15)
     if args.variants and config in ("c2", "c5"):
-        line["variants"] = variants(m, step, args, value, wall)
+        line["variants"] = variants(m, step, args, value, wall, seeded=config == "c5")
     if args.variants and config == "c3":
         line["variants"] = fp32x_variant(m, step, args, value)
     if not args.no_cpu_baseline:

@@ -398,31 +398,37 @@ __device__ __forceinline__ int cardioid_box_filter(double x, double y) {
 }
 
 // Pre-iteration of the start kernel: a lane's state, its own flag and the
-// wave's scalar mask of lanes still iterating.
+// wave's scalar mask of lanes still iterating.  A lane that is not iterating
+// holds z = c = 0, which stays 0 and never passes the escape test, so the test
+// of a step needs no mask: a lane leaves at its first escape by recording it and
+// parking at 0.
 struct PreLane {
-  double &x, &y, &r2, &i2;
-  const double cx, cy;
+  double &x, &y, &r2, &i2, &cx, &cy;
   unsigned long long& live;
   bool& alive;
   int& kesc;      // update whose test first held (-1: none yet)
   double& mgesc;  // |z|^2 there
 };
 
-// Updates k0 + J .. k0 + 3 of a lockstep block of four (unrolled by recursion);
-// a lane leaves the live mask at its first escape, which it records.
+__device__ __forceinline__ void park(double& x, double& y, double& r2, double& i2, double& cx, double& cy) {
+  x = y = r2 = i2 = cx = cy = 0.0;
+}
+
+// Updates k0 + J .. k0 + 3 of a lockstep block of four (unrolled by recursion).
 template <int J, bool FMA2>
 __device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int k0) {
   if constexpr (J < 4) {
     plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
     const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
     const bool e = mg > b2;
-    const unsigned long long esc = __ballot(e) & L.live;
+    const unsigned long long esc = __ballot(e);
     if (__builtin_expect(esc != 0, 0)) {
       L.live &= ~esc;
-      if (L.alive && e) {
+      if (e) {
         L.kesc = k0 + J;
         L.mgesc = mg;
         L.alive = false;
+        park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
       }
     }
     pre_block<J + 1, FMA2>(L, b2, k0);
@@ -471,12 +477,13 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
       }
       // z <- sq(z) + z0 in the reference's operation order (complex.h:19-27), in
       // lockstep blocks of four updates while some lane of the wave iterates
-      double x = cx, y = cy, r2 = x * x, i2 = y * y, mgesc = 0.0;
       int kesc = -1;
       bool alive = valid && !in;
+      double zx = alive ? cx : 0.0, zy = alive ? cy : 0.0;  // lanes not iterating hold z = c = 0
+      double x = zx, y = zy, r2 = x * x, i2 = y * y, mgesc = 0.0;
       unsigned long long live = __ballot(alive);
       int kend = 0;
-      PreLane L{x, y, r2, i2, cx, cy, live, alive, kesc, mgesc};
+      PreLane L{x, y, r2, i2, zx, zy, live, alive, kesc, mgesc};
       while (live != 0 && kend < pre) {
         pre_block<0, FMA2>(L, c.b2, kend);
         kend += 4;
@@ -729,32 +736,31 @@ struct ChunkMap {
 
 // A lane's iteration state inside a block of the plain iterate kernel; `live`
 // is the wave's scalar mask of lanes still iterating, `act` the lane's own flag.
+// A lane that is not iterating a sample holds z = c = 0 (parked), so the escape
+// test of a step needs no mask (one compare and one branch on its result).
 struct PlainLane {
-  double &x, &y, &r2, &i2;
-  const double cx, cy;
-  unsigned long long& live;
+  double &x, &y, &r2, &i2, &cx, &cy;
   bool& act;
   int& jesc;  // update of the block at which the lane escaped (-1: none)
 };
 
 // Updates J .. K - 1 of a block (unrolled by recursion).  The handler of an
 // escape writes the count and |z|^2 (finalize turns them into smoothing; the
-// block's end replaces a count at or beyond N by N) and leaves the live mask;
-// the no-escape case falls through.
+// block's end replaces a count at or beyond N by N) and parks the lane; the
+// no-escape case falls through.
 template <int J, int K, bool FMA2>
 __device__ __forceinline__ void plain_block(PlainLane& L, const Common& c, const uint32_t pix, const int k0) {
   if constexpr (J < K) {
     plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
     const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
     const bool e = mg > c.b2;
-    const unsigned long long esc = __ballot(e) & L.live;
-    if (__builtin_expect(esc != 0, 0)) {
-      L.live &= ~esc;
-      if (L.act && e) {
+    if (__builtin_expect(__ballot(e) != 0, 0)) {
+      if (e) {
         c.out[pix] = EscapeValue{k0 + J, 0.0f};
         c.r2[pix] = mg;  // smoothing: k_finalize
         L.jesc = J;
         L.act = false;
+        park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
       }
     }
     plain_block<J + 1, K, FMA2>(L, c, pix, k0);
@@ -824,7 +830,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     const int k0 = k;
     const bool was = act;
     int jesc = -1;
-    PlainLane L{x, y, r2, i2, cx, cy, live, act, jesc};
+    PlainLane L{x, y, r2, i2, cx, cy, act, jesc};
     plain_block<0, K, FMA2>(L, c, pix, k0);
     if (was) {
       // the reference tests k >= N before each update: an escape at k0 + j
@@ -833,6 +839,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
       if (k0 + ran > N || (jesc < 0 && k0 + K == N)) {
         c.out[pix] = EscapeValue{N, 0.0f};
         act = false;
+        park(x, y, r2, i2, cx, cy);
       }
       executed += (unsigned long long)min(ran, N - k0);
       k = k0 + K;

@@ -10,8 +10,13 @@
   bit) and against the reference method (series scan + HP continuation) from the
   same reference on >= 99.99%; the viewer's per-keyframe findProbe
   (viewer.cpp:255-294) is reported beside it.
-* Learned skip (3840x2160, width 1e-100, N 100000, 64x64 tiles): the learned
-  tile skip against the oracle's GPU method with the same per-tile skips.
+* Precision variants (2560x1440, width 1e-200, N 50000, the detail walk's
+  view, series in floatexp): every 288th row against the oracle's floatexp
+  restatement of the GPU method (bit for bit), with FP64 and with FP32X deltas;
+  the FP32X counts' agreement with FP64 is reported.
+* Learned skip (3840x2160, width 1e-100, N 100000, the detail walk's view,
+  series in floatexp, 64x64 tiles): the learned tile skip against the oracle's
+  GPU method with the same per-tile skips.
 * View files (viewer.cpp:6-29, mandelbrot.cpp:19-35): a saved view loaded back
   by loadLegacy renders as the oracle's loadLegacy of the same file.
 """
@@ -148,12 +153,47 @@ def test_zoom_video_batch_frames(gpu):
         k, int((grids[0][rows[:1]].reshape(-1)["iterations"] != own["iterations"]).sum()), own.size))
 
 
+def test_precision_config_view(gpu):
+    nm = gpu
+    v = _VIEWS["c3_detail1"]
+    nr, nc = 1440, 2560
+    m = nm.Mandelbrot(nr, nc)
+    m.set_view(v["N"], v["re"], v["im"], v["width"])
+    m.set_series_mode(nm.NM_SERIES_FLOATEXP)
+    m.precompute()
+    m.render()
+    g64 = m.grid().copy()
+    m.set_delta_mode(nm.NM_DELTA_FP32X)
+    st = m.render()
+    g32 = m.grid().copy()
+    o = O.OView(nr, nc)
+    o.set_view(v["N"], v["re"], v["im"], v["width"])
+    re, im = m.reference_point()
+    o.precompute_at(re, im)
+    o.set_series_mode(1)
+    rows = np.arange(144, nr, 288)
+    rc = _rc(rows, nc)
+    meth = o.render_samples(rc, THREADS, pert=True)
+    mine = g64[rows].reshape(-1)
+    assert np.array_equal(mine["iterations"], meth["iterations"])
+    assert _smooth_ok(mine["smoothing"], meth["smoothing"]).all()
+    fx = o.pert_samples_fx(rc, st["block_k"], None, THREADS)
+    m32 = g32[rows].reshape(-1)
+    assert np.array_equal(m32["iterations"], fx["iterations"])
+    assert _smooth_ok(m32["smoothing"], fx["smoothing"]).all()
+    same = g32["iterations"] == g64["iterations"]
+    print("2560x1440 at 1e-200: %d distinct counts; FP32X counts equal to FP64 on %.4f%% of samples"
+          % (len(np.unique(g64["iterations"])), 100 * same.mean()))
+    assert len(np.unique(g64["iterations"])) > 100
+
+
 def test_learned_skip_config_view(gpu):
     nm = gpu
-    v = _VIEWS["c5_seed1"]
+    v = _VIEWS["c5_detail1"]
     nr, nc, T = 2160, 3840, 64
     m = nm.Mandelbrot(nr, nc)
     m.set_view(v["N"], v["re"], v["im"], v["width"])
+    m.set_series_mode(nm.NM_SERIES_FLOATEXP)
     m.precompute()
     # a network trained on this view's own probe searches (every prediction is
     # validated, so the grid does not depend on how well it fits)
@@ -169,6 +209,7 @@ def test_learned_skip_config_view(gpu):
     o.set_view(v["N"], v["re"], v["im"], v["width"])
     re, im = m.reference_point()
     o.precompute_at(re, im)
+    o.set_series_mode(1)
     tua, _ = o.tile_probe_ua(T)
     assert np.array_equal(tua.ravel(), ua)
     rows = np.arange(135, nr, 270)
