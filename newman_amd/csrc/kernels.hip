@@ -1297,10 +1297,11 @@ __device__ __forceinline__ double2 sample_eps(const SeriesArgs& a, uint32_t gid)
   return make_double2(a.epsre[f.tab_c + lc.c], a.epsim[f.tab_r + lc.r]);
 }
 
-// an orbit entry as one 16-byte vector, and pointers to the orbit in LDS and HBM
+// an orbit entry as one 16-byte vector, and pointers to the orbit: in LDS (the
+// whole wave), or per lane in LDS or HBM (generic addresses, flat loads)
 typedef double ld_double2 __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(3))) const ld_double2* lds_orbit;  // the orbit in LDS
-typedef __attribute__((address_space(1))) const ld_double2* hbm_orbit;  // the orbit in HBM
+typedef __attribute__((address_space(3))) const ld_double2* lds_orbit;
+typedef const ld_double2* gen_orbit;
 
 // p - j orbit entries, as an address (the result may point before the array; it
 // is only read at offsets j' > j)
@@ -1316,10 +1317,13 @@ __device__ __forceinline__ int orbit_index(P p, P head) {
 // One block of K perturbation steps for a lane (Y_it = Zd[m] + d):
 //   d' = d (Zd[m] + Y) + e, rebase to the orbit start (m = 0, d = Y) when
 //   |Y| < |d| (or, in END blocks, when m reaches the end of the reference).
-// Step j reads its orbit entry at zp[j], zp = the lane's Zd[m0] in LDS or HBM
-// (one memory space per wave and block); a rebase in step j moves zp to
-// head - j, so step j' > j reads Zd[j' - j].  Each entry is loaded one step
-// ahead, as soon as that step's rebase is decided.
+// Step j reads its orbit entry at zp[j], zp = the lane's Zd[m0]: in LDS when
+// every lane's block lies in an LDS window, else per lane in a window or in HBM
+// (a flat load: only the lanes reading HBM cost the L1 a line lookup, which
+// limits the kernel when a wave's lanes are spread over the orbit); a rebase in
+// step j moves zp to head - j (the LDS head window), so step j' > j reads
+// Zd[j' - j].  Each entry is loaded one step ahead, as soon as that step's
+// rebase is decided.
 // The first escape (exact sqMag > bailout^2, mandelbrot.cpp:60) records its
 // step and |Y|^2: the step whose test holds while no earlier step's of the block
 // did (a flag in a scalar mask: escape need not be absorbing, |Y| can fall back
@@ -1586,13 +1590,16 @@ void k_series_iterate(SeriesArgs a) {
           pert_block_fx<K, false>(wr, wi, ex, esr, esi, er, ei, m, it, zp, lhead, 0, mend, c.b2, b2g, esc, esc_r2,
                                   rb);
       } else {
-        const hbm_orbit ghead = (hbm_orbit)(const void*)a.Zd;
+        const gen_orbit fhead = (gen_orbit)(const void*)zhead;
+        const gen_orbit fz = in_head ? fhead + m
+                                     : in_tail ? (gen_orbit)(const void*)zlds + (m - tail_lo)
+                                               : (gen_orbit)(const void*)a.Zd + m;
         if (end_zone)
-          pert_block_fx<K, true>(wr, wi, ex, esr, esi, er, ei, m, it, ghead + m, ghead, mend - m, mend, c.b2, b2g,
-                                 esc, esc_r2, rb);
+          pert_block_fx<K, true>(wr, wi, ex, esr, esi, er, ei, m, it, fz, fhead, mend - m, mend, c.b2, b2g, esc,
+                                 esc_r2, rb);
         else
-          pert_block_fx<K, false>(wr, wi, ex, esr, esi, er, ei, m, it, ghead + m, ghead, 0, mend, c.b2, b2g, esc,
-                                  esc_r2, rb);
+          pert_block_fx<K, false>(wr, wi, ex, esr, esi, er, ei, m, it, fz, fhead, 0, mend, c.b2, b2g, esc, esc_r2,
+                                  rb);
       }
       // keep the mantissa near 1: |w| grows by at most |W| per step
       const float mx = fmaxf(fabsf(wr), fabsf(wi));
@@ -1617,13 +1624,18 @@ void k_series_iterate(SeriesArgs a) {
         kdone = pert_block<K, false>(dr, di, er, ei, m, it, zp, lhead, 0, mend, c.b2, !act, can_refill, esc, esc_r2,
                                      rb);
     } else {
-      // the orbit from HBM (L2) for the whole wave
-      const hbm_orbit ghead = (hbm_orbit)(const void*)a.Zd;
+      // per lane: an LDS window where the lane's block lies in one, else HBM
+      // (generic addresses: one flat load per step serves both spaces, and only
+      // the lanes reading HBM cost the L1 a line lookup)
+      const gen_orbit fhead = (gen_orbit)(const void*)zhead;
+      const gen_orbit fz = in_head ? fhead + m
+                                   : in_tail ? (gen_orbit)(const void*)zlds + (m - tail_lo)
+                                             : (gen_orbit)(const void*)a.Zd + m;
       if (end_zone)
-        kdone = pert_block<K, true>(dr, di, er, ei, m, it, ghead + m, ghead, mend - m, mend, c.b2, !act, can_refill,
-                                    esc, esc_r2, rb);
+        kdone = pert_block<K, true>(dr, di, er, ei, m, it, fz, fhead, mend - m, mend, c.b2, !act, can_refill, esc,
+                                    esc_r2, rb);
       else
-        kdone = pert_block<K, false>(dr, di, er, ei, m, it, ghead + m, ghead, 0, mend, c.b2, !act, can_refill, esc,
+        kdone = pert_block<K, false>(dr, di, er, ei, m, it, fz, fhead, 0, mend, c.b2, !act, can_refill, esc,
                                      esc_r2, rb);
     }
     if (act) {
