@@ -45,9 +45,10 @@ SPECS = {
     "c3_detail1": (1, 50000, "1e-200", 256, "detail"),
     "c5_detail1": (1, 100000, "1e-100", 256, "detail"),
 }
-# the learned skip's training centres: the detail walk of seeds 2 ... 13
+# the learned skip's training centres: the detail walk of seeds 2 ... 13 (at
+# 64 x 64 renders, 16 times cheaper than the configurations' 256 x 256)
 for _s in range(2, 14):
-    SPECS["c5_train%d" % _s] = (_s, 100000, "1e-100", 128, "detail")
+    SPECS["c5_train%d" % _s] = (_s, 100000, "1e-100", 64, "detail")
 
 _M64 = 2**64 - 1
 

@@ -150,3 +150,38 @@ def test_floatexp_stop_index_is_the_exact_test():
             assert abs(ratio - 1) < 1e-12, (r, c, ua, first)
         fx_found += int(ua != 0)
     assert fx_found > 0
+
+
+def test_floatexp_counts_equal_direct_iteration_where_the_double_test_degenerates():
+    """Width 1e-200 (the detail walk's view, N 50000), 6 x 8 samples: with the
+    double series every sample skips to the orbit's end (e^2 underflows, no index
+    tests unstable), so every count is N; with the floatexp series the reference
+    method (skip + HP continuation) and the GPU method (skip + FP64 perturbation)
+    give, on the first 12 samples, the counts of direct iteration from z = c at
+    750 bits."""
+    if "c3_detail1" not in _views():
+        pytest.skip("c3_detail1 is not in views.json")
+    nr, nc = 6, 8
+    o, v = _view("c3_detail1", nr, nc)
+    o.precompute(4)
+    rc = np.array([(r, c) for r in range(nr) for c in range(nc)], dtype=np.int32)
+    dbl = o.render_samples(rc, 4, pert=True)
+    assert (dbl["iterations"] == v["N"]).all() and o.sa_start(1, 1)[0] == 0
+    o.set_series_mode(1)
+    ref = o.render_samples(rc, 4)
+    pert = o.render_samples(rc, 4, pert=True)
+    assert len(np.unique(ref["iterations"])) > 10
+    N = int(v["N"])
+    with mp.workprec(750):
+        sz = mp.mpf(v["width"]) / nc
+        for k, (r, c) in enumerate(rc[:12]):
+            cr = mp.mpf(v["re"]) + (int(c) - nc // 2) * sz
+            ci = mp.mpf(v["im"]) + (nr // 2 - int(r) - 1) * sz
+            zr, zi, want = cr, ci, N
+            for i in range(N):
+                a, b = float(zr), float(zi)
+                if a * a + b * b > 1024.0 ** 2:
+                    want = i
+                    break
+                zr, zi = zr * zr - zi * zi + cr, 2 * zr * zi + ci
+            assert ref["iterations"][k] == want and pert["iterations"][k] == want, (r, c, want)
