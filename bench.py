@@ -131,9 +131,11 @@ FETCH_SCALE = 1048576 / 526656
 WRITE_SCALE = 1048576 / 1084704
 # (the zoom-video summary averages over the autotuner's subsample dispatches too,
 # so it does not give the full frame set's traffic)
-PMC_FILES = {"c2": "profiles/series_iterate_deepzoom_1e-30_pmc.txt",
-             "c1": "profiles/plain_c1_640x480_pmc.txt",
-             "c4": "profiles/series_iterate_zoomvideo_120x1280x720_pmc.txt"}
+PMC_FILES = {"c1": "profiles/plain_640x480x512_pmc.txt",
+             "c2": "profiles/series_deepzoom_7680x4320_1e-30_pmc.txt",
+             "c3": "profiles/series_precision_2560x1440_1e-200_pmc.txt",
+             "c4": "profiles/series_zoomvideo_120x1280x720_pmc.txt",
+             "c5": "profiles/series_learnedskip_3840x2160_1e-100_pmc.txt"}
 
 
 def pmc_traffic(config, kernel):
@@ -156,17 +158,21 @@ def pmc_traffic(config, kernel):
     return (fetch * FETCH_SCALE + write * WRITE_SCALE) * 1024.0
 
 
-def roofline(config, flop, s):
+def roofline(config, flop, s, fxs=False):
     """Roofline of the dominant kernel: algorithmic FLOPs of the steps it executed
     over its HIP-event duration (mean over the timed launches), against the FP64
     vector peak; both kernels are listed."""
     exec_iter = s["executed"] - s["executed_start"]
     K = s["block_k"]
-    kernels = {"iterate": {"name": "k_series_iterate" if flop == FLOP_PER_PERT_STEP else "k_plain_iterate",
-                           "instance": ("k_series_iterate<%d,false>" if flop == FLOP_PER_PERT_STEP
-                                        else "k_plain_iterate<%d>") % K,
+    series = flop == FLOP_PER_PERT_STEP
+    fma2 = "true" if s.get("fma2", 0) else "false"
+    kernels = {"iterate": {"name": "k_series_iterate" if series else "k_plain_iterate",
+                           "instance": ("k_series_iterate<%d,false>" % K) if series
+                           else ("k_plain_iterate<%d,%s>" % (K, fma2)),
                            "ms": s["ms_iterate"], "steps": exec_iter},
-               "start": {"name": "k_series_start" if flop == FLOP_PER_PERT_STEP else "k_plain_start",
+               "start": {"name": "k_series_start" if series else "k_plain_start",
+                         "instance": ("k_series_start<%s>" % ("true" if fxs else "false")) if series
+                         else ("k_plain_start<%s>" % fma2),
                          "ms": s["ms_start"], "steps": s["executed_start"]}}
     for k in kernels.values():
         k["tflops"] = flop * k["steps"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
@@ -636,7 +642,7 @@ def measure(config, args, block_k=None, waves=None):
         "data": "synthetic (fixed view)" if config == "c1" else "synthetic (seeded zoom-walk view)",
         "config": dict(desc, parallelism="single GPU", block_k=s["block_k"], waves_per_simd=s["waves_per_simd"],
                        tile=s["tile"], graph=s["graph"], pre_iter=m.get_launch()["pre_iter"]),
-        "roofline": roofline(config, flop, s),
+        "roofline": roofline(config, flop, s, fxs=desc.get("series") == "floatexp"),
         "detail": {"credited_px_it": credited, "executed_px_it": s["executed"],
                    "executed_in_start_kernel": s["executed_start"], "skipped": s["skipped"],
                    "n_iterated": s["n_iterated"], "n_sa_resolved": s["n_sa_resolved"],
