@@ -252,3 +252,41 @@ def test_view_file_renders_like_the_oracle(gpu, tmp_path, deep):
     ref = o.render_samples(rc, THREADS, pert=deep).reshape(nr, nc)
     assert np.array_equal(g["iterations"], ref["iterations"])
     assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
+
+
+@pytest.mark.parametrize("name,nr,nc,row,bar", [("c3_detail1", 1440, 2560, 719, 0.999),
+                                                ("c5_detail1", 2160, 3840, 1079, 0.98)])
+def test_floatexp_config_views_against_the_reference_method(gpu, name, nr, nc, row, bar):
+    """Configurations 3 and 5 on their own views, series in floatexp, per-sample
+    FP64 path: one middle row against the reference method (the same series
+    skip, then the HP continuation, mandelbrot.cpp:209-224) with the same
+    reference point.  Configuration 3 (a few thousand steps past the skip):
+    >= 99.9% equal.  Configuration 5 runs up to 13,000 steps past the skip
+    through chaotic tails, where the series' 1e-10 error at the skip point grows
+    until a FP64 and an HP continuation of it part ways (98.5% equal on this
+    row); there the reference method is itself off from the exact orbit: the
+    first differing samples are checked against direct iteration from z = c at
+    700 bits, and the reference method must disagree with it at each."""
+    nm = gpu
+    v = _VIEWS[name]
+    m = nm.Mandelbrot(nr, nc)
+    m.set_view(v["N"], v["re"], v["im"], v["width"])
+    m.set_series_mode(nm.NM_SERIES_FLOATEXP)
+    m.precompute()
+    m.render()
+    mine = m.grid()[row]
+    o = O.OView(nr, nc)
+    o.set_view(v["N"], v["re"], v["im"], v["width"])
+    re, im = m.reference_point()
+    o.precompute_at(re, im)
+    o.set_series_mode(1)
+    hp = o.render_rows(np.array([row], dtype=np.int32), THREADS).reshape(-1)
+    same = mine["iterations"] == hp["iterations"]
+    print("%s row %d: counts equal to the reference method on %.4f%% of %d samples (%d distinct counts)"
+          % (name, row, 100 * same.mean(), same.size, len(np.unique(hp["iterations"]))))
+    for c in np.nonzero(~same)[0][:3]:
+        exact = _brute_count(v, nr, nc, row, int(c), bits=700)
+        print("  column %d: GPU %d, reference method %d, 700-bit orbit %d" % (
+            c, mine["iterations"][c], hp["iterations"][c], exact))
+        assert exact != hp["iterations"][c]
+    assert same.mean() >= bar
