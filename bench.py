@@ -46,7 +46,7 @@ def load_views():
         return json.load(f)
 
 
-C1_BATCH = 512  # config 1 is ~2 us of work per frame: frames per launch set
+C1_BATCH = 512  # the 640x480 frame is ~2 us of work: frames per launch set
 C4_FRAMES = 120
 
 
@@ -69,7 +69,7 @@ def make_workload(name):
         m = nm.Mandelbrot(480, 640)
         m.set_view(1024, "-0.75", "0", "3.0")
         widths = ["3.0"] * C1_BATCH
-        desc = {"workload": "plain double path (config 1): 640x480, centre -0.75+0i, width 3.0, N 1024, "
+        desc = {"workload": "plain double path: 640x480, centre -0.75+0i, width 3.0, N 1024, "
                             "bailout 1024; %d copies of the frame per launch set" % C1_BATCH,
                 "samples": 640 * 480 * C1_BATCH, "N": 1024, "frames_per_step": C1_BATCH}
 
@@ -572,7 +572,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the frame would shard by "
-              "rows/tiles with the reference orbit broadcast once; see DESIGN.md)", file=sys.stderr)
+              "rows/tiles with the reference orbit broadcast once)", file=sys.stderr)
         sys.exit(2)
 
     line = measure(args.config, args)
@@ -580,7 +580,7 @@ def main():
     if args.config == "c1" and args.deep:
         # the deep-zoom configuration under its own key, with its variants, the
         # colouring and video legs and its CPU baseline
-        print("# deep_zoom (config 2) ...", file=sys.stderr, flush=True)
+        print("# deep_zoom (7680x4320 at 1e-30) ...", file=sys.stderr, flush=True)
         line["deep_zoom"] = measure("c2", args, block_k=0, waves=0)
     print(json.dumps(line), flush=True)
 

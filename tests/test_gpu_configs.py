@@ -257,11 +257,12 @@ def test_view_file_renders_like_the_oracle(gpu, tmp_path, deep):
 @pytest.mark.parametrize("name,nr,nc,row,bar", [("c3_detail1", 1440, 2560, 719, 0.999),
                                                 ("c5_detail1", 2160, 3840, 1079, 0.98)])
 def test_floatexp_config_views_against_the_reference_method(gpu, name, nr, nc, row, bar):
-    """Configurations 3 and 5 on their own views, series in floatexp, per-sample
-    FP64 path: one middle row against the reference method (the same series
-    skip, then the HP continuation, mandelbrot.cpp:209-224) with the same
-    reference point.  Configuration 3 (a few thousand steps past the skip):
-    >= 99.9% equal.  Configuration 5 runs up to 13,000 steps past the skip
+    """The 2560x1440 view at 1e-200 (N 50000) and the 3840x2160 view at 1e-100
+    (N 100000), series in floatexp, per-sample FP64 path: one middle row against
+    the reference method (the same series skip, then the HP continuation,
+    mandelbrot.cpp:209-224) with the same reference point.  At 1e-200 (a few
+    thousand steps past the skip): >= 99.9% equal.  At 1e-100 samples run up to
+    13,000 steps past the skip
     through chaotic tails, where the series' 1e-10 error at the skip point grows
     until a FP64 and an HP continuation of it part ways (98.5% equal on this
     row); there the reference method is itself off from the exact orbit: the

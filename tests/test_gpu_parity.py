@@ -35,7 +35,7 @@ def _pair(nr, nc, N, cre, cim, width, bailout=None):
 
 
 def test_plain_path_config1_full_frame(gpu):
-    """Config 1: 640x480, centre -0.75+0i, width 3.0, N=1024, bailout 1024: the
+    """The plain-path view: 640x480, centre -0.75+0i, width 3.0, N=1024, bailout 1024: the
     reference's plain-double path (mandelbrot.cpp:231-254), every sample."""
     m, o = _pair(480, 640, 1024, "-0.75", "0", "3.0")
     assert m.useHardware() and o.use_hardware()
