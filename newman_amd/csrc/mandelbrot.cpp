@@ -1306,8 +1306,8 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
   const int tiles[4] = {0, 32, 64, 128};
   const int waves[4] = {2, 4, 6, 8};
   // series blocks: FP64 up to 64 steps, FP32X (renormalised at block ends) up to 32
-  const int ks_sa[3] = {16, 32, 64}, ks_fx[3] = {8, 16, 32}, ks_hw[2] = {16, 32};
-  const int nks = set.hw ? 2 : 3;
+  const int ks_sa[3] = {16, 32, 64}, ks_fx[3] = {8, 16, 32}, ks_hw[3] = {16, 32, 64};
+  const int nks = 3;
   Launch best;
   best.ms = 1e300;
   if (table) table->clear();

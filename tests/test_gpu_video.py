@@ -87,6 +87,22 @@ def test_autotuner_picks_a_measured_setting(gpu):
     assert np.array_equal(got, ref)
 
 
+def test_autotuner_on_a_plain_frame_set(gpu):
+    """A frame set on the plain path (shallow widths): the sweep covers 16-, 32-
+    and 64-update blocks and the tuned launch renders the same grids."""
+    nm = gpu
+    m = nm.Mandelbrot(96, 128)
+    m.set_view(512, "-0.75", "0", "3.0")
+    widths = ["3.0", "2.5", "2.0", "1.5"]
+    ref, _ = m.render_frames(widths)
+    best, table = m.autotune_frames(widths)
+    assert len(table) == 4 * 4 * 3
+    assert {r["block_k"] for r in table} == {16, 32, 64}
+    assert best["ms"] == min(r["ms"] for r in table) and best["ms"] > 0
+    got, _ = m.render_frames(widths)
+    assert np.array_equal(got, ref)
+
+
 def test_device_frame_tables_follow_the_view(gpu):
     """The engine keeps a frame set's tables on the device between renders of the
     same set; any change of view, widths, sample order or object re-uploads."""
