@@ -249,6 +249,7 @@ def train_skipnet_seeded(nm, m, T, epochs=120):
         ul, how, st = tv.tile_skips(T, mode=2, net=net)
         scan = tv.tile_skips(T, mode=0)[2]
         acc[name] = {"tiles": int(len(how)), "accepted": float((how == 2).mean()),
+                     "lower_bound": float((how == 4).mean()),
                      "equal_to_scan": bool(np.array_equal(ul, ua)), "exact_tests_learned": int(st["evals"]),
                      "exact_tests_scan": int(scan["evals"])}
     return net, {"samples": int(len(t)), "views": len(train), "seeds": TRAIN_SEEDS, "epochs": epochs,

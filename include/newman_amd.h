@@ -216,9 +216,9 @@ int nm_autotune_frames(nm_mandelbrot* h, const char* const* widths, int nframes,
 typedef struct nm_skipnet nm_skipnet;
 typedef struct nm_tile_stats {
   int32_t ntiles;
-  int32_t accepted;   /* predictions the check accepted */
-  int32_t rejected;   /* predictions rejected: the tile was scanned */
-  int32_t reserved;
+  int32_t accepted;   /* predictions that were the first unstable index */
+  int32_t rejected;   /* predictions the check rejected: the tile was scanned from index 1 */
+  int32_t bounded;    /* predictions the check accepted as a lower bound: scanned from them */
   uint64_t evals;     /* exact stability tests evaluated at the probes */
   double ms_search;   /* tile kernel, ms */
   double ms_predict;  /* features, network and snapping, ms (host wall time) */
@@ -240,7 +240,8 @@ int nm_skipnet_gradient(nm_skipnet* net, const float* xn, const float* t, int n,
 /* the view's tile features (NM_SKIPNET_INPUTS per tile, row-major tiles) */
 int nm_tile_features(nm_mandelbrot* h, int tile, float* x, int cap_tiles, int* ntiles);
 /* first unstable index of every tile probe (0 = none): mode 0 scan, 1 tables,
- * 2 learned (net required); how (NULL ok): 0 scan, 1 tables, 2 accepted, 3 rejected */
+ * 2 learned (net required); how (NULL ok): 0 scan, 1 tables, 2 prediction exact,
+ * 3 rejected (scanned from 1), 4 a certified lower bound (scanned from it) */
 int nm_tile_skips(nm_mandelbrot* h, int tile, int mode, nm_skipnet* net, int* ua, int* how, int cap_tiles,
                   nm_tile_stats* stats);
 /* the network's training target of each first unstable index for this view: the

@@ -65,7 +65,7 @@ class Launch(C.Structure):
 
 
 class TileStats(C.Structure):
-    _fields_ = [("ntiles", C.c_int32), ("accepted", C.c_int32), ("rejected", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("ntiles", C.c_int32), ("accepted", C.c_int32), ("rejected", C.c_int32), ("bounded", C.c_int32),
                 ("evals", C.c_uint64), ("ms_search", C.c_double), ("ms_predict", C.c_double)]
 
     def as_dict(self):

@@ -487,7 +487,7 @@ static void copy_tile_stats(const nm::Mandelbrot::TileStats& s, nm_tile_stats* o
   o->ntiles = s.ntiles;
   o->accepted = s.accepted;
   o->rejected = s.rejected;
-  o->reserved = 0;
+  o->bounded = s.bounded;
   o->evals = s.evals;
   o->ms_search = s.ms_search;
   o->ms_predict = s.ms_predict;

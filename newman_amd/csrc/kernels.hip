@@ -1087,12 +1087,16 @@ __device__ int sa_stop_index_fx(const SeriesArgs& a, double L, int2 br, const Ep
 // One wave per tile.  mode 0: the reference's scan over i = 1, 2, ... (64 indices
 // per round, ballot for the first unstable one); mode 1: the threshold-table
 // search of the per-sample path; mode 2: validate pred[t] and fall back to the
-// scan.  A prediction p is accepted only when it is exactly the first unstable
-// index: every i < p is certainly stable by the prefix minimum of the
-// thresholds (PM[p-1] > log2|e|^2 + margin, with the magnitude checks that make
-// the exact test safe) and the exact test fails at p (p = 0: PM[last] clears).
-// how[t]: 0 scanned, 1 table, 2 prediction accepted, 3 prediction rejected
-// (scanned); evals[t]: exact tests evaluated.
+// scan.  The check of a start s: every i < s is certainly stable by the prefix
+// minimum of the thresholds (PM[s-1] > log2|e|^2 + margin, with the magnitude
+// checks that make the exact test safe; for the prediction p = 0, "never
+// unstable", the whole prefix).  The scan runs from the first of p, p - p/64,
+// p - p/16, p - p/4, p/2, p/4 and p/16 that passes the check (p itself is the
+// answer when the exact test fails there), else from 1.  The result is the
+// scan's in every case.
+// how[t]: 0 scanned, 1 table, 2 prediction exact, 3 prediction rejected (scanned
+// from 1), 4 prediction a certified lower bound (scanned from it); evals[t]:
+// exact tests evaluated.
 __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* probe, const int* pred, int mode,
                                                   int ntiles, int* ua_out, int* how, int* evals) {
   const int t = blockIdx.x;
@@ -1105,7 +1109,7 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
   const double L = log2_sqmag(e);
   const EpsFx pf = a.fxs ? eps_fx(e) : EpsFx{};
   const int last = a.len - 1;
-  int ua = -1, h = 0, ev = 0;
+  int ua = -1, h = 0, ev = 0, from = 1;
   if (last < 1 || L == -INFINITY) {  // e = 0: the test never fires
     ua = 0;
     h = 1;
@@ -1115,24 +1119,31 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
     h = 1;
   } else if (mode == 2) {
     const int p = pred[t];
-    const int below = p > 0 ? p - 1 : last;
     // the floatexp test has no range limits: the prefix minimum alone certifies
-    bool ok = p >= 0 && p <= last && below >= 1 && a.PM[below] > L + kMarg &&
-              (a.fxs || (a.SP[below] == 0 && rounding_certified(a, L, a.BN[below], a.BX[below], a.CX[below])));
-    if (ok && p > 0) {
-      ok = a.fxs ? unstable_fx(a, p, pf) : unstable_at(a, p, e2, e3);
-      ev = 1;
-    }
-    if (ok) {
-      ua = p;
+    auto stable_below = [&](int s) {
+      const int below = s - 1;
+      return below >= 1 && a.PM[below] > L + kMarg &&
+             (a.fxs || (a.SP[below] == 0 && rounding_certified(a, L, a.BN[below], a.BX[below], a.CX[below])));
+    };
+    h = 3;
+    if (p == 0 && stable_below(last + 1)) {
+      ua = 0;
       h = 2;
-    } else {
-      h = 3;
+    } else if (p == 0 || (p > 1 && p <= last)) {
+      // "never unstable" that fails its check backs off from the last index
+      const int q = p == 0 ? last : p;
+      const int cand[7] = {q, q - max(1, q >> 6), q - (q >> 4), q - (q >> 2), q >> 1, q >> 2, q >> 4};
+      for (int k = 0; k < 7; k++)
+        if (cand[k] > 1 && stable_below(cand[k])) {
+          from = cand[k];
+          h = 4;
+          break;
+        }
     }
   }
-  if (ua < 0) {  // the reference's scan
+  if (ua < 0) {  // the reference's scan, from `from`
     ua = 0;
-    for (int base = 1; base <= last; base += 64) {
+    for (int base = from; base <= last; base += 64) {
       const int i = base + lane;
       const bool u = i <= last && (a.fxs ? unstable_fx(a, i, pf) : unstable_at(a, i, e2, e3));
       const unsigned long long m = __ballot(u);
@@ -1140,6 +1151,7 @@ __global__ __launch_bounds__(64) void k_tile_skip(SeriesArgs a, const int2* prob
         const int k = __ffsll((unsigned long long)m);  // 1 + the first unstable lane
         ua = base + k - 1;
         ev += k;
+        if (h == 4 && ua == from) h = 2;  // the prediction itself
         break;
       }
       ev += min(64, last - base + 1);

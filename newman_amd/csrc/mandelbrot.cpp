@@ -1093,11 +1093,18 @@ void Mandelbrot::ensureRecords() {
     if (ref_.PM[i] < (i > 1 ? ref_.PM[i - 1] : INFINITY)) records_.push_back(i);
 }
 
+// The network's output as a first unstable index: the record of the nearest
+// rank, moved kSnapBack of the records earlier.  The tile search starts its scan
+// at the prediction when every index below it is certainly stable (an exact hit
+// costs one test), else at the first of six earlier points that is, else at 1:
+// an early prediction costs the tests between it and the answer.
+constexpr double kSnapBack = 0.02;
 int Mandelbrot::snapIndex(double frac) const {
   const int R = (int)records_.size();
   const long rank = std::lround(frac * (R + 1)) - 1;
   if (rank >= R) return 0;  // no unstable index
-  return records_[(size_t)std::max(0L, rank)];
+  const long back = (long)std::ceil(kSnapBack * R);
+  return records_[(size_t)std::max(0L, rank - back)];
 }
 
 double Mandelbrot::targetOf(int ua) const {
@@ -1171,6 +1178,7 @@ int Mandelbrot::tileSkipsFor(Engine* eng, const std::vector<const SaFrameHost*>&
   for (size_t t = 0; t < ua.size(); t++) {
     tile_stats.accepted += ts.how[t] == 2;
     tile_stats.rejected += ts.how[t] == 3;
+    tile_stats.bounded += ts.how[t] == 4;
     tile_stats.evals += (uint64_t)ts.evals[t];
   }
   tile_stats.ms_search = ts.ms;

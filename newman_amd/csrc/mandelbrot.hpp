@@ -73,7 +73,7 @@ class Mandelbrot {
   int dev_frames_ = 0;         // frames in that render
   bool dev_grid_ = false;      // that render was of this object's grid (not a frame set)
   struct TileStats {
-    int ntiles = 0, accepted = 0, rejected = 0;
+    int ntiles = 0, accepted = 0, rejected = 0, bounded = 0;
     uint64_t evals = 0;         // exact stability tests evaluated at the probes
     double ms_search = 0.0;     // the tile kernel
     double ms_predict = 0.0;    // features + network + snapping (host wall time)

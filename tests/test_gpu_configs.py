@@ -204,7 +204,7 @@ def test_learned_skip_config_view(gpu):
     m.render()
     g = m.grid()
     st = m.tile_stats()
-    assert st["ntiles"] == ua.size and st["accepted"] + st["rejected"] == st["ntiles"]
+    assert st["ntiles"] == ua.size and st["accepted"] + st["rejected"] + st["bounded"] == st["ntiles"]
     o = O.OView(nr, nc)
     o.set_view(v["N"], v["re"], v["im"], v["width"])
     re, im = m.reference_point()
