@@ -412,6 +412,20 @@ def _one_thread(run, what):
             "sample": "%s (%d samples, %.3g credited px-it, %.1f s)" % (what, n, credited, t)}
 
 
+def host_cpu():
+    """The host's CPU model and the cores this process may use."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "cores_available": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(desc, name, nthreads, rows_every):
     """The oracle (the reference's method on the CPU) on a bounded sample of the
     same workload, timed on this host's cores, and on one core (`single_thread`)."""
@@ -672,6 +686,7 @@ def measure(config, args, block_k=None, waves=None):
     if not args.no_cpu_baseline:
         print("# cpu baseline (%s) ..." % config, file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(desc, config, args.cpu_threads, args.cpu_rows_every)
+        line["cpu_baseline"]["host"] = host_cpu()
     return line
 
 
