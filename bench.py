@@ -59,7 +59,9 @@ def make_workload(name):
         m = nm.Mandelbrot(1080 * 4, 1920 * 4)
         m.set_view(v["N"], v["re"], v["im"], v["width"])
         desc = {"workload": "deep zoom, FP64 deltas: 1920x1080 output x 4x4 multisampling (7680x4320 samples), "
-                            "width 1e-30, N 20000, bailout 1024, tol 1e-10, centre = zoom walk seed 1",
+                            "width 1e-30, N 20000, bailout 1024, tol 1e-10, centre = zoom walk seed 1; the timed "
+                            "pass renders the 7680x4320 samples, the 4x4 reduction to 1920x1080 (~0.05 ms) is "
+                            "timed in the colour leg",
                 "samples": 7680 * 4320, "N": 20000, "width": "1e-30", "precision_bits": m.precision_bits()}
 
         def step(k, w, reps=1):
