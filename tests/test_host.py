@@ -161,6 +161,9 @@ def test_bad_arguments_return_status():
     assert L.nm_beauty_render(h, None, 256, 1, out, None) == nm.NM_ERR_ARG
     assert L.nm_beauty_render(h, out, 1, 1, None, None) == nm.NM_ERR_ARG
     assert L.nm_set_bailout(h, 1.0) == nm.NM_ERR_ARG  # the bailout must exceed 1
+    assert L.nm_set_series_mode(h, 2) == nm.NM_ERR_ARG
+    assert L.nm_set_series_mode(h, nm.NM_SERIES_FLOATEXP) == nm.NM_OK
+    assert L.nm_set_series_mode(h, nm.NM_SERIES_DOUBLE) == nm.NM_OK
     L.nm_destroy(h)
 
 
