@@ -414,10 +414,45 @@ __device__ __forceinline__ void park(double& x, double& y, double& r2, double& i
   x = y = r2 = i2 = cx = cy = 0.0;
 }
 
-// Updates k0 + J .. k0 + 3 of a lockstep block of four (unrolled by recursion).
-template <int J, bool FMA2>
+// The escape test of a pair of updates.  For bailout radius R, 2 <= R <= 2^100,
+// and every |c| <= 0.999 (R^2 - R): when the test of update j holds (computed
+// |z_j|^2 > R^2) and that of update j - 1 did not, the test of update j + 1 holds
+// too.  |z_j|^2 > R^2 (1 - 2^-51) exactly, so the computed |z_{j+1}| is at least
+// |z_j|^2 (1 - 2^-49) - |c| (1 + 2^-49) >= R + 0.001 R (R - 1) - 2^-48 R^2 >=
+// R + 0.0004 R^2, and its computed square exceeds R^2; |z_j| <= 2 R^2 bounds
+// |z_{j+1}|^2 by (4 R^4 + R^2)^2 < 2^805 (no overflow, no NaN).  So the pair tests only its
+// second update's |z|^2 and, on an escape there, finds the first escaping update
+// from the first update's r2 + i2, the reference's sqMag of that update bit for
+// bit (r2 and i2 stay in registers): one add and one compare-and-branch less
+// per two updates.  A lane that escaped at j runs update j + 1 unrecorded.
+inline bool pair_test_ok(double radius, double c_bound) {
+  return radius >= 2.0 && radius <= 0x1p100 && c_bound <= 0.999 * (radius * radius - radius);
+}
+
+// Updates k0 + J .. k0 + 3 of a lockstep block of four (unrolled by recursion);
+// PAIR tests pairs of updates (pair_test_ok).
+template <int J, bool FMA2, bool PAIR>
 __device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int k0) {
-  if constexpr (J < 4) {
+  if constexpr (J < 4 && PAIR) {
+    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+    const double r2p = L.r2, i2p = L.i2;
+    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+    const double mg = L.r2 + L.i2;
+    const bool e = mg > b2;
+    const unsigned long long esc = __ballot(e);
+    if (__builtin_expect(esc != 0, 0)) {
+      L.live &= ~esc;
+      if (e) {
+        const double mp = r2p + i2p;  // sqMag of the pair's first update
+        const bool first = mp > b2;
+        L.kesc = k0 + (first ? J : J + 1);
+        L.mgesc = first ? mp : mg;
+        L.alive = false;
+        park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+      }
+    }
+    pre_block<J + 2, FMA2, PAIR>(L, b2, k0);
+  } else if constexpr (J < 4) {
     plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
     const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
     const bool e = mg > b2;
@@ -431,11 +466,11 @@ __device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int
         park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
       }
     }
-    pre_block<J + 1, FMA2>(L, b2, k0);
+    pre_block<J + 1, FMA2, PAIR>(L, b2, k0);
   }
 }
 
-template <bool FMA2>
+template <bool FMA2, bool PAIR>
 __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   const Common& c = a.c;
   __shared__ double2 ltab[kLog2Tab];
@@ -485,7 +520,7 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
       int kend = 0;
       PreLane L{x, y, r2, i2, zx, zy, live, alive, kesc, mgesc};
       while (live != 0 && kend < pre) {
-        pre_block<0, FMA2>(L, c.b2, kend);
+        pre_block<0, FMA2, PAIR>(L, c.b2, kend);
         kend += 4;
       }
       bool want = false;
@@ -741,33 +776,51 @@ struct ChunkMap {
 struct PlainLane {
   double &x, &y, &r2, &i2, &cx, &cy;
   bool& act;
-  int& jesc;  // update of the block at which the lane escaped (-1: none)
+  int& jesc;      // update of the block at which the lane escaped (-1: none)
+  double& mgesc;  // |z|^2 there
 };
 
 // Updates J .. K - 1 of a block (unrolled by recursion).  The handler of an
-// escape writes the count and |z|^2 (finalize turns them into smoothing; the
-// block's end replaces a count at or beyond N by N) and parks the lane; the
-// no-escape case falls through.
-template <int J, int K, bool FMA2>
-__device__ __forceinline__ void plain_block(PlainLane& L, const Common& c, const uint32_t pix, const int k0) {
-  if constexpr (J < K) {
+// escape records the update and |z|^2 and parks the lane (the block's end
+// writes them); the no-escape case falls through.  PAIR tests pairs of updates
+// (pair_test_ok).
+template <int J, int K, bool FMA2, bool PAIR>
+__device__ __forceinline__ void plain_block(PlainLane& L, const double b2) {
+  static_assert(!PAIR || K % 2 == 0, "a paired block holds whole pairs");
+  if constexpr (J < K && PAIR) {
     plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
-    const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
-    const bool e = mg > c.b2;
+    const double r2p = L.r2, i2p = L.i2;
+    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+    const double mg = L.r2 + L.i2;
+    const bool e = mg > b2;
     if (__builtin_expect(__ballot(e) != 0, 0)) {
       if (e) {
-        c.out[pix] = EscapeValue{k0 + J, 0.0f};
-        c.r2[pix] = mg;  // smoothing: k_finalize
-        L.jesc = J;
+        const double mp = r2p + i2p;  // sqMag of the pair's first update
+        const bool first = mp > b2;
+        L.jesc = first ? J : J + 1;
+        L.mgesc = first ? mp : mg;
         L.act = false;
         park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
       }
     }
-    plain_block<J + 1, K, FMA2>(L, c, pix, k0);
+    plain_block<J + 2, K, FMA2, PAIR>(L, b2);
+  } else if constexpr (J < K) {
+    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+    const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
+    const bool e = mg > b2;
+    if (__builtin_expect(__ballot(e) != 0, 0)) {
+      if (e) {
+        L.jesc = J;
+        L.mgesc = mg;
+        L.act = false;
+        park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+      }
+    }
+    plain_block<J + 1, K, FMA2, PAIR>(L, b2);
   }
 }
 
-template <int K, bool FMA2>
+template <int K, bool FMA2, bool PAIR>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_plain_iterate(
     PlainArgs a) {
   const Common& c = a.c;
@@ -830,8 +883,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     const int k0 = k;
     const bool was = act;
     int jesc = -1;
-    PlainLane L{x, y, r2, i2, cx, cy, act, jesc};
-    plain_block<0, K, FMA2>(L, c, pix, k0);
+    double mgesc = 0.0;
+    PlainLane L{x, y, r2, i2, cx, cy, act, jesc, mgesc};
+    plain_block<0, K, FMA2, PAIR>(L, c.b2);
     if (was) {
       // the reference tests k >= N before each update: an escape at k0 + j
       // counts only below N, and a sample that reaches N in the block ends there
@@ -840,6 +894,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
         c.out[pix] = EscapeValue{N, 0.0f};
         act = false;
         park(x, y, r2, i2, cx, cy);
+      } else if (jesc >= 0) {
+        c.out[pix] = EscapeValue{k0 + jesc, 0.0f};
+        c.r2[pix] = mgesc;  // smoothing: k_finalize
       }
       executed += (unsigned long long)min(ran, N - k0);
       k = k0 + K;
@@ -2220,13 +2277,24 @@ class EngineImpl : public Engine {
     // 256-thread workgroups = 4 waves = one per SIMD: num_cu * waves workgroups
     // the folded update (plain_update) needs every frame's coordinates clear of
     // tiny nonzero values and a bailout of at most 2^400
-    bool fma2 = opt.bailout <= 0x1p400;
-    for (auto* f : frames) fma2 = fma2 && f->fma_ok;
-    if (stats) stats->fma2 = fma2 ? 1 : 0;
-#define NM_PLAIN(KK)                                                                                      \
-  return fma2 ? run(a, k_plain_start<true>, k_plain_iterate<KK, true>, waves, opt, stats, nout, host_out, \
-                    emit_to)                                                                              \
-              : run(a, k_plain_start<false>, k_plain_iterate<KK, false>, waves, opt, stats, nout, host_out, emit_to)
+    bool fma2 = opt.bailout <= 0x1p400, pair = true;
+    for (auto* f : frames) {
+      fma2 = fma2 && f->fma_ok;
+      pair = pair && pair_test_ok(opt.bailout, f->c_bound);
+    }
+    if (stats) {
+      stats->fma2 = fma2 ? 1 : 0;
+      stats->paired = pair ? 1 : 0;
+    }
+#define NM_PLAIN(KK)                                                                                           \
+  return fma2 && pair ? run(a, k_plain_start<true, true>, k_plain_iterate<KK, true, true>, waves, opt, stats,  \
+                            nout, host_out, emit_to)                                                           \
+         : fma2      ? run(a, k_plain_start<true, false>, k_plain_iterate<KK, true, false>, waves, opt, stats, \
+                           nout, host_out, emit_to)                                                            \
+         : pair      ? run(a, k_plain_start<false, true>, k_plain_iterate<KK, false, true>, waves, opt, stats, \
+                           nout, host_out, emit_to)                                                            \
+                     : run(a, k_plain_start<false, false>, k_plain_iterate<KK, false, false>, waves, opt,      \
+                           stats, nout, host_out, emit_to)
     switch (K) {
       case 8: NM_PLAIN(8);
       case 16: NM_PLAIN(16);
@@ -2710,11 +2778,15 @@ int render_hw_noalloc(const HwFrameView& f, int N, double bailout, EscapeValue* 
   a.crelo = tabs + f.nc;
   a.cim = tabs + 2 * f.nc;
   a.cimlo = tabs + 2 * f.nc + f.nr;
-  const bool fma2 = f.fma_ok && bailout <= 0x1p400;
-  NA_HIP(fma2 ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<true>, k_plain_iterate<64, true>, 8,
-                               nullptr, nullptr)
-              : launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<false>, k_plain_iterate<64, false>, 8,
-                               nullptr, nullptr));
+  const bool fma2 = f.fma_ok && bailout <= 0x1p400, pair = pair_test_ok(bailout, f.c_bound);
+  NA_HIP(fma2 && pair ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<true, true>,
+                                       k_plain_iterate<64, true, true>, 8, nullptr, nullptr)
+         : fma2       ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<true, false>,
+                                       k_plain_iterate<64, true, false>, 8, nullptr, nullptr)
+         : pair       ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<false, true>,
+                                       k_plain_iterate<64, false, true>, 8, nullptr, nullptr)
+                      : launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<false, false>,
+                                       k_plain_iterate<64, false, false>, 8, nullptr, nullptr));
   NA_HIP(hipMemcpyAsync(host_out, out, n * sizeof(EscapeValue), hipMemcpyDeviceToHost, s));
   NA_HIP(hipStreamSynchronize(s));
 #undef NA_HIP

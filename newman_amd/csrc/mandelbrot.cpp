@@ -719,14 +719,15 @@ int Mandelbrot::buildHwFrame(HwFrameHost& f, const HPComplex& cen, const HPCompl
   f.nc = cols();
   f.cre.resize(f.nc); f.crelo.resize(f.nc);
   f.cim.resize(f.nr); f.cimlo.resize(f.nr);
-  fillHwTables(cen, s, f.cre.data(), f.crelo.data(), f.cim.data(), f.cimlo.data(), &f.card_mode, &f.fma_ok);
+  fillHwTables(cen, s, f.cre.data(), f.crelo.data(), f.cim.data(), f.cimlo.data(), &f.card_mode, &f.fma_ok,
+               &f.c_bound);
   return NM_OK;
 }
 
 // The tables of buildHwFrame into caller-owned arrays (cols() and rows()
 // entries); allocates nothing.
 void Mandelbrot::fillHwTables(const HPComplex& cen, const HPComplex& s, double* cre, double* crelo, double* cim,
-                              double* cimlo, int* card_mode, bool* fma_ok) const {
+                              double* cimlo, int* card_mode, bool* fma_ok, double* c_bound) const {
   const int L = hp_limbs_for_bits(hp_default_bits());
   const int nr = rows(), nc = cols();
   hpf p(L), rem(L), td(2);
@@ -750,6 +751,10 @@ void Mandelbrot::fillHwTables(const HPComplex& cen, const HPComplex& s, double* 
   *card_mode = m < 0 ? CARD_TEST_DD : m;
   auto clear = [](double v) { return v == 0.0 || std::fabs(v) >= 0x1p-200; };
   *fma_ok = std::all_of(cre, cre + nc, clear) && std::all_of(cim, cim + nr, clear);
+  double mx = 0.0, my = 0.0;
+  for (int c = 0; c < nc; c++) mx = std::max(mx, std::fabs(cre[c]));
+  for (int r = 0; r < nr; r++) my = std::max(my, std::fabs(cim[r]));
+  *c_bound = std::sqrt(mx * mx + my * my) * (1.0 + 0x1p-40);  // rounded up
 }
 
 // computeAll without device allocation (render_hw_noalloc): the plain path
@@ -768,7 +773,7 @@ int Mandelbrot::computeAllNoAlloc(EscapeValue* out, size_t n) {
   v.nr = nr;
   v.nc = nc;
   double *cre = tab, *crelo = tab + kSide, *cim = tab + 2 * kSide, *cimlo = tab + 3 * kSide;
-  fillHwTables(center, sz, cre, crelo, cim, cimlo, &v.card_mode, &v.fma_ok);
+  fillHwTables(center, sz, cre, crelo, cim, cimlo, &v.card_mode, &v.fma_ok, &v.c_bound);
   v.cre = cre;
   v.crelo = crelo;
   v.cim = cim;

@@ -179,7 +179,7 @@ class Mandelbrot {
   int buildSaFrame(SaFrameHost& f) { return buildSaFrame(f, center, sz); }
   int buildHwFrame(HwFrameHost& f, const HPComplex& c, const HPComplex& s);
   void fillHwTables(const HPComplex& c, const HPComplex& s, double* cre, double* crelo, double* cim, double* cimlo,
-                    int* card_mode, bool* fma_ok) const;
+                    int* card_mode, bool* fma_ok, double* c_bound) const;
   // computeAll into out (rows x cols) on the plain path without allocating
   // device memory (render_hw_noalloc); noalloc_err_ holds the last error
   int computeAllNoAlloc(EscapeValue* out, size_t n);

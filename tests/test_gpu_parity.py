@@ -51,14 +51,34 @@ def test_plain_path_config1_full_frame(gpu):
     assert st["credited"] == esc.sum() - st["n_cardioid"] * 1024
 
 
-@pytest.mark.parametrize("bailout", [2.0, 1.5, 1e6])
-def test_plain_path_bailout_2(gpu, bailout):
+# (radius, escape test once per pair of updates): the frame's |c| reaches
+# |-2.25 + 1.125i| = 2.52, so the pair test needs 0.999 (R^2 - R) >= 2.52 and R <= 2^100
+@pytest.mark.parametrize("bailout,paired", [(2.0, False), (1.5, False), (3.0, True), (1e6, True),
+                                            (2.0 ** 101, False)])
+def test_plain_path_bailout_2(gpu, bailout, paired):
     """Other bailout radii (the configuration names radius 2): the smoothing's
-    series logarithms for radii >= 2, the library's below 2."""
+    series logarithms for radii >= 2, the library's below 2; the escape test per
+    update and per pair of updates."""
     m, o = _pair(120, 160, 1024, "-0.75", "0", "3.0", bailout=bailout)
-    m.render()
+    st = m.render()
+    assert st["paired"] == int(paired)
     g = m.grid()
     ref = o.render_rows(np.arange(120))
+    assert np.array_equal(g["iterations"], ref["iterations"])
+    assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
+
+
+@pytest.mark.parametrize("bailout,paired", [(1024.0, True), (2.0, False)])
+def test_plain_path_unfolded_update(gpu, bailout, paired):
+    """A frame whose row coordinates include 1e-70 (nonzero and below 2^-200)
+    runs the three-operation update y' = (2x) y + c (the fold into one fma is
+    exact only for coordinates 0 or >= 2^-200), with and without the pair test."""
+    m, o = _pair(61, 80, 700, "-0.75", "1e-70", "3.0", bailout=bailout)
+    assert m.useHardware()
+    st = m.render()
+    assert st["fma2"] == 0 and st["paired"] == int(paired)
+    g = m.grid()
+    ref = o.render_rows(np.arange(61))
     assert np.array_equal(g["iterations"], ref["iterations"])
     assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
 
