@@ -168,14 +168,14 @@ def roofline(config, flop, s, fxs=False):
     K = s["block_k"]
     series = flop == FLOP_PER_PERT_STEP
     fma2 = "true" if s.get("fma2", 0) else "false"
-    pair = "true" if s.get("paired", 0) else "false"
+    grouped = "true" if s.get("grouped", 0) else "false"
     kernels = {"iterate": {"name": "k_series_iterate" if series else "k_plain_iterate",
                            "instance": ("k_series_iterate<%d,false>" % K) if series
-                           else ("k_plain_iterate<%d,%s,%s>" % (K, fma2, pair)),
+                           else ("k_plain_iterate<%d,%s,%s>" % (K, fma2, grouped)),
                            "ms": s["ms_iterate"], "steps": exec_iter},
                "start": {"name": "k_series_start" if series else "k_plain_start",
                          "instance": ("k_series_start<%s>" % ("true" if fxs else "false")) if series
-                         else ("k_plain_start<%s,%s>" % (fma2, pair)),
+                         else ("k_plain_start<%s,%s>" % (fma2, grouped)),
                          "ms": s["ms_start"], "steps": s["executed_start"]}}
     for k in kernels.values():
         k["tflops"] = flop * k["steps"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
@@ -569,8 +569,8 @@ def cpu_baseline(desc, name, nthreads, rows_every):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c1")
     ap.add_argument("--deep", type=int, default=1,
                     help="c1: also run the deep-zoom configuration (c2) as the deep_zoom key")
@@ -667,7 +667,7 @@ def measure(config, args, block_k=None, waves=None):
                    "n_cardioid": s["n_cardioid"], "n_rebased_blocks": s["n_rebased"],
                    "ms_start": s["ms_start"], "ms_compact": s["ms_compact"], "ms_iterate": s["ms_iterate"],
                    "ms_finalize": s["ms_finalize"], "ms_kernels": s["ms_total"], "precompute_s": t_pre,
-                   "probe_orbits_hp": n_hp, "folded_update": bool(s.get("fma2", 0)), "paired_test": bool(s.get("paired", 0)),
+                   "probe_orbits_hp": n_hp, "folded_update": bool(s.get("fma2", 0)), "grouped_test": bool(s.get("grouped", 0)),
                    "kernel_px_it_per_s": credited / (s["ms_total"] * 1e-3)},
         "cpu_baseline": None,
     }

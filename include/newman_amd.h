@@ -65,9 +65,9 @@ typedef struct nm_render_stats {
   int32_t tile;          /* sample order used: 0 row-major, T = T x T tiles */
   int32_t fma2;          /* plain path: 1 when y' = (2x) y + c ran as fma(x y, 2, c), bit-identical for frames
                             whose coordinates are 0 or at least 2^-200 in magnitude */
-  int32_t paired;        /* plain path: 1 when the escape test ran once per pair of updates, the first escaping
-                            update found from the pair's first |z|^2 (bailout radius 2 to 2^100 and every
-                            |c| <= 0.999 (R^2 - R): an escape at one update implies one at the next) */
+  int32_t grouped;       /* plain path: 1 when the escape test ran once per group of four updates, the first
+                            escaping update found from the group's earlier |z|^2 (bailout radius 2 to 2^30 and
+                            every |c| <= 0.999 (R^2 - R): an escape at one update implies one at the next) */
 } nm_render_stats;
 
 typedef struct nm_mandelbrot nm_mandelbrot; /* class Mandelbrot (mandelbrot.h:7) */

@@ -81,7 +81,7 @@ static void copy_stats(const nm::RenderStats& s, nm_render_stats* o) {
   o->graph = s.graph;
   o->tile = s.tile;
   o->fma2 = s.fma2;
-  o->paired = s.paired;
+  o->grouped = s.grouped;
 }
 
 extern "C" {

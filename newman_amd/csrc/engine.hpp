@@ -38,7 +38,7 @@ struct HwFrameHost {
   // every coordinate is 0 or at least 2^-200 in magnitude: the device may fold
   // (2x) y + c into fma(x y, 2, c) bit for bit (see plain_update)
   bool fma_ok = false;
-  double c_bound = 0.0;            // at least every |c| of the frame (pair_test_ok)
+  double c_bound = 0.0;            // at least every |c| of the frame (grouped_test_ok)
   std::vector<double> cre, crelo;  // nc
   std::vector<double> cim, cimlo;  // nr
 };
@@ -118,7 +118,7 @@ struct RenderStats {
   int graph = 0;             // 1: ms_total is the mean of HIP graph replays
   int tile = 0;              // the sample order used
   int fma2 = 0;              // plain path: the folded update ran (plain_update)
-  int paired = 0;            // plain path: updates were tested in pairs (pair_test_ok)
+  int grouped = 0;           // plain path: updates were tested in groups of four (grouped_test_ok)
   int block_k = 0, waves_per_simd = 0;
 };
 

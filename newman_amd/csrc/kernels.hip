@@ -414,146 +414,202 @@ __device__ __forceinline__ void park(double& x, double& y, double& r2, double& i
   x = y = r2 = i2 = cx = cy = 0.0;
 }
 
-// The escape test of a pair of updates.  For bailout radius R, 2 <= R <= 2^100,
-// and every |c| <= 0.999 (R^2 - R): when the test of update j holds (computed
-// |z_j|^2 > R^2) and that of update j - 1 did not, the test of update j + 1 holds
-// too.  |z_j|^2 > R^2 (1 - 2^-51) exactly, so the computed |z_{j+1}| is at least
+// The escape test of a group of four updates.  For bailout radius R, 2 <= R <=
+// 2^30, and every |c| <= 0.999 (R^2 - R): when the test of an update holds
+// (computed |z_j|^2 > R^2), that of the next update holds too.  |z_j|^2 >
+// R^2 (1 - 2^-51) exactly, so the computed |z_{j+1}| is at least
 // |z_j|^2 (1 - 2^-49) - |c| (1 + 2^-49) >= R + 0.001 R (R - 1) - 2^-48 R^2 >=
-// R + 0.0004 R^2, and its computed square exceeds R^2; |z_j| <= 2 R^2 bounds
-// |z_{j+1}|^2 by (4 R^4 + R^2)^2 < 2^805 (no overflow, no NaN).  So the pair tests only its
-// second update's |z|^2 and, on an escape there, finds the first escaping update
-// from the first update's r2 + i2, the reference's sqMag of that update bit for
-// bit (r2 and i2 stay in registers): one add and one compare-and-branch less
-// per two updates.  A lane that escaped at j runs update j + 1 unrecorded.
-inline bool pair_test_ok(double radius, double c_bound) {
-  return radius >= 2.0 && radius <= 0x1p100 && c_bound <= 0.999 * (radius * radius - radius);
+// R + 0.0004 R^2, and its computed square exceeds R^2.  The test of the update
+// before the group did not hold, so |z| <= 2 R^2, 5 R^4, 26 R^8, 677 R^16 over
+// the group's four updates and every value stays below 2^980 (no overflow, no
+// NaN).  So a group tests only its last update's |z|^2 and, on an escape there,
+// finds the first escaping update from the earlier updates' r2 + i2, the
+// reference's sqMag of each bit for bit (their r2 and i2 stay in registers):
+// three adds and three compare-and-branches less per four updates.  A lane that
+// escaped inside the group runs the group's remaining updates unrecorded.
+inline bool grouped_test_ok(double radius, double c_bound) {
+  return radius >= 2.0 && radius <= 0x1p30 && c_bound <= 0.999 * (radius * radius - radius);
 }
 
-// Updates k0 + J .. k0 + 3 of a lockstep block of four (unrolled by recursion);
-// PAIR tests pairs of updates (pair_test_ok).
-template <int J, bool FMA2, bool PAIR>
-__device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int k0) {
-  if constexpr (J < 4 && PAIR) {
-    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
-    const double r2p = L.r2, i2p = L.i2;
-    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
-    const double mg = L.r2 + L.i2;
-    const bool e = mg > b2;
-    const unsigned long long esc = __ballot(e);
-    if (__builtin_expect(esc != 0, 0)) {
-      L.live &= ~esc;
-      if (e) {
-        const double mp = r2p + i2p;  // sqMag of the pair's first update
-        const bool first = mp > b2;
-        L.kesc = k0 + (first ? J : J + 1);
-        L.mgesc = first ? mp : mg;
-        L.alive = false;
-        park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
+// G updates and their escape test: G = 1 tests every update as the reference
+// does; G = 2 or 4 tests only the group's last |z|^2 (grouped_test_ok).  When
+// some lane's test holds, the wave calls esc(bool e, int j, double mg) once: e
+// is the lane's test, j (0..G-1) its group's first escaping update and mg that
+// update's |z|^2, r2 + i2 as the reference's sqMag (mandelbrot.cpp:245), kept
+// in registers for the earlier updates (valid where e holds).
+template <int G, bool FMA2, class Esc>
+__device__ __forceinline__ void update_group(double& x, double& y, double& r2, double& i2, const double cx,
+                                             const double cy, const double b2, Esc&& esc) {
+  static_assert(G == 1 || G == 2 || G == 4, "groups of 1, 2 or 4 updates");
+  double r2s[G], i2s[G];
+#pragma unroll
+  for (int t = 0; t < G; t++) {
+    plain_update<FMA2>(x, y, r2, i2, cx, cy);
+    r2s[t] = r2;
+    i2s[t] = i2;
+  }
+  const double md = r2 + i2;
+  const bool e = md > b2;
+  if (__builtin_expect(__ballot(e) != 0, 0)) {
+    int j = G - 1;
+    double mg = md;
+#pragma unroll
+    for (int t = G - 2; t >= 0; t--) {
+      const double m = r2s[t] + i2s[t];
+      if (m > b2) {
+        j = t;
+        mg = m;
       }
     }
-    pre_block<J + 2, FMA2, PAIR>(L, b2, k0);
-  } else if constexpr (J < 4) {
-    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
-    const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
-    const bool e = mg > b2;
-    const unsigned long long esc = __ballot(e);
-    if (__builtin_expect(esc != 0, 0)) {
-      L.live &= ~esc;
+    esc(e, j, mg);
+  }
+}
+
+// Updates k0 + J .. k0 + 3 of a lockstep block of four in groups of G
+// (unrolled by recursion).
+template <int J, int G, bool FMA2>
+__device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int k0) {
+  if constexpr (J < 4) {
+    update_group<G, FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy, b2, [&](bool e, int j, double mg) {
+      L.live &= ~__ballot(e);
       if (e) {
-        L.kesc = k0 + J;
+        L.kesc = k0 + J + j;
         L.mgesc = mg;
         L.alive = false;
         park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
       }
-    }
-    pre_block<J + 1, FMA2, PAIR>(L, b2, k0);
+    });
+    pre_block<J + G, G, FMA2>(L, b2, k0);
   }
 }
 
-template <bool FMA2, bool PAIR>
+// Group sizes of the grouped escape test: the start kernel's pre-iteration,
+// where most lanes escape within a few updates, and the iteration kernel.
+constexpr int kStartGroup = 4;
+constexpr int kIterGroup = 4;
+
+// The frame record at a wave-uniform index, through the constant address space
+// (the records are written before the launch and only read by it), so the
+// fields load once into scalar registers.
+__device__ __forceinline__ DevFrame uniform_frame(const DevFrame* frames, uint32_t idx) {
+#if __HIP_DEVICE_COMPILE__
+  typedef __attribute__((address_space(4))) const uint32_t ConstWord;
+  const ConstWord* w = (const ConstWord*)(frames + __builtin_amdgcn_readfirstlane(idx));
+  uint32_t v[sizeof(DevFrame) / 4];
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(DevFrame) / 4; i++) v[i] = w[i];
+  DevFrame f;
+  __builtin_memcpy(&f, v, sizeof f);
+  return f;
+#else
+  return frames[idx];
+#endif
+}
+
+// Per-workgroup counters of the start kernel.
+struct StartCounts {
+  unsigned long long ncard = 0, nres = 0, executed = 0, credited = 0;
+};
+
+// One sample of the start kernel (getIterationsHW, mandelbrot.cpp:231-254): the
+// lane's sample is frame-local index gid - f.sample_base of frame f when valid.
+// Every lane of the wave runs this (no early return), so the pre-iteration is
+// wave-uniform control flow.  Returns whether the sample is handed on.
+template <bool FMA2, bool GROUPED>
+__device__ __forceinline__ bool start_sample(const PlainArgs& a, const DevFrame& f, const uint32_t gid, bool valid,
+                                             const int pre, const double2* ltab, StartCounts& n) {
+  const Common& c = a.c;
+  const int N = c.N;
+  bool in = false;
+  double cx = 0.0, cy = 0.0;
+  if (valid) {
+    const Loc lc = locate(f, gid - f.sample_base);
+    valid = lc.valid;
+    if (valid) {
+      cx = a.cre[f.tab_c + lc.c];
+      cy = a.cim[f.tab_r + lc.r];
+      if (f.card_mode == CARD_ALL) in = true;
+      else if (f.card_mode == CARD_TEST_DD && card_box(cx, cy)) {
+        int fc = cardioid_box_filter(cx, cy);
+        if (fc == 0) {
+          const double xl = a.crelo[f.tab_c + lc.c], yl = a.cimlo[f.tab_r + lc.r];
+          fc = dd_in_cardioid(cx, xl, cy, yl) ? 1 : -1;
+        }
+        in = fc > 0;
+      }
+    }
+  }
+  // z <- sq(z) + z0 in the reference's operation order (complex.h:19-27), in
+  // lockstep blocks of four updates while some lane of the wave iterates
+  int kesc = -1;
+  bool alive = valid && !in;
+  double zx = alive ? cx : 0.0, zy = alive ? cy : 0.0;  // lanes not iterating hold z = c = 0
+  double x = zx, y = zy, r2 = x * x, i2 = y * y, mgesc = 0.0;
+  unsigned long long live = __ballot(alive);
+  int kend = 0;
+  PreLane L{x, y, r2, i2, zx, zy, live, alive, kesc, mgesc};
+  while (live != 0 && kend < pre) {
+    pre_block<0, GROUPED ? kStartGroup : 1, FMA2>(L, c.b2, kend);
+    kend += 4;
+  }
+  bool want = false;
+  if (valid && in) {
+    c.out[gid] = EscapeValue{N, 0.0f};
+    n.ncard++;
+  } else if (valid) {
+    // the reference tests k >= N before each update: an escape counts
+    // only below N; a lane still iterating after kend updates is handed on
+    if (kesc >= 0 && kesc < N) {
+      n.executed += (unsigned long long)(kesc + 1);
+      c.out[gid] = EscapeValue{kesc, smooth_of(mgesc, c, ltab)};
+      n.nres++;
+      n.credited += (unsigned long long)kesc;
+    } else if (kesc >= N || kend >= N) {
+      n.executed += (unsigned long long)N;
+      c.out[gid] = EscapeValue{N, 0.0f};
+      n.nres++;
+      n.credited += (unsigned long long)N;
+    } else {
+      n.executed += (unsigned long long)kend;
+      // kend == k_start for every lane handed on (the lockstep loop ran to
+      // the first multiple of 4 at or past pre_iter)
+      c.state[gid] = make_double4(x, y, cx, cy);
+      want = true;
+    }
+  }
+  if (gid < c.total) c.flag[gid] = want ? 1 : 0;
+  return want;
+}
+
+template <bool FMA2, bool GROUPED>
 __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
   const Common& c = a.c;
   __shared__ double2 ltab[kLog2Tab];
   load_log2tab(ltab);
-  unsigned long long ncard = 0, nres = 0, executed = 0, credited = 0;
+  StartCounts n;
   const uint32_t ntiles = start_tiles(c.total);
-  const int N = c.N;
-  const int pre = min(a.pre_iter, N);
+  const int pre = min(a.pre_iter, c.N);
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     uint32_t mine = 0;
-    // a tile inside one frame reads its frame record through scalar loads
     uint32_t f0, f1;
     tile_frame_span(tile, c.total, c.inv_frame, &f0, &f1);
-    const bool one_frame = f0 == f1;
-    for (int j = 0; j < kPerThread; j++) {
-      // every lane of the wave runs this body (no early continue), so the
-      // pre-iteration below is wave-uniform control flow
-      const uint32_t gid = start_gid(tile, j, threadIdx.x);
-      bool valid = gid < c.total;
-      bool in = false;
-      double cx = 0.0, cy = 0.0;
-      if (valid) {
-        const DevFrame& f = c.frames[one_frame ? __builtin_amdgcn_readfirstlane(f0) : frame_of(gid, c.inv_frame)];
-        const Loc lc = locate(f, gid - f.sample_base);
-        valid = lc.valid;
-        if (valid) {
-          cx = a.cre[f.tab_c + lc.c];
-          cy = a.cim[f.tab_r + lc.r];
-          if (f.card_mode == CARD_ALL) in = true;
-          else if (f.card_mode == CARD_TEST_DD && card_box(cx, cy)) {
-            int fc = cardioid_box_filter(cx, cy);
-            if (fc == 0) {
-              const double xl = a.crelo[f.tab_c + lc.c], yl = a.cimlo[f.tab_r + lc.r];
-              fc = dd_in_cardioid(cx, xl, cy, yl) ? 1 : -1;
-            }
-            in = fc > 0;
-          }
-        }
+    if (f0 == f1) {  // the tile lies in one frame: its record in scalar registers
+      const DevFrame f = uniform_frame(c.frames, f0);
+      for (int j = 0; j < kPerThread; j++) {
+        const uint32_t gid = start_gid(tile, j, threadIdx.x);
+        mine += start_sample<FMA2, GROUPED>(a, f, gid, gid < c.total, pre, ltab, n) ? 1 : 0;
       }
-      // z <- sq(z) + z0 in the reference's operation order (complex.h:19-27), in
-      // lockstep blocks of four updates while some lane of the wave iterates
-      int kesc = -1;
-      bool alive = valid && !in;
-      double zx = alive ? cx : 0.0, zy = alive ? cy : 0.0;  // lanes not iterating hold z = c = 0
-      double x = zx, y = zy, r2 = x * x, i2 = y * y, mgesc = 0.0;
-      unsigned long long live = __ballot(alive);
-      int kend = 0;
-      PreLane L{x, y, r2, i2, zx, zy, live, alive, kesc, mgesc};
-      while (live != 0 && kend < pre) {
-        pre_block<0, FMA2, PAIR>(L, c.b2, kend);
-        kend += 4;
+    } else {
+      for (int j = 0; j < kPerThread; j++) {
+        const uint32_t gid = start_gid(tile, j, threadIdx.x);
+        const bool valid = gid < c.total;
+        const DevFrame& f = c.frames[valid ? frame_of(gid, c.inv_frame) : 0u];
+        mine += start_sample<FMA2, GROUPED>(a, f, gid, valid, pre, ltab, n) ? 1 : 0;
       }
-      bool want = false;
-      if (valid && in) {
-        c.out[gid] = EscapeValue{N, 0.0f};
-        ncard++;
-      } else if (valid) {
-        // the reference tests k >= N before each update: an escape counts
-        // only below N; a lane still iterating after kend updates is handed on
-        if (kesc >= 0 && kesc < N) {
-          executed += (unsigned long long)(kesc + 1);
-          c.out[gid] = EscapeValue{kesc, smooth_of(mgesc, c, (const double2*)ltab)};
-          nres++;
-          credited += (unsigned long long)kesc;
-        } else if (kesc >= N || kend >= N) {
-          executed += (unsigned long long)N;
-          c.out[gid] = EscapeValue{N, 0.0f};
-          nres++;
-          credited += (unsigned long long)N;
-        } else {
-          executed += (unsigned long long)kend;
-          // kend == k_start for every lane handed on (the lockstep loop ran to
-          // the first multiple of 4 at or past pre_iter)
-          c.state[gid] = make_double4(x, y, cx, cy);
-          want = true;
-        }
-      }
-      if (gid < c.total) c.flag[gid] = want ? 1 : 0;
-      mine += want ? 1 : 0;
     }
     tile_count_store(c.tile_count, tile, mine);
   }
-  const unsigned long long v[4] = {ncard, nres, executed, credited};
+  const unsigned long long v[4] = {n.ncard, n.nres, n.executed, n.credited};
   const int idx[4] = {C_CARDIOID, C_SA_RES, C_EXEC_START, C_CREDITED};
   block_partial(c.part, kSlotStart + blockIdx.x, v, idx, 4);
 }
@@ -780,47 +836,26 @@ struct PlainLane {
   double& mgesc;  // |z|^2 there
 };
 
-// Updates J .. K - 1 of a block (unrolled by recursion).  The handler of an
-// escape records the update and |z|^2 and parks the lane (the block's end
-// writes them); the no-escape case falls through.  PAIR tests pairs of updates
-// (pair_test_ok).
-template <int J, int K, bool FMA2, bool PAIR>
+// Updates J .. K - 1 of a block in groups of G (unrolled by recursion).  The
+// handler of an escape records the update and |z|^2 and parks the lane (the
+// block's end writes them); the no-escape case falls through.
+template <int J, int K, int G, bool FMA2>
 __device__ __forceinline__ void plain_block(PlainLane& L, const double b2) {
-  static_assert(!PAIR || K % 2 == 0, "a paired block holds whole pairs");
-  if constexpr (J < K && PAIR) {
-    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
-    const double r2p = L.r2, i2p = L.i2;
-    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
-    const double mg = L.r2 + L.i2;
-    const bool e = mg > b2;
-    if (__builtin_expect(__ballot(e) != 0, 0)) {
+  static_assert(K % G == 0, "a block holds whole groups");
+  if constexpr (J < K) {
+    update_group<G, FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy, b2, [&](bool e, int j, double mg) {
       if (e) {
-        const double mp = r2p + i2p;  // sqMag of the pair's first update
-        const bool first = mp > b2;
-        L.jesc = first ? J : J + 1;
-        L.mgesc = first ? mp : mg;
-        L.act = false;
-        park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
-      }
-    }
-    plain_block<J + 2, K, FMA2, PAIR>(L, b2);
-  } else if constexpr (J < K) {
-    plain_update<FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
-    const double mg = L.r2 + L.i2;  // sqMag (mandelbrot.cpp:245)
-    const bool e = mg > b2;
-    if (__builtin_expect(__ballot(e) != 0, 0)) {
-      if (e) {
-        L.jesc = J;
+        L.jesc = J + j;
         L.mgesc = mg;
         L.act = false;
         park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
       }
-    }
-    plain_block<J + 1, K, FMA2, PAIR>(L, b2);
+    });
+    plain_block<J + G, K, G, FMA2>(L, b2);
   }
 }
 
-template <int K, bool FMA2, bool PAIR>
+template <int K, bool FMA2, bool GROUPED>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_plain_iterate(
     PlainArgs a) {
   const Common& c = a.c;
@@ -885,7 +920,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     int jesc = -1;
     double mgesc = 0.0;
     PlainLane L{x, y, r2, i2, cx, cy, act, jesc, mgesc};
-    plain_block<0, K, FMA2, PAIR>(L, c.b2);
+    plain_block<0, K, GROUPED ? kIterGroup : 1, FMA2>(L, c.b2);
     if (was) {
       // the reference tests k >= N before each update: an escape at k0 + j
       // counts only below N, and a sample that reaches N in the block ends there
@@ -2277,21 +2312,21 @@ class EngineImpl : public Engine {
     // 256-thread workgroups = 4 waves = one per SIMD: num_cu * waves workgroups
     // the folded update (plain_update) needs every frame's coordinates clear of
     // tiny nonzero values and a bailout of at most 2^400
-    bool fma2 = opt.bailout <= 0x1p400, pair = true;
+    bool fma2 = opt.bailout <= 0x1p400, grouped = true;
     for (auto* f : frames) {
       fma2 = fma2 && f->fma_ok;
-      pair = pair && pair_test_ok(opt.bailout, f->c_bound);
+      grouped = grouped && grouped_test_ok(opt.bailout, f->c_bound);
     }
     if (stats) {
       stats->fma2 = fma2 ? 1 : 0;
-      stats->paired = pair ? 1 : 0;
+      stats->grouped = grouped ? 1 : 0;
     }
 #define NM_PLAIN(KK)                                                                                           \
-  return fma2 && pair ? run(a, k_plain_start<true, true>, k_plain_iterate<KK, true, true>, waves, opt, stats,  \
+  return fma2 && grouped ? run(a, k_plain_start<true, true>, k_plain_iterate<KK, true, true>, waves, opt, stats,  \
                             nout, host_out, emit_to)                                                           \
          : fma2      ? run(a, k_plain_start<true, false>, k_plain_iterate<KK, true, false>, waves, opt, stats, \
                            nout, host_out, emit_to)                                                            \
-         : pair      ? run(a, k_plain_start<false, true>, k_plain_iterate<KK, false, true>, waves, opt, stats, \
+         : grouped   ? run(a, k_plain_start<false, true>, k_plain_iterate<KK, false, true>, waves, opt, stats, \
                            nout, host_out, emit_to)                                                            \
                      : run(a, k_plain_start<false, false>, k_plain_iterate<KK, false, false>, waves, opt,      \
                            stats, nout, host_out, emit_to)
@@ -2778,12 +2813,12 @@ int render_hw_noalloc(const HwFrameView& f, int N, double bailout, EscapeValue* 
   a.crelo = tabs + f.nc;
   a.cim = tabs + 2 * f.nc;
   a.cimlo = tabs + 2 * f.nc + f.nr;
-  const bool fma2 = f.fma_ok && bailout <= 0x1p400, pair = pair_test_ok(bailout, f.c_bound);
-  NA_HIP(fma2 && pair ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<true, true>,
+  const bool fma2 = f.fma_ok && bailout <= 0x1p400, grouped = grouped_test_ok(bailout, f.c_bound);
+  NA_HIP(fma2 && grouped ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<true, true>,
                                        k_plain_iterate<64, true, true>, 8, nullptr, nullptr)
          : fma2       ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<true, false>,
                                        k_plain_iterate<64, true, false>, 8, nullptr, nullptr)
-         : pair       ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<false, true>,
+         : grouped    ? launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<false, true>,
                                        k_plain_iterate<64, false, true>, 8, nullptr, nullptr)
                       : launch_pass_on(s, d->num_cu, part, nactive, a, k_plain_start<false, false>,
                                        k_plain_iterate<64, false, false>, 8, nullptr, nullptr));

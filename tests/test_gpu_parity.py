@@ -51,32 +51,32 @@ def test_plain_path_config1_full_frame(gpu):
     assert st["credited"] == esc.sum() - st["n_cardioid"] * 1024
 
 
-# (radius, escape test once per pair of updates): the frame's |c| reaches
-# |-2.25 + 1.125i| = 2.52, so the pair test needs 0.999 (R^2 - R) >= 2.52 and R <= 2^100
-@pytest.mark.parametrize("bailout,paired", [(2.0, False), (1.5, False), (3.0, True), (1e6, True),
-                                            (2.0 ** 101, False)])
-def test_plain_path_bailout_2(gpu, bailout, paired):
+# (radius, escape test once per group of four updates): the frame's |c| reaches
+# |-2.25 + 1.125i| = 2.52, so the grouped test needs 0.999 (R^2 - R) >= 2.52 and R <= 2^30
+@pytest.mark.parametrize("bailout,grouped", [(2.0, False), (1.5, False), (3.0, True), (1e6, True),
+                                             (1e9, True), (2.0 ** 31, False)])
+def test_plain_path_bailout_2(gpu, bailout, grouped):
     """Other bailout radii (the configuration names radius 2): the smoothing's
     series logarithms for radii >= 2, the library's below 2; the escape test per
-    update and per pair of updates."""
+    update and per group of four updates."""
     m, o = _pair(120, 160, 1024, "-0.75", "0", "3.0", bailout=bailout)
     st = m.render()
-    assert st["paired"] == int(paired)
+    assert st["grouped"] == int(grouped)
     g = m.grid()
     ref = o.render_rows(np.arange(120))
     assert np.array_equal(g["iterations"], ref["iterations"])
     assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
 
 
-@pytest.mark.parametrize("bailout,paired", [(1024.0, True), (2.0, False)])
-def test_plain_path_unfolded_update(gpu, bailout, paired):
+@pytest.mark.parametrize("bailout,grouped", [(1024.0, True), (2.0, False)])
+def test_plain_path_unfolded_update(gpu, bailout, grouped):
     """A frame whose row coordinates include 1e-70 (nonzero and below 2^-200)
     runs the three-operation update y' = (2x) y + c (the fold into one fma is
-    exact only for coordinates 0 or >= 2^-200), with and without the pair test."""
+    exact only for coordinates 0 or >= 2^-200), with and without the grouped test."""
     m, o = _pair(61, 80, 700, "-0.75", "1e-70", "3.0", bailout=bailout)
     assert m.useHardware()
     st = m.render()
-    assert st["fma2"] == 0 and st["paired"] == int(paired)
+    assert st["fma2"] == 0 and st["grouped"] == int(grouped)
     g = m.grid()
     ref = o.render_rows(np.arange(61))
     assert np.array_equal(g["iterations"], ref["iterations"])
@@ -297,12 +297,16 @@ def test_frame_batch_equals_single_renders(gpu):
         assert np.array_equal(batch[k]["smoothing"], one[0]["smoothing"])
 
 
-def test_plain_batch_equals_single_frame(gpu):
+# 48 x 64 = 3 start tiles of 1024 samples per frame; 37 x 29 = 1073 samples, so
+# start tiles straddle frames (the per-sample frame lookup) and the widths differ
+@pytest.mark.parametrize("nr,nc", [(48, 64), (37, 29)])
+def test_plain_batch_equals_single_frame(gpu, nr, nc):
     import newman_amd as nm
-    m = nm.Mandelbrot(48, 64)
+    m = nm.Mandelbrot(nr, nc)
+    widths = ["3.0", "2.5", "3.0", "1.25", "3.0"]
     m.set_view(500, "-0.75", "0", "3.0")
-    batch, _ = m.render_frames(["3.0"] * 5)
-    m.render()
-    g = m.grid()
-    for k in range(5):
-        assert np.array_equal(batch[k], g)
+    batch, _ = m.render_frames(widths)
+    for k, w in enumerate(widths):
+        m.set_view(500, "-0.75", "0", w)
+        m.render()
+        assert np.array_equal(batch[k], m.grid()), (k, w)
