@@ -8,7 +8,10 @@
 //    the launch grid: the tile's frame span, the sample's frame (and that the
 //    one-frame shortcut names the sample's own frame), its position, the column /
 //    row table entries, the cardioid-mask word, the tile-skip entry, and the
-//    per-sample and per-tile stores;
+//    per-sample, per-wave (hand-on word) and per-tile stores;
+//  * k_scatter: for hand-on words with pseudo-random bits on the valid sample
+//    ids, its word loads, and that the positions it writes are exactly
+//    0 .. count - 1, each once, in sample order;
 //  * k_emit: every sample's output element of the row-major grids, and that the
 //    valid samples cover every pixel of every frame exactly once;
 //  * the active-list chunks of both iterate kernels for a range of list
@@ -58,7 +61,8 @@ extern "C" {
 // skip_T: tile side of per-tile series skips (0: none).  plant (the checker's
 // own negative controls, 0 for none): 1 moves the last frame's column tables
 // one entry on, 2 gives the frames one sample more than the sample order has,
-// 3 interleaves the plain chunks with one chunk too few.
+// 3 interleaves the plain chunks with one chunk too few, 4 counts a lane's own
+// bit into its scatter position.
 long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int mask, int skip_T, int plant,
                          char* msg, int cap, unsigned long long* checked) {
   using namespace nm;
@@ -106,7 +110,11 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
       for (int j = 0; j < kPerThread; j++)
         for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
           const uint32_t gid = start_gid(t, j, tid);
-          if (gid >= total) continue;  // an idle lane: the kernels load and store nothing for it
+          if (tid % 64 == 0) {  // lane 0 stores the wave's hand-on word, idle waves too
+            if (gid % 64) R.fail("wave of sample %u does not start a hand-on word", gid);
+            R.in(hand_word(gid), hand_words(total), "hand-on words", gid);
+          }
+          if (gid >= total) continue;  // an idle lane: the kernels load and store nothing else for it
           const uint32_t fi = f0 == f1 ? f0 : frame_of(gid, inv_frame);
           if (fi != gid / fs) R.fail("sample %u: frame %u, want %llu", gid, fi, (unsigned long long)(gid / fs));
           if (frame_of(gid, inv_frame) != gid / fs) R.fail("sample %u: frame_of is not the quotient", gid);
@@ -131,6 +139,53 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
                  "tile skips", gid);
         }
     }
+
+  // ---- k_scatter over hand-on words with pseudo-random bits on valid ids
+  {
+    const uint32_t nwords = hand_words(total);
+    std::vector<uint64_t> words(nwords, 0);
+    uint64_t x = 0x9E3779B97F4A7C15ull ^ total;
+    for (uint32_t gid = 0; gid < total; gid++) {
+      x ^= x << 13;
+      x ^= x >> 7;
+      x ^= x << 17;
+      if ((x & 7) < 3) words[hand_word(gid)] |= 1ull << (gid % 64);  // about 3 in 8 marked
+    }
+    std::vector<uint32_t> off(ntiles, 0);
+    uint64_t count = 0;
+    for (uint32_t t = 0; t < ntiles; t++) {
+      off[t] = (uint32_t)count;
+      for (int k = 0; k < kHandWordsPerTile; k++) count += (uint64_t)__builtin_popcountll(words[t * kHandWordsPerTile + k]);
+    }
+    std::vector<int64_t> at(count, -1);
+    for (uint32_t t = 0; t < ntiles; t++) {
+      uint32_t before = 0;
+      for (int k = 0; k < kHandWordsPerTile; k++) {
+        const uint64_t wi = (uint64_t)t * kHandWordsPerTile + k;
+        if (!R.in(wi, nwords, "hand-on words (scatter)", (uint64_t)t * kTile)) continue;
+        const uint64_t w = words[wi];
+        for (uint32_t l = 0; l < 64; l++) {
+          if (!((w >> l) & 1ull)) continue;
+          const uint32_t gid = t * kTile + (uint32_t)k * 64 + l;
+          const uint32_t pos = hand_position(off[t], before, w, plant == 4 ? l + 1 : l);
+          if (!R.in(pos, count, "active (scatter)", gid)) continue;
+          if (at[pos] >= 0) R.fail("active position %u written twice (sample %u)", pos, gid);
+          at[pos] = gid;
+        }
+        before += (uint32_t)__builtin_popcountll(w);
+      }
+    }
+    for (uint64_t p = 0; p < count; p++) {
+      if (at[p] < 0) {
+        R.fail("active position %llu written by no sample", (unsigned long long)p);
+        break;
+      }
+      if (p && at[p] <= at[p - 1]) {
+        R.fail("active position %llu out of sample order", (unsigned long long)p);
+        break;
+      }
+    }
+  }
 
   // ---- k_emit: sample order -> the row-major grids
   for (uint32_t gid = 0; gid < total; gid++) {

@@ -9,7 +9,8 @@
 //   out[]           EscapeValue {int32, float} per sample, in sample order (the
 //                   grid.h layout itself for row-major orders; k_emit scatters
 //                   tiled orders into the row-major grids)
-//   flag[]          u8 per sample: 1 = handed to the iteration kernel
+//   hand[]          u64 per 64 samples: bit l = sample handed to the iteration
+//                   kernel (one word per wave and sample slot of the start kernels)
 //   state[]         per sample: delta (or z) and epsilon (or c), 4 x f64; it, m
 //   r2[]            |Y|^2 at the escape of iterated samples (smoothing input)
 //   active[]        u32 sample ids that iterate, compacted in sample order
@@ -23,9 +24,10 @@
 //   *_start     tiles of 1024 samples: cardioid/bulb test and up to pre_iter
 //               updates (plain path), or the reference's per-sample series start
 //               (series path: stop index, series skip, escape inside the skip,
-//               first step); resolves what it can, flags the rest, counts per tile
+//               first step); resolves what it can, marks the rest (one ballot
+//               word per wave), counts per tile
 //   scan        exclusive scan of the tile counts (one workgroup)
-//   scatter     compacts flagged sample ids into active[] in sample order
+//   scatter     compacts the marked sample ids into active[] in sample order
 //   *_iterate   persistent waves, one sample per lane; blocks of K steps; the
 //               first escape of a lane in a block records its step and |z|^2
 //               (FP32X: and poisons the lane with NaN); lanes are refilled (ballot +
@@ -98,7 +100,7 @@ struct Common {
   double* r2;  // |z|^2 at the escape of iterated samples (per sample)
   double4* state;  // per sample: plain (x, y, cx, cy); series: (dr, di) as double2[]
   int2* state_i;   // per sample, series path: (it, m)
-  uint8_t* flag;
+  uint64_t* hand;  // hand-on bits (hand_word)
   uint32_t* tile_count;
   uint32_t* active;
   const uint32_t* nactive;  // written by the scan
@@ -577,8 +579,13 @@ __device__ __forceinline__ bool start_sample(const PlainArgs& a, const DevFrame&
       want = true;
     }
   }
-  if (gid < c.total) c.flag[gid] = want ? 1 : 0;
   return want;
+}
+
+// The wave's hand-on word of a sample slot (every lane calls it).
+__device__ __forceinline__ void store_hand(uint64_t* hand, uint32_t gid, bool want) {
+  const unsigned long long w = __ballot(want);
+  if (lane_id() == 0) hand[hand_word(gid)] = w;
 }
 
 template <bool FMA2, bool GROUPED>
@@ -597,14 +604,18 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
       const DevFrame f = uniform_frame(c.frames, f0);
       for (int j = 0; j < kPerThread; j++) {
         const uint32_t gid = start_gid(tile, j, threadIdx.x);
-        mine += start_sample<FMA2, GROUPED>(a, f, gid, gid < c.total, pre, ltab, n) ? 1 : 0;
+        const bool want = start_sample<FMA2, GROUPED>(a, f, gid, gid < c.total, pre, ltab, n);
+        store_hand(c.hand, gid, want);
+        mine += want ? 1 : 0;
       }
     } else {
       for (int j = 0; j < kPerThread; j++) {
         const uint32_t gid = start_gid(tile, j, threadIdx.x);
         const bool valid = gid < c.total;
         const DevFrame& f = c.frames[valid ? frame_of(gid, c.inv_frame) : 0u];
-        mine += start_sample<FMA2, GROUPED>(a, f, gid, valid, pre, ltab, n) ? 1 : 0;
+        const bool want = start_sample<FMA2, GROUPED>(a, f, gid, valid, pre, ltab, n);
+        store_hand(c.hand, gid, want);
+        mine += want ? 1 : 0;
       }
     }
     tile_count_store(c.tile_count, tile, mine);
@@ -696,49 +707,34 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint3
   if (threadIdx.x == 0) *nactive = carry;
 }
 
-// flagged sample ids -> active[] in sample order.  A thread takes 4 consecutive
-// flags (one 32-bit load: flags are 0/1 bytes, kTile = 4 x the workgroup), so a
-// tile is one wave scan and one barrier pair; the next tile's loads are issued
-// before this tile's scan.
-static_assert(kTile == 4 * kThreads, "k_scatter: 4 flags per thread per tile");
-__device__ __forceinline__ uint32_t scatter_flags(const uint8_t* flag, uint32_t g0, uint32_t total) {
-  if (g0 + 4 <= total) return *(const uint32_t*)(flag + g0);
-  uint32_t w = 0;
-  for (int k = 0; k < 4; k++)
-    if (g0 + k < total && flag[g0 + k]) w |= 1u << (8 * k);
-  return w;
-}
-
-__global__ __launch_bounds__(kThreads) void k_scatter(const uint8_t* flag, const uint32_t* tile_off, uint32_t total,
+// marked sample ids -> active[] in sample order: one wave per tile, lanes
+// 0..15 holding its 16 hand-on words (one load each) and their exclusive prefix
+// of set bits; then for each word (scalar) every lane writes its own sample's id
+// at hand_position (consecutive lanes, consecutive positions).
+static_assert(kHandWordsPerTile == 16, "k_scatter: 16 words per tile, one per lane of lanes 0..15");
+__global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* hand, const uint32_t* tile_off, uint32_t total,
                                                       uint32_t* active) {
-  __shared__ uint32_t wcount[kThreads / 64];
-  const uint32_t ntiles = (total + kTile - 1) / kTile;
-  const int wave = threadIdx.x >> 6, lane = lane_id();
-  uint32_t tile = blockIdx.x;
-  if (tile >= ntiles) return;
-  uint32_t w_next = scatter_flags(flag, tile * kTile + 4 * threadIdx.x, total), off_next = tile_off[tile];
-  for (; tile < ntiles; tile += gridDim.x) {
-    const uint32_t g0 = tile * kTile + 4 * threadIdx.x;
-    const uint32_t w = w_next, off = off_next;
-    const uint32_t nt = tile + gridDim.x;
-    if (nt < ntiles) {
-      w_next = scatter_flags(flag, nt * kTile + 4 * threadIdx.x, total);
-      off_next = tile_off[nt];
-    }
-    const uint32_t n = (uint32_t)__popc(w);
-    uint32_t x = n;  // inclusive wave scan
-    for (int o = 1; o < 64; o <<= 1) {
+  const uint32_t ntiles = start_tiles(total);
+  const uint32_t lane = (uint32_t)lane_id();
+  const uint32_t waves = gridDim.x * (kThreads / 64);
+  for (uint32_t tile = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); tile < ntiles; tile += waves) {
+    const uint64_t w = lane < (uint32_t)kHandWordsPerTile ? hand[tile * kHandWordsPerTile + lane] : 0ull;
+    const uint32_t n = (uint32_t)__popcll(w);
+    uint32_t x = n;  // inclusive scan over lanes 0..15
+    for (int o = 1; o < kHandWordsPerTile; o <<= 1) {
       const uint32_t y = __shfl_up(x, o);
-      if (lane >= o) x += y;
+      if (lane >= (uint32_t)o) x += y;
     }
-    if (lane == 63) wcount[wave] = x;
-    __syncthreads();
-    uint32_t before = 0;
-    for (int v = 0; v < kThreads / 64; v++) before += v < wave ? wcount[v] : 0u;
-    uint32_t pos = off + before + x - n;
-    for (int k = 0; k < 4; k++)
-      if ((w >> (8 * k)) & 1u) active[pos++] = g0 + k;
-    __syncthreads();
+    const uint32_t before = x - n;
+    const uint32_t off = __builtin_amdgcn_readfirstlane(tile_off[tile]);
+#pragma unroll 4
+    for (int k = 0; k < kHandWordsPerTile; k++) {
+      const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)w, k), hi = __builtin_amdgcn_readlane((uint32_t)(w >> 32), k);
+      const uint64_t wk = ((uint64_t)hi << 32) | lo;
+      if (wk == 0) continue;  // wave-uniform
+      const uint32_t bk = __builtin_amdgcn_readlane(before, k);
+      if ((wk >> lane) & 1ull) active[hand_position(off, bk, wk, lane)] = tile * kTile + (uint32_t)k * 64 + lane;
+    }
   }
 }
 
@@ -1299,90 +1295,88 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
         const Loc lc = locate(f, gid - f.sample_base);
         skip_tile = fi * a.skip_tpf + (lc.r / a.skip_T) * a.skip_ntc + lc.c / a.skip_T;
       }
-      // phase 3: the reference's series start for this sample
-      if (!valid) {
-        if (gid < c.total) c.flag[gid] = 0;
-        continue;
-      }
+      // phase 3 (valid lanes): the reference's series start for this sample
       bool want = false;
-      EscapeValue res{c.N, 0.0f};
-      if (in) {
-        ncard++;
-      } else {
-        // eps2 = sq(eps), eps3 = eps * eps2 (complex.h:19-21, mandelbrot.cpp:160-161)
-        double2 e2, e3;
-        EpsFx pf;
-        if constexpr (FXS) {
-          pf = eps_fx(e);
+      if (valid) {
+        EscapeValue res{c.N, 0.0f};
+        if (in) {
+          ncard++;
         } else {
-          e2 = make_double2(e.x * e.x - e.y * e.y, 2.0 * e.x * e.y);
-          e3 = cmul(e, e2);
-        }
-        auto term = [&](int i) {
-          if constexpr (FXS) return sa_term_fx(a, i, e, pf);
-          else return sa_term(a, i, e, e2, e3);
-        };
-        int ua;
-        if (a.tile_ua) ua = a.tile_ua[skip_tile];
-        else if constexpr (FXS) ua = sa_stop_index_fx(a, L, br, pf);
-        else ua = sa_stop_index(a, L, br, e2, e3);
-        int dsize = ua ? max(ua - 3, 1) : a.len;
-        int found = dsize - 1;
-        double2 d = term(found);
-        double2 zf = a.Zd[found + 1], xl = a.Xlo[found];
-        double yr = trunc_sum3(zf.x, xl.x, d.x), yi = trunc_sum3(zf.y, xl.y, d.y);
-        if (yr * yr + yi * yi > c.b2) {
-          // escape inside the skip: the reference's binary search (mandelbrot.cpp:186-206)
-          int low = 0, high = dsize - 1, mid = dsize / 2;
-          while (low <= high) {
-            double2 dm = term(mid);
-            double2 zm = a.Zd[mid + 1], xm = a.Xlo[mid];
-            double ym = trunc_sum3(zm.x, xm.x, dm.x), yn = trunc_sum3(zm.y, xm.y, dm.y);
-            if (!(ym * ym + yn * yn > c.b2)) low = mid + 1;
-            else {
-              high = mid - 1;
-              found = mid;
+          // eps2 = sq(eps), eps3 = eps * eps2 (complex.h:19-21, mandelbrot.cpp:160-161)
+          double2 e2, e3;
+          EpsFx pf;
+          if constexpr (FXS) {
+            pf = eps_fx(e);
+          } else {
+            e2 = make_double2(e.x * e.x - e.y * e.y, 2.0 * e.x * e.y);
+            e3 = cmul(e, e2);
+          }
+          auto term = [&](int i) {
+            if constexpr (FXS) return sa_term_fx(a, i, e, pf);
+            else return sa_term(a, i, e, e2, e3);
+          };
+          int ua;
+          if (a.tile_ua) ua = a.tile_ua[skip_tile];
+          else if constexpr (FXS) ua = sa_stop_index_fx(a, L, br, pf);
+          else ua = sa_stop_index(a, L, br, e2, e3);
+          int dsize = ua ? max(ua - 3, 1) : a.len;
+          int found = dsize - 1;
+          double2 d = term(found);
+          double2 zf = a.Zd[found + 1], xl = a.Xlo[found];
+          double yr = trunc_sum3(zf.x, xl.x, d.x), yi = trunc_sum3(zf.y, xl.y, d.y);
+          if (yr * yr + yi * yi > c.b2) {
+            // escape inside the skip: the reference's binary search (mandelbrot.cpp:186-206)
+            int low = 0, high = dsize - 1, mid = dsize / 2;
+            while (low <= high) {
+              double2 dm = term(mid);
+              double2 zm = a.Zd[mid + 1], xm = a.Xlo[mid];
+              double ym = trunc_sum3(zm.x, xm.x, dm.x), yn = trunc_sum3(zm.y, xm.y, dm.y);
+              if (!(ym * ym + yn * yn > c.b2)) low = mid + 1;
+              else {
+                high = mid - 1;
+                found = mid;
+              }
+              mid = (low + high) / 2;
             }
-            mid = (low + high) / 2;
+            d = term(found);
+            zf = a.Zd[found + 1];
+            xl = a.Xlo[found];
+            yr = trunc_sum3(zf.x, xl.x, d.x);
+            yi = trunc_sum3(zf.y, xl.y, d.y);
+            res = EscapeValue{found, smooth_of(yr * yr + yi * yi, c)};
+            nres++;
+          } else if (found + 1 < c.N) {
+            // first perturbation step from Y[found] = X[found] + d (its escape
+            // test was the one above); rebase if |Y| < |d|
+            double dr = d.x, di = d.y;
+            int m = found + 1;
+            double zr = zf.x, zi = zf.y;
+            double pr = zr + dr, pi = zi + di;
+            double r2 = pr * pr + pi * pi;
+            double dmag = fma(dr, dr, di * di);
+            double wr = zr + pr, wi = zi + pi;
+            if (r2 < dmag || m == a.mend) {
+              dr = pr;
+              di = pi;
+              wr = pr;
+              wi = pi;
+              m = 0;
+            }
+            double ndr = fma(dr, wr, fma(-di, wi, e.x));
+            double ndi = fma(dr, wi, fma(di, wr, e.y));
+            // the delta only: the iterate kernel reads epsilon from the tables
+            ((double2*)c.state)[gid] = make_double2(ndr, ndi);
+            c.state_i[gid] = make_int2(found + 1, m + 1);
+            want = true;
+            skipped += (unsigned long long)(found + 1);
+          } else {
+            nres++;  // the skip reached N: (N, 0) as the empty HP loop returns
           }
-          d = term(found);
-          zf = a.Zd[found + 1];
-          xl = a.Xlo[found];
-          yr = trunc_sum3(zf.x, xl.x, d.x);
-          yi = trunc_sum3(zf.y, xl.y, d.y);
-          res = EscapeValue{found, smooth_of(yr * yr + yi * yi, c)};
-          nres++;
-        } else if (found + 1 < c.N) {
-          // first perturbation step from Y[found] = X[found] + d (its escape
-          // test was the one above); rebase if |Y| < |d|
-          double dr = d.x, di = d.y;
-          int m = found + 1;
-          double zr = zf.x, zi = zf.y;
-          double pr = zr + dr, pi = zi + di;
-          double r2 = pr * pr + pi * pi;
-          double dmag = fma(dr, dr, di * di);
-          double wr = zr + pr, wi = zi + pi;
-          if (r2 < dmag || m == a.mend) {
-            dr = pr;
-            di = pi;
-            wr = pr;
-            wi = pi;
-            m = 0;
-          }
-          double ndr = fma(dr, wr, fma(-di, wi, e.x));
-          double ndi = fma(dr, wi, fma(di, wr, e.y));
-          // the delta only: the iterate kernel reads epsilon from the tables
-          ((double2*)c.state)[gid] = make_double2(ndr, ndi);
-          c.state_i[gid] = make_int2(found + 1, m + 1);
-          want = true;
-          skipped += (unsigned long long)(found + 1);
-        } else {
-          nres++;  // the skip reached N: (N, 0) as the empty HP loop returns
         }
+        if (!want) c.out[gid] = res;
+        if (!want && !in) credited += (unsigned long long)res.iterations;
       }
-      if (!want) c.out[gid] = res;
-      if (!want && !in) credited += (unsigned long long)res.iterations;
-      c.flag[gid] = want ? 1 : 0;
+      store_hand(c.hand, gid, want);
       mine += want ? 1 : 0;
     }
     tile_count_store(c.tile_count, tile, mine);
@@ -1889,7 +1883,7 @@ static hipError_t launch_pass_on(hipStream_t stream, int num_cu, unsigned long l
   NM_LAUNCH_OK(hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args));
   if (evs && (e = hipEventRecord(evs[1], stream)) != hipSuccess) return e;
   NM_LAUNCH_OK(hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, c.tile_count, ntiles, nactive));
-  NM_LAUNCH_OK(hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint8_t*)c.flag,
+  NM_LAUNCH_OK(hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint64_t*)c.hand,
                                   (const uint32_t*)c.tile_count, c.total, c.active));
   if (evs && (e = hipEventRecord(evs[2], stream)) != hipSuccess) return e;
   NM_LAUNCH_OK(hipLaunchKernelGGL(iter, dim3(giter), dim3(ti), 0, stream, args));
@@ -1926,7 +1920,7 @@ class EngineImpl : public Engine {
   void* d_tabs = nullptr; size_t n_tabs_bytes = 0;
   void* d_mask = nullptr; size_t n_mask_bytes = 0;
   void* d_out = nullptr; size_t n_out_bytes = 0;
-  void* d_flag = nullptr; size_t n_flag_bytes = 0;
+  void* d_hand = nullptr; size_t n_hand_bytes = 0;
   void* d_r2 = nullptr; size_t n_r2_bytes = 0;
   void* d_tiles = nullptr; size_t n_tiles_bytes = 0;
   void* d_active = nullptr; size_t n_active_bytes = 0;
@@ -2099,7 +2093,7 @@ class EngineImpl : public Engine {
     uint64_t ntiles = (total + kTile - 1) / kTile;
     int st;
     if ((st = grow(&d_out, &n_out_bytes, total * sizeof(EscapeValue)))) return st;
-    if ((st = grow(&d_flag, &n_flag_bytes, total))) return st;
+    if ((st = grow(&d_hand, &n_hand_bytes, (uint64_t)hand_words((uint32_t)total) * 8))) return st;
     if ((st = grow(&d_r2, &n_r2_bytes, total * 8))) return st;
     if ((st = grow(&d_state, &n_state_bytes, total * sizeof(double4)))) return st;
     if ((st = grow(&d_state_i, &n_state_i_bytes, total * sizeof(int2)))) return st;
@@ -2111,7 +2105,7 @@ class EngineImpl : public Engine {
     c.r2 = (double*)d_r2;
     c.state = (double4*)d_state;
     c.state_i = (int2*)d_state_i;
-    c.flag = (uint8_t*)d_flag;
+    c.hand = (uint64_t*)d_hand;
     c.tile_count = (uint32_t*)d_tiles;
     c.active = (uint32_t*)d_active;
     c.nactive = d_nactive;
@@ -2663,7 +2657,7 @@ int Engine::instance(Engine** out) {
 // ================================================ allocation-free plain render
 // render_hw_noalloc: the plain path (getIterationsHW, mandelbrot.cpp:231-254) of
 // one frame with every device buffer carved from `scratch`, a file-scope device
-// array: coordinate tables, the frame record, per-sample results, flags,
+// array: coordinate tables, the frame record, per-sample results, hand-on words,
 // states, |z|^2, the active list, tile counts and the counters.  It calls no
 // hipMalloc/hipFree, creates no stream or event (the null stream) and touches no
 // engine buffer, for callers that cannot allocate device memory.  Each device
@@ -2679,7 +2673,8 @@ constexpr size_t na_align(size_t b) { return (b + 255) & ~(size_t)255; }
 constexpr size_t na_bytes(uint64_t n, uint64_t nr, uint64_t nc) {
   return na_align((2 * nc + 2 * nr) * sizeof(double)) + na_align(sizeof(DevFrame)) +
          na_align(n * sizeof(EscapeValue)) + na_align(n * sizeof(double)) + na_align(n * sizeof(double4)) +
-         na_align(n) + na_align((n + kTile - 1) / kTile * 4) + na_align(n * 4) +
+         na_align((n + kTile - 1) / kTile * kHandWordsPerTile * 8) + na_align((n + kTile - 1) / kTile * 4) +
+         na_align(n * 4) +
          na_align((size_t)kPartWords * sizeof(unsigned long long)) + na_align(4);
 }
 constexpr size_t kNoAllocScratchBytes =
@@ -2747,7 +2742,7 @@ int render_hw_noalloc(const HwFrameView& f, int N, double bailout, EscapeValue* 
   NoAllocDevice* d = nullptr;
   int st = noalloc_setup(dev, &d, err);
   if (st != NM_OK) return st;
-  // the layout: tables, frame, out, r2, state, flag, tile counts, active,
+  // the layout: tables, frame, out, r2, state, hand-on words, tile counts, active,
   // counters, active count
   char* b = (char*)d->ptr;
   size_t off = 0;
@@ -2761,7 +2756,7 @@ int render_hw_noalloc(const HwFrameView& f, int N, double bailout, EscapeValue* 
   EscapeValue* out = (EscapeValue*)take(n * sizeof(EscapeValue));
   double* r2 = (double*)take(n * sizeof(double));
   double4* state = (double4*)take(n * sizeof(double4));
-  uint8_t* flag = (uint8_t*)take(n);
+  uint64_t* hand = (uint64_t*)take((size_t)hand_words((uint32_t)n) * 8);
   uint32_t* tiles = (uint32_t*)take((n + kTile - 1) / kTile * 4);
   uint32_t* active = (uint32_t*)take(n * 4);
   unsigned long long* part = (unsigned long long*)take((size_t)kPartWords * sizeof(unsigned long long));
@@ -2797,7 +2792,7 @@ int render_hw_noalloc(const HwFrameView& f, int N, double bailout, EscapeValue* 
   c.r2 = r2;
   c.state = state;
   c.state_i = nullptr;  // the plain path keeps no per-sample count
-  c.flag = flag;
+  c.hand = hand;
   c.tile_count = tiles;
   c.active = active;
   c.nactive = nactive;

@@ -77,6 +77,21 @@ __host__ __device__ inline void tile_frame_span(uint32_t tile, uint32_t total, d
   *f1 = frame_of(b, inv_frame);
 }
 
+// The start kernels' hand-on bits: one 64-bit word per wave and sample slot (64
+// consecutive sample ids from a multiple of 64; bit l = lane l's sample goes to
+// the iterate kernel), written by every wave of every tile, so no clearing is
+// needed; a tile owns words 16 t .. 16 t + 15.
+constexpr int kHandWordsPerTile = kTile / 64;
+__host__ __device__ inline uint32_t hand_words(uint32_t total) { return start_tiles(total) * kHandWordsPerTile; }
+__host__ __device__ inline uint32_t hand_word(uint32_t gid) { return gid >> 6; }
+// k_scatter: the active-list position of bit l of word k of a tile, given the
+// tile's offset (the exclusive scan of the tile counts) and the set bits of the
+// tile's earlier words (before); sample ids keep their order
+__host__ __device__ inline uint32_t hand_position(uint32_t tile_off, uint32_t before, uint64_t word, uint32_t l) {
+  const uint64_t lower = l ? word & (~0ull >> (64 - l)) : 0ull;
+  return tile_off + before + (uint32_t)__builtin_popcountll(lower);
+}
+
 // ---- the active list (sample ids that iterate) as the iterate kernels read it
 constexpr int kChunk = 64;   // active samples per chunk (one per lane)
 constexpr int kQueues = 8;   // claim ranges of the chunks

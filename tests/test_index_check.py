@@ -58,10 +58,11 @@ def test_every_kernel_index_is_in_bounds(case):
     assert n > 0
 
 
-@pytest.mark.parametrize("plant", [1, 2, 3])
+@pytest.mark.parametrize("plant", [1, 2, 3, 4])
 def test_the_checker_finds_planted_mistakes(plant):
     """Negative controls of the checker itself: a column-table offset one entry
-    on, frames one sample longer than their sample order, and plain chunks
-    interleaved with one chunk too few are each reported."""
+    on, frames one sample longer than their sample order, plain chunks
+    interleaved with one chunk too few, and scatter positions that count a
+    lane's own bit are each reported."""
     v, msg, _ = _check(3, 7, 5, 16, 7, 1, 8, plant=plant)
     assert v > 0 and msg
