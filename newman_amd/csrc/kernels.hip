@@ -17,10 +17,11 @@
 //   reference block Zd (the orbit as f64 pairs, Zd[0] = 0, padded past its end),
 //                   Xlo, A, B, C and the stop-search tables
 //
-// Pipeline per render pass, with no device-scope atomics on the hot path (one
-// contended counter serialises at ~10 ns per atomic on this part):
-//   k_zero      clears the per-workgroup counter slots (a kernel, so a captured
-//               HIP graph holds no memset node)
+// Pipeline per render pass, with no contended device-scope atomic on the hot
+// path (one contended counter serialises at ~10 ns per atomic on this part; the
+// start kernels add one count per wave and tile, four adds per counter):
+//   k_zero      clears the per-workgroup counter slots and the tile counts (a
+//               kernel, so a captured HIP graph holds no memset node)
 //   *_start     tiles of 1024 samples: cardioid/bulb test and up to pre_iter
 //               updates (plain path), or the reference's per-sample series start
 //               (series path: stop index, series skip, escape inside the skip,
@@ -323,25 +324,10 @@ __device__ __forceinline__ double trunc_sum3(double hi, double lo, double d) {
   return r;
 }
 
-// Each workgroup walks its tiles; the block-wide count of flagged samples of a
-// tile goes to tile_count[tile].
-__device__ __forceinline__ void tile_count_store(uint32_t* tile_count, uint32_t tile, uint32_t mine) {
-  __shared__ uint32_t wsum[kThreads / 64];
-  unsigned long long s = wave_sum((unsigned long long)mine);
-  if (lane_id() == 0) wsum[threadIdx.x >> 6] = (uint32_t)s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int w = 0; w < kThreads / 64; w++) t += wsum[w];
-    tile_count[tile] = t;
-  }
-  __syncthreads();
-}
-
 // ================================================================ plain path
 // getIterationsHW (mandelbrot.cpp:231-254), start kernel: the cardioid/bulb
 // test, then up to kPreIter updates per sample with the exact escape test (most
-// samples of a shallow view escape here, converged); the rest are flagged for
+// samples of a shallow view escape here, converged); the rest are marked for
 // the persistent kernel with their state (z, c, updates done).
 constexpr int kPreIterDefault = 16;
 
@@ -582,10 +568,18 @@ __device__ __forceinline__ bool start_sample(const PlainArgs& a, const DevFrame&
   return want;
 }
 
-// The wave's hand-on word of a sample slot (every lane calls it).
-__device__ __forceinline__ void store_hand(uint64_t* hand, uint32_t gid, bool want) {
+// The wave's hand-on word of a sample slot (every lane calls it); returns its
+// count of marked samples (wave-uniform).
+__device__ __forceinline__ uint32_t store_hand(uint64_t* hand, uint32_t gid, bool want) {
   const unsigned long long w = __ballot(want);
   if (lane_id() == 0) hand[hand_word(gid)] = w;
+  return (uint32_t)__popcll(w);
+}
+
+// A wave's count of marked samples of a tile into tile_count[tile] (cleared by
+// k_zero): one atomic add per wave and tile, no barrier between the waves.
+__device__ __forceinline__ void tile_count_add(uint32_t* tile_count, uint32_t tile, uint32_t wave_count) {
+  if (lane_id() == 0 && wave_count) atomicAdd(&tile_count[tile], wave_count);
 }
 
 template <bool FMA2, bool GROUPED>
@@ -605,8 +599,7 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
       for (int j = 0; j < kPerThread; j++) {
         const uint32_t gid = start_gid(tile, j, threadIdx.x);
         const bool want = start_sample<FMA2, GROUPED>(a, f, gid, gid < c.total, pre, ltab, n);
-        store_hand(c.hand, gid, want);
-        mine += want ? 1 : 0;
+        mine += store_hand(c.hand, gid, want);
       }
     } else {
       for (int j = 0; j < kPerThread; j++) {
@@ -614,11 +607,10 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
         const bool valid = gid < c.total;
         const DevFrame& f = c.frames[valid ? frame_of(gid, c.inv_frame) : 0u];
         const bool want = start_sample<FMA2, GROUPED>(a, f, gid, valid, pre, ltab, n);
-        store_hand(c.hand, gid, want);
-        mine += want ? 1 : 0;
+        mine += store_hand(c.hand, gid, want);
       }
     }
-    tile_count_store(c.tile_count, tile, mine);
+    tile_count_add(c.tile_count, tile, mine);
   }
   const unsigned long long v[4] = {n.ncard, n.nres, n.executed, n.credited};
   const int idx[4] = {C_CARDIOID, C_SA_RES, C_EXEC_START, C_CREDITED};
@@ -1376,10 +1368,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 8))
         if (!want) c.out[gid] = res;
         if (!want && !in) credited += (unsigned long long)res.iterations;
       }
-      store_hand(c.hand, gid, want);
-      mine += want ? 1 : 0;
+      mine += store_hand(c.hand, gid, want);
     }
-    tile_count_store(c.tile_count, tile, mine);
+    tile_count_add(c.tile_count, tile, mine);
   }
   const unsigned long long v[4] = {ncard, nres, skipped, credited};
   const int idx[4] = {C_CARDIOID, C_SA_RES, C_SKIPPED, C_CREDITED};
@@ -1836,8 +1827,10 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_partials(const unsign
   }
 }
 
-__global__ void k_zero(unsigned long long* p, int n) {
+// clears the counter slots p[0, n) and the tile counts q[0, m)
+__global__ void k_zero(unsigned long long* p, int n, uint32_t* q, uint32_t m) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) q[i] = 0u;
 }
 
 // ================================================================== engine
@@ -1878,7 +1871,7 @@ static hipError_t launch_pass_on(hipStream_t stream, int num_cu, unsigned long l
     expr;                                    \
     if ((e = hipGetLastError()) != hipSuccess) return e; \
   } while (0)
-  NM_LAUNCH_OK(hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, part, kPartWords));
+  NM_LAUNCH_OK(hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, stream, part, kPartWords, c.tile_count, ntiles));
   if (evs && (e = hipEventRecord(evs[0], stream)) != hipSuccess) return e;
   NM_LAUNCH_OK(hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args));
   if (evs && (e = hipEventRecord(evs[1], stream)) != hipSuccess) return e;
