@@ -1,6 +1,7 @@
 // engine.hpp - host-side interface of the HIP render engine (no HIP types here,
 // so the host classes compile with any C++ compiler).
 #pragma once
+#include <cmath>
 #include <cstdint>
 #include <cstddef>
 #include <string>
@@ -38,7 +39,7 @@ struct HwFrameHost {
   // every coordinate is 0 or at least 2^-200 in magnitude: the device may fold
   // (2x) y + c into fma(x y, 2, c) bit for bit (see plain_update)
   bool fma_ok = false;
-  double c_bound = 0.0;            // at least every |c| of the frame (grouped_test_ok)
+  double c_bound = HUGE_VAL;        // at least every |c| of the frame (grouped_test_ok; unknown: +inf)
   std::vector<double> cre, crelo;  // nc
   std::vector<double> cim, cimlo;  // nr
 };
@@ -127,7 +128,7 @@ struct HwFrameView {
   int nr = 0, nc = 0;
   int card_mode = CARD_TEST_DD;
   bool fma_ok = false;
-  double c_bound = 0.0;
+  double c_bound = HUGE_VAL;
   const double *cre = nullptr, *crelo = nullptr;  // nc each
   const double *cim = nullptr, *cimlo = nullptr;  // nr each
 };
