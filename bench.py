@@ -584,7 +584,10 @@ def main():
                          "main line only, as the profiles are taken")
     ap.add_argument("--variants", type=int, default=1,
                     help="also time FP32X deltas and the learned tile skip on the same frame (c2, c5)")
-    ap.add_argument("--tune", action="store_true", help="sweep block size and waves per SIMD first")
+    ap.add_argument("--tune", action="store_true",
+                    help="sweep block size and waves per SIMD first (series: then the chunk interleaving)")
+    ap.add_argument("--chunk-run", type=int, default=0,
+                    help="series path: active-list chunks of 64 consecutive samples (0) or runs of 4, 8, 16, 32")
     ap.add_argument("--cpu-rows-every", type=int, default=36)
     ap.add_argument("--cpu-threads", type=int, default=16)
     args = ap.parse_args()
@@ -620,22 +623,33 @@ def measure(config, args, block_k=None, waves=None):
     w = args.waves if waves is None else waves
     if config != "c4":
         m.set_launch(tile=args.tile if config == args.config else 0,
-                     pre_iter=args.pre_iter if config == args.config else 0)
+                     pre_iter=args.pre_iter if config == args.config else 0,
+                     chunk_run=args.chunk_run if config == args.config else 0)
     tuned = None
     if config == "c4":
         # the project's launch autotuner (sample order x waves x block size), then
         # graph replay of the repeated passes
         t0 = time.perf_counter()
         best, table = m.autotune_frames(step.widths)
-        m.set_launch(best["block_k"], best["waves_per_simd"], best["tile"], graph=True)
+        m.set_launch(best["block_k"], best["waves_per_simd"], best["tile"], graph=True, chunk_run=best["chunk_run"])
         k, w = best["block_k"], best["waves_per_simd"]
         tuned = {"best": best, "sweep": table, "seconds": time.perf_counter() - t0}
         print("# autotune: %s (%.1f s)" % (best, tuned["seconds"]), file=sys.stderr, flush=True)
     elif args.tune and config == args.config:
         best, rows = tune(step, [8, 16, 32, 64] if config == "c1" else [12, 16, 32, 64], [2, 4, 6, 8])
         k, w = best[0], best[1]
-        tuned = {"best": {"block_k": k, "waves_per_simd": w, "ms": best[2]},
+        tuned = {"best": {"block_k": k, "waves_per_simd": w, "chunk_run": args.chunk_run, "ms": best[2]},
                  "sweep": [{"block_k": a, "waves_per_simd": b, "ms": c} for a, b, c in rows]}
+        if config != "c1":  # the series kernel's chunk interleaving at the best setting
+            for run in (4, 16, 32, 64):
+                m.set_launch(tile=args.tile, chunk_run=run)
+                step(k, w)
+                ms = step(k, w)["ms_total"]
+                print("# tune chunk_run=%d: %.3f ms" % (run, ms), file=sys.stderr, flush=True)
+                tuned["sweep"].append({"block_k": k, "waves_per_simd": w, "chunk_run": run, "ms": ms})
+                if ms < tuned["best"]["ms"]:
+                    tuned["best"] = {"block_k": k, "waves_per_simd": w, "chunk_run": run, "ms": ms}
+            m.set_launch(tile=args.tile, chunk_run=tuned["best"]["chunk_run"])
 
     if args.warmup:
         step(k, w, args.warmup)
@@ -659,7 +673,8 @@ def measure(config, args, block_k=None, waves=None):
         "dtype": "f64",
         "data": "synthetic (fixed view)" if config == "c1" else "synthetic (seeded zoom-walk view)",
         "config": dict(desc, parallelism="single GPU", block_k=s["block_k"], waves_per_simd=s["waves_per_simd"],
-                       tile=s["tile"], graph=s["graph"], pre_iter=m.get_launch()["pre_iter"]),
+                       tile=s["tile"], graph=s["graph"], pre_iter=m.get_launch()["pre_iter"],
+                       chunk_run=m.get_launch()["chunk_run"]),
         "roofline": roofline(config, flop, s, fxs=desc.get("series") == "floatexp"),
         "detail": {"credited_px_it": credited, "executed_px_it": s["executed"],
                    "executed_in_start_kernel": s["executed_start"], "skipped": s["skipped"],

@@ -58,7 +58,7 @@ class RenderStats(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block_k", C.c_int32), ("waves_per_simd", C.c_int32), ("tile", C.c_int32), ("graph", C.c_int32),
-                ("pre_iter", C.c_int32), ("reserved", C.c_int32), ("ms", C.c_double)]
+                ("pre_iter", C.c_int32), ("chunk_run", C.c_int32), ("ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -582,8 +582,8 @@ class Mandelbrot:
         _check(lib().nm_get_tile_stats(self._h, C.byref(st)), "nm_get_tile_stats")
         return st.as_dict()
 
-    def set_launch(self, block_k=0, waves_per_simd=0, tile=0, graph=False, pre_iter=0):
-        l = Launch(block_k, waves_per_simd, tile, 1 if graph else 0, pre_iter, 0, 0.0)
+    def set_launch(self, block_k=0, waves_per_simd=0, tile=0, graph=False, pre_iter=0, chunk_run=0):
+        l = Launch(block_k, waves_per_simd, tile, 1 if graph else 0, pre_iter, chunk_run, 0.0)
         _check(lib().nm_set_launch(self._h, C.byref(l)), "nm_set_launch")
 
     def get_launch(self):

@@ -366,11 +366,13 @@ int nm_set_launch(nm_mandelbrot* h, const nm_launch* l) {
   if (l->pre_iter < 0 || l->pre_iter > 1024) return NM_ERR_ARG;
   const int t = l->tile;
   if (t != 0 && t != 8 && t != 16 && t != 32 && t != 64 && t != 128 && t != 256) return NM_ERR_ARG;
+  if (nm::chunk_run_shift(l->chunk_run) < 0) return NM_ERR_ARG;
   h->m.launch.block_k = l->block_k;
   h->m.launch.waves_per_simd = l->waves_per_simd;
   h->m.launch.tile = t;
   h->m.launch.graph = l->graph != 0;
   h->m.launch.pre_iter = l->pre_iter;
+  h->m.launch.chunk_run = l->chunk_run;
   h->m.launch.ms = 0.0;
   return NM_OK;
 }
@@ -382,7 +384,7 @@ int nm_get_launch(const nm_mandelbrot* h, nm_launch* l) {
   l->tile = h->m.launch.tile;
   l->graph = h->m.launch.graph;
   l->pre_iter = h->m.launch.pre_iter;
-  l->reserved = 0;
+  l->chunk_run = h->m.launch.chunk_run;
   l->ms = h->m.launch.ms;
   return NM_OK;
 }
@@ -398,7 +400,7 @@ int nm_autotune_frames(nm_mandelbrot* h, const char* const* widths, int nframes,
     if (best) nm_get_launch(h, best);
     int n = 0;
     for (const auto& r : rows) {
-      if (table && n < cap) table[n] = nm_launch{r.block_k, r.waves_per_simd, r.tile, r.graph, r.pre_iter, 0, r.ms};
+      if (table && n < cap) table[n] = nm_launch{r.block_k, r.waves_per_simd, r.tile, r.graph, r.pre_iter, r.chunk_run, r.ms};
       n++;
     }
     if (n_table) *n_table = std::min(n, cap);

@@ -14,9 +14,9 @@
 //    0 .. count - 1, each once, in sample order;
 //  * k_emit: every sample's output element of the row-major grids, and that the
 //    valid samples cover every pixel of every frame exactly once;
-//  * the active-list chunks of both iterate kernels for a range of list
-//    lengths: every entry read below the length, each entry taken once, and the
-//    claim ranges partitioning the chunks.
+//  * the active-list chunks of both iterate kernels (every run length) for a
+//    range of list lengths: every entry read below the length, each entry taken
+//    once, and the claim ranges partitioning the chunks.
 // Test support, not part of the render path: built as its own library and called
 // from the CPU tests.
 #include <cstdarg>
@@ -221,39 +221,29 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
       next = hi;
     }
     if (next != nch) R.fail("claim ranges end at %u of %u chunks", next, nch);
-    // plain: interleaved chunks
-    took.assign(n, 0);
-    const uint32_t nchp = plant == 3 && nch > 1 ? nch - 1 : nch;
-    for (uint32_t c = 0; c < nch; c++) {
-      uint32_t items = 0;
-      for (uint32_t l = 0; l < (uint32_t)kChunk; l++) items += chunk_pos(l, c, nchp) < n;
-      for (uint32_t l = 0; l < (uint32_t)kChunk; l++) {
-        const uint32_t p = chunk_pos(l, c, nchp);
-        if ((l < items) != (p < n)) R.fail("chunk %u of %u entries: the lanes with entries are not a prefix", c, n);
-        if (l < items && R.in(p, n, "active (plain chunks)", c) && took[p]++)
-          R.fail("active entry %u taken twice", p);
+    // interleaved chunks: runs of 4 (plain), runs of 8 .. 64 (series: 64 is 64
+    // consecutive entries); plant 3 interleaves the plain chunks with one too few
+    for (uint32_t shift = 2; shift <= 6; shift++) {
+      took.assign(n, 0);
+      const uint32_t nchp = plant == 3 && shift == 2 && nch > 1 ? nch - 1 : nch;
+      for (uint32_t c = 0; c < nch; c++) {
+        uint32_t items = 0;
+        auto at = [&](uint32_t l) { return shift == 2 ? chunk_pos(l, c, nchp) : chunk_pos_run(l, c, nchp, shift); };
+        for (uint32_t l = 0; l < (uint32_t)kChunk; l++) items += at(l) < n;
+        for (uint32_t l = 0; l < (uint32_t)kChunk; l++) {
+          const uint32_t p = at(l);
+          if ((l < items) != (p < n))
+            R.fail("chunk %u of %u entries (runs of %u): the lanes with entries are not a prefix", c, n, 1u << shift);
+          if (l < items && R.in(p, n, "active (chunks)", c) && took[p]++)
+            R.fail("active entry %u taken twice (runs of %u)", p, 1u << shift);
+        }
       }
+      for (uint32_t p = 0; p < n; p++)
+        if (!took[p]) {
+          R.fail("active entry %u of %u taken by no chunk (runs of %u)", p, n, 1u << shift);
+          break;
+        }
     }
-    for (uint32_t p = 0; p < n; p++)
-      if (!took[p]) {
-        R.fail("active entry %u of %u taken by no chunk (plain)", p, n);
-        break;
-      }
-    // series: contiguous chunks
-    took.assign(n, 0);
-    for (uint32_t c = 0; c < nch; c++) {
-      const uint32_t len = chunk_len(c, n);
-      for (uint32_t l = 0; l < len; l++) {
-        const uint64_t p = (uint64_t)c * kChunk + l;
-        if (R.in(p, n, "active (series chunks)", c) && took[p]++) R.fail("active entry %llu taken twice", p);
-      }
-    }
-    for (uint32_t p = 0; p < n; p++)
-      if (!took[p]) {
-        R.fail("active entry %u of %u taken by no chunk (series)", p, n);
-        break;
-      }
-    if (chunk_len(0xffffffffu, n) != 0) R.fail("the no-chunk marker has entries");
   }
   if (checked) *checked = R.checked;
   return (long long)R.violations;

@@ -159,6 +159,7 @@ struct SeriesArgs {
   const double2 *Am, *Bm, *Cm;
   const int2 *Ae, *Be, *Ce;
   int fxs;
+  uint32_t run_shift;  // active-list chunks (ChunkMap): runs of 2^run_shift entries
 };
 
 // ------------------------------------------------------------ device helpers
@@ -731,17 +732,11 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* hand, cons
 }
 
 // ---- work pipeline of a persistent wave -------------------------------------
-// Chunk c (64 consecutive entries of active[]) belongs to wave c mod W.  Each
-// lane holds one entry of three chunks: CUR (decoded: handed to lanes that need
-// work through cross-lane shuffles), NEXT (its decode loads in flight) and the
-// chunk after (its sample-id load in flight), so a refill reads registers only.
-// entries of consecutive chunk `chunk` (0 for kNoChunk)
-__device__ __forceinline__ uint32_t chunk_items(uint32_t chunk, uint32_t total) { return chunk_len(chunk, total); }
-
-__device__ __forceinline__ uint32_t load_pix(const uint32_t* active, uint32_t chunk, uint32_t total) {
-  uint32_t n = chunk_items(chunk, total);
-  return (uint32_t)lane_id() < n ? active[chunk * kChunk + lane_id()] : 0u;
-}
+// A wave takes chunks of 64 entries of active[] (ChunkMap) by claims
+// (ChunkClaim).  Each lane holds one entry of the chunks in flight: CUR
+// (decoded: handed to lanes that need work through cross-lane shuffles), NEXT
+// (its decode loads in flight) and, in the plain kernel, the chunk after (its
+// sample-id load in flight), so a refill reads registers only.
 
 // Chunk claims: persistent waves take chunks of the active list dynamically.
 // The chunks are split into kQueues equal ranges with one claim counter each
@@ -787,21 +782,26 @@ struct ChunkClaim {
   }
 };
 
-// The plain iterate kernel's chunks interleave the active list: lane l of chunk
-// c takes entry 4 ((l / 4) nch + c) + l % 4 (16 runs of 4 consecutive entries,
-// nch chunks apart), so every chunk is a systematic sample of the whole list and
-// the per-sample work (from a few updates to N) averages out within a chunk.
-// Runs of 4 adjacent entries keep the 32-byte state loads in 128-byte pieces.
+// Chunks that interleave the active list: lane l of chunk c takes entry
+// R ((l / R) nch + c) + l % R, R = 2^shift (64 / R runs of R consecutive
+// entries, nch chunks apart), so every chunk is a systematic sample of the whole
+// list and the per-sample work (from a few updates to N) averages out within a
+// chunk.  The plain iterate kernel uses R = 4 (runs keep its 32-byte state
+// loads in 128-byte pieces); the series kernel R = 64 (64 consecutive entries:
+// neighbouring samples stay on the orbit's LDS windows together) unless the
+// launch asks for interleaving (deep views whose long samples cluster).
 struct ChunkMap {
-  uint32_t n;    // entries of the active list
-  uint32_t nch;  // chunks: ceil(n / 64)
-  __device__ void init(uint32_t total) {
+  uint32_t n;      // entries of the active list
+  uint32_t nch;    // chunks: ceil(n / 64)
+  uint32_t shift;  // log2 R
+  __device__ void init(uint32_t total, uint32_t run_shift = 2) {
     n = total;
     nch = chunk_count(total);
+    shift = run_shift;
   }
-  // below 4 (16 nch) <= n + 63 < 2^32 for a real chunk (prepare_common caps
+  // below R (64 / R nch) <= n + 63 < 2^32 for a real chunk (prepare_common caps
   // the sample count); kNoChunk is tested first
-  __device__ uint32_t pos(uint32_t c) const { return chunk_pos((uint32_t)lane_id(), c, nch); }
+  __device__ uint32_t pos(uint32_t c) const { return chunk_pos_run((uint32_t)lane_id(), c, nch, shift); }
   // the chunk's entries are the lanes below the returned count (pos grows with
   // the lane)
   __device__ uint32_t items(uint32_t c) const {
@@ -1586,11 +1586,13 @@ void k_series_iterate(SeriesArgs a) {
   }
   __syncthreads();
   const double2* zhead = zlds + kLdsTail;  // rebased lanes read the orbit from its start
+  ChunkMap cm;
+  cm.init(total, a.run_shift);
   ChunkClaim q;
-  q.init(c.part, chunk_count(total));
+  q.init(c.part, cm.nch);
   const uint32_t c0 = q.next(), c1 = q.next();
-  uint32_t ncur = chunk_items(c0, total), nnext = chunk_items(c1, total);
-  uint32_t cur_pix = load_pix(c.active, c0, total), nxt_pix = load_pix(c.active, c1, total);
+  uint32_t ncur = cm.items(c0), nnext = cm.items(c1);
+  uint32_t cur_pix = cm.pix(c.active, c0, ncur), nxt_pix = cm.pix(c.active, c1, nnext);
   const double2* dstate = (const double2*)c.state;  // the series start's deltas
   double2 cur_s = make_double2(0.0, 0.0), cur_e = cur_s;
   int2 cur_i = make_int2(0, 0);
@@ -1643,8 +1645,8 @@ void k_series_iterate(SeriesArgs a) {
           cur_e = sample_eps(a, cur_pix);
         }
         const uint32_t cn = q.next();
-        nnext = chunk_items(cn, total);
-        nxt_pix = load_pix(c.active, cn, total);
+        nnext = cm.items(cn);
+        nxt_pix = cm.pix(c.active, cn, nnext);
         pos = 0;
       }
       need = __ballot(!act);
@@ -2401,6 +2403,9 @@ class EngineImpl : public Engine {
     fill_reference(a, opt);
     a.tile_ua = nullptr;
     a.skip_T = a.skip_ntc = a.skip_tpf = 0;
+    const int run_shift = chunk_run_shift(opt.chunk_run);
+    if (run_shift < 0) return NM_ERR_ARG;
+    a.run_shift = (uint32_t)run_shift;
     if (opt.tile_ua) {
       if (opt.skip_T < 8) return NM_ERR_ARG;
       const uint32_t T = (uint32_t)opt.skip_T, ntc = (frames[0]->nc + T - 1) / T, ntr = (frames[0]->nr + T - 1) / T;

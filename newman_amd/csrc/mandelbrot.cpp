@@ -857,6 +857,7 @@ void Mandelbrot::applyLaunch(RenderOpts& o) const {
   o.tile = launch.tile;
   o.graph = launch.graph != 0;
   if (o.pre_iter == 0) o.pre_iter = launch.pre_iter;
+  if (o.chunk_run == 0) o.chunk_run = launch.chunk_run;
 }
 
 int Mandelbrot::render(RenderOpts* opts, bool copy_back) {
@@ -1307,9 +1308,11 @@ int Mandelbrot::zoomVideoStep(const uint8_t* pal_rgb, int npal, int smooth, int 
 
 // The project's launch autotuner.  Candidates: sample order (row-major, 32x32,
 // 64x64 and 128x128 tiles), persistent waves per SIMD (2, 4, 6, 8) and the block
-// size (8, 16, 32 for the series kernel; 16, 32 for the plain one).  Each candidate
-// renders an evenly spaced subsample of at most 16 frames (so the set's depth
-// mix is kept) once untimed and once timed; the fastest total kernel time wins.
+// size (16, 32, 64; FP32X deltas 8, 16, 32); then, on the series path, the
+// fastest of those with the active list interleaved in runs of 16 (chunk_run).
+// Each candidate renders an evenly spaced subsample of at most 16 frames (so the
+// set's depth mix is kept) once untimed and once timed; the fastest total kernel
+// time wins.
 int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vector<Launch>* table) {
   FrameSet set;
   int st = buildFrames(widths, set);
@@ -1338,6 +1341,7 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
         o.tile = tile;
         o.graph = false;
         o.repeats = 1;
+        o.chunk_run = 64;
         o.set_serial = subset_serial;
         if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
         if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
@@ -1347,10 +1351,29 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
         l.tile = tile;
         l.graph = launch.graph;
         l.pre_iter = launch.pre_iter;
+        l.chunk_run = 0;
         l.ms = stats_.ms_total;
         if (table) table->push_back(l);
         if (l.ms < best.ms) best = l;
       }
+  if (!set.hw) {  // the best setting with interleaved chunks
+    RenderOpts o;
+    o.block_k = best.block_k;
+    o.waves_per_simd = best.waves_per_simd;
+    applyLaunch(o);
+    o.tile = best.tile;
+    o.graph = false;
+    o.repeats = 1;
+    o.chunk_run = 16;
+    o.set_serial = subset_serial;
+    if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
+    if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
+    Launch l = best;
+    l.chunk_run = 16;
+    l.ms = stats_.ms_total;
+    if (table) table->push_back(l);
+    if (l.ms < best.ms) best = l;
+  }
   stats_ = keep;
   launch = best;
   return NM_OK;

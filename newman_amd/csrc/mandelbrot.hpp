@@ -47,7 +47,7 @@ class Mandelbrot {
   // per block, persistent waves per SIMD, sample order (0 row-major, T = T x T
   // tiles) and HIP-graph replay of repeated passes.  autotuneFrames sets them.
   struct Launch {
-    int block_k = 0, waves_per_simd = 0, tile = 0, graph = 0, pre_iter = 0;
+    int block_k = 0, waves_per_simd = 0, tile = 0, graph = 0, pre_iter = 0, chunk_run = 0;
     double ms = 0.0;
   };
   Launch launch;

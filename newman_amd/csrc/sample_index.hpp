@@ -96,18 +96,16 @@ __host__ __device__ inline uint32_t hand_position(uint32_t tile_off, uint32_t be
 constexpr int kChunk = 64;   // active samples per chunk (one per lane)
 constexpr int kQueues = 8;   // claim ranges of the chunks
 __host__ __device__ inline uint32_t chunk_count(uint32_t n) { return (n + kChunk - 1) / kChunk; }
-// contiguous chunks (the series iterate kernel): lane l < chunk_len of chunk c
-// takes entry 64 c + l
-__host__ __device__ inline uint32_t chunk_len(uint32_t chunk, uint32_t n) {
-  const uint64_t b = (uint64_t)chunk * kChunk;
-  return b >= n ? 0u : (uint32_t)((uint64_t)n - b < (uint64_t)kChunk ? (uint64_t)n - b : (uint64_t)kChunk);
+// interleaved chunks: lane l of chunk c takes entry R ((l / R) nch + c) + l % R,
+// R = 2^shift, when that is below n (64 / R runs of R consecutive entries, nch
+// chunks apart; R = 64 is 64 consecutive entries); the entry grows with the
+// lane, so the lanes that take one are a prefix of the wave
+__host__ __device__ inline uint32_t chunk_pos_run(uint32_t lane, uint32_t c, uint32_t nch, uint32_t shift) {
+  return (((lane >> shift) * nch + c) << shift) + (lane & ((1u << shift) - 1u));
 }
-// interleaved chunks (the plain iterate kernel): lane l of chunk c takes entry
-// 4 ((l / 4) nch + c) + l % 4 when that is below n (16 runs of 4 consecutive
-// entries, nch chunks apart); the entry grows with the lane, so the lanes that
-// take one are a prefix of the wave
+// the plain iterate kernel's chunks: runs of 4
 __host__ __device__ inline uint32_t chunk_pos(uint32_t lane, uint32_t c, uint32_t nch) {
-  return 4u * ((lane >> 2) * nch + c) + (lane & 3u);
+  return chunk_pos_run(lane, c, nch, 2);
 }
 // the chunks of claim range h: [claim_lo(nch, h), claim_lo(nch, h + 1))
 __host__ __device__ inline uint32_t claim_lo(uint32_t nch, uint32_t h) {

@@ -205,6 +205,24 @@ def test_series_block_size_keeps_results(gpu, name, nr, nc, width, K):
     assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
 
 
+@pytest.mark.parametrize("run", [4, 8, 16, 32, 64])
+def test_series_chunk_interleaving_keeps_results(gpu, run):
+    """The series kernel's chunk interleaving only changes which wave takes a
+    sample, so every run length gives the oracle's grid bit for bit."""
+    m, o = _deep(*DEEP_CASES[4])
+    m.precompute()
+    o.precompute()
+    m.set_launch(chunk_run=run)
+    m.render()
+    assert m.get_launch()["chunk_run"] == run
+    nr, nc = DEEP_CASES[4][1], DEEP_CASES[4][2]
+    g = m.grid()
+    rc = np.array([(r, c) for r in range(nr) for c in range(nc)], dtype=np.int32)
+    ref = o.render_samples(rc, pert=True).reshape(nr, nc)
+    assert np.array_equal(g["iterations"], ref["iterations"])
+    assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
+
+
 @pytest.mark.parametrize("name,nr,nc,width", DEEP_CASES[:4])
 def test_series_path_vs_reference_hp(gpu, name, nr, nc, width):
     """Against the reference method (SA scan + HP continuation, mandelbrot.cpp:144-229):

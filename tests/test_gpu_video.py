@@ -76,11 +76,16 @@ def test_autotuner_picks_a_measured_setting(gpu):
     m, widths = _video(nm, nframes=20)
     ref, _ = m.render_frames(widths)
     best, table = m.autotune_frames(widths)
-    assert len(table) == 4 * 4 * 3
+    # the sweep, then its fastest setting with interleaved chunks (series sets)
+    assert len(table) == 4 * 4 * 3 + 1
     assert {r["tile"] for r in table} == {0, 32, 64, 128}
     assert {r["block_k"] for r in table} == {16, 32, 64}
+    assert [r["chunk_run"] for r in table] == [0] * 48 + [16]
+    fastest = min(table[:48], key=lambda r: r["ms"])
+    assert {k: table[48][k] for k in ("block_k", "waves_per_simd", "tile")} == \
+        {k: fastest[k] for k in ("block_k", "waves_per_simd", "tile")}
     assert best["ms"] == min(r["ms"] for r in table) and best["ms"] > 0
-    assert m.get_launch()["tile"] == best["tile"]
+    assert (m.get_launch()["tile"], m.get_launch()["chunk_run"]) == (best["tile"], best["chunk_run"])
     got, st = m.render_frames(widths)
     assert (st["block_k"], st["waves_per_simd"], st["tile"]) == (best["block_k"], best["waves_per_simd"],
                                                                   best["tile"])
