@@ -641,7 +641,7 @@ def measure(config, args, block_k=None, waves=None):
         tuned = {"best": {"block_k": k, "waves_per_simd": w, "chunk_run": args.chunk_run, "ms": best[2]},
                  "sweep": [{"block_k": a, "waves_per_simd": b, "ms": c} for a, b, c in rows]}
         if config != "c1":  # the series kernel's chunk interleaving at the best setting
-            for run in (4, 16, 32, 64):
+            for run in (1, 4, 16, 64):
                 m.set_launch(tile=args.tile, chunk_run=run)
                 step(k, w)
                 ms = step(k, w)["ms_total"]

@@ -190,7 +190,7 @@ typedef struct nm_launch {
   int32_t pre_iter; /* plain path: updates tried in the start kernel (0 = default 16) */
   int32_t chunk_run; /* series path: each wave takes 64 consecutive samples of the active list (0 or 64,
                         the default) or 64 / R runs of R consecutive samples spread over the list
-                        (R = 4, 8, 16, 32): spreads deep views' long samples over the waves */
+                        (R = 1, 2, 4, 8, 16, 32): spreads deep views' long samples over the waves */
   double ms;
 } nm_launch;
 int nm_set_launch(nm_mandelbrot* h, const nm_launch* l);

@@ -85,11 +85,14 @@ struct SaReferenceHost {
 };
 
 // The series iterate kernel's active-list chunk setting (RenderOpts::chunk_run;
-// 0 or 64: 64 consecutive entries per chunk; 4, 8, 16, 32: interleaved runs of
-// that many entries, chunk_pos_run) as the run's log2, or -1 when invalid.
+// 0 or 64: 64 consecutive entries per chunk; 1, 2, 4, 8, 16, 32: interleaved
+// runs of that many entries, chunk_pos_run) as the run's log2, or -1 when
+// invalid.
 inline int chunk_run_shift(int run) {
   switch (run) {
     case 0: case 64: return 6;
+    case 1: return 0;
+    case 2: return 1;
     case 4: return 2;
     case 8: return 3;
     case 16: return 4;
@@ -110,7 +113,7 @@ struct RenderOpts {
   bool graph = false;     // replay repeats 2.. from one captured HIP graph
   int pre_iter = 0;       // plain path: updates in the start kernel (0 = default)
   int chunk_run = 0;      // series path: active-list chunks of 64 consecutive samples (0, 64)
-                          // or interleaved runs of 4, 8, 16, 32 (chunk_run_shift)
+                          // or interleaved runs of 1, 2, 4, 8, 16, 32 (chunk_run_shift)
   const int* tile_ua = nullptr;  // series path: per-tile first unstable index (null: per sample)
   int skip_T = 0;                // its tile side
   uint64_t set_serial = 0; // identifies the frame set: the engine keeps its tables

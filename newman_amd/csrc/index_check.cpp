@@ -221,26 +221,26 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
       next = hi;
     }
     if (next != nch) R.fail("claim ranges end at %u of %u chunks", next, nch);
-    // interleaved chunks: runs of 4 (plain), runs of 8 .. 64 (series: 64 is 64
+    // interleaved chunks: runs of 1 (plain), runs of 1 .. 64 (series: 64 is 64
     // consecutive entries); plant 3 interleaves the plain chunks with one too few
-    for (uint32_t shift = 2; shift <= 6; shift++) {
+    for (uint32_t shift = 0; shift <= 7; shift++) {  // 7: the plain kernel's own chunk_pos
       took.assign(n, 0);
-      const uint32_t nchp = plant == 3 && shift == 2 && nch > 1 ? nch - 1 : nch;
+      const uint32_t nchp = plant == 3 && shift == 7 && nch > 1 ? nch - 1 : nch;
       for (uint32_t c = 0; c < nch; c++) {
         uint32_t items = 0;
-        auto at = [&](uint32_t l) { return shift == 2 ? chunk_pos(l, c, nchp) : chunk_pos_run(l, c, nchp, shift); };
+        auto at = [&](uint32_t l) { return shift == 7 ? chunk_pos(l, c, nchp) : chunk_pos_run(l, c, nchp, shift); };
         for (uint32_t l = 0; l < (uint32_t)kChunk; l++) items += at(l) < n;
         for (uint32_t l = 0; l < (uint32_t)kChunk; l++) {
           const uint32_t p = at(l);
           if ((l < items) != (p < n))
-            R.fail("chunk %u of %u entries (runs of %u): the lanes with entries are not a prefix", c, n, 1u << shift);
+            R.fail("chunk %u of %u entries (shift %u): the lanes with entries are not a prefix", c, n, shift);
           if (l < items && R.in(p, n, "active (chunks)", c) && took[p]++)
-            R.fail("active entry %u taken twice (runs of %u)", p, 1u << shift);
+            R.fail("active entry %u taken twice (shift %u)", p, shift);
         }
       }
       for (uint32_t p = 0; p < n; p++)
         if (!took[p]) {
-          R.fail("active entry %u of %u taken by no chunk (runs of %u)", p, n, 1u << shift);
+          R.fail("active entry %u of %u taken by no chunk (shift %u)", p, n, shift);
           break;
         }
     }

@@ -1309,7 +1309,7 @@ int Mandelbrot::zoomVideoStep(const uint8_t* pal_rgb, int npal, int smooth, int 
 // The project's launch autotuner.  Candidates: sample order (row-major, 32x32,
 // 64x64 and 128x128 tiles), persistent waves per SIMD (2, 4, 6, 8) and the block
 // size (16, 32, 64; FP32X deltas 8, 16, 32); then, on the series path, the
-// fastest of those with the active list interleaved in runs of 16 (chunk_run).
+// fastest of those with the active list interleaved in runs of 4 (chunk_run).
 // Each candidate renders an evenly spaced subsample of at most 16 frames (so the
 // set's depth mix is kept) once untimed and once timed; the fastest total kernel
 // time wins.
@@ -1364,12 +1364,12 @@ int Mandelbrot::autotuneFrames(const std::vector<std::string>& widths, std::vect
     o.tile = best.tile;
     o.graph = false;
     o.repeats = 1;
-    o.chunk_run = 16;
+    o.chunk_run = 4;
     o.set_serial = subset_serial;
     if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
     if ((st = renderSet(set, o, nullptr, 0, step)) != NM_OK) return st;
     Launch l = best;
-    l.chunk_run = 16;
+    l.chunk_run = 4;
     l.ms = stats_.ms_total;
     if (table) table->push_back(l);
     if (l.ms < best.ms) best = l;

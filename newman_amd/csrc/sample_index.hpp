@@ -103,9 +103,9 @@ __host__ __device__ inline uint32_t chunk_count(uint32_t n) { return (n + kChunk
 __host__ __device__ inline uint32_t chunk_pos_run(uint32_t lane, uint32_t c, uint32_t nch, uint32_t shift) {
   return (((lane >> shift) * nch + c) << shift) + (lane & ((1u << shift) - 1u));
 }
-// the plain iterate kernel's chunks: runs of 4
+// the plain iterate kernel's chunks: 64 entries nch apart (runs of 1)
 __host__ __device__ inline uint32_t chunk_pos(uint32_t lane, uint32_t c, uint32_t nch) {
-  return chunk_pos_run(lane, c, nch, 2);
+  return chunk_pos_run(lane, c, nch, 0);
 }
 // the chunks of claim range h: [claim_lo(nch, h), claim_lo(nch, h + 1))
 __host__ __device__ inline uint32_t claim_lo(uint32_t nch, uint32_t h) {

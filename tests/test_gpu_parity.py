@@ -205,7 +205,7 @@ def test_series_block_size_keeps_results(gpu, name, nr, nc, width, K):
     assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
 
 
-@pytest.mark.parametrize("run", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("run", [1, 4, 16, 64])
 def test_series_chunk_interleaving_keeps_results(gpu, run):
     """The series kernel's chunk interleaving only changes which wave takes a
     sample, so every run length gives the oracle's grid bit for bit."""

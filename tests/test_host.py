@@ -188,15 +188,15 @@ def test_deep_render_before_precompute_is_a_state_error():
 
 
 def test_launch_chunk_run_argument_checks():
-    """The series kernel's chunk interleaving takes runs of 4, 8, 16, 32 or 64
-    samples (0: the default, 64 consecutive samples); anything else is refused
-    and leaves the launch settings as they were."""
+    """The series kernel's chunk interleaving takes runs of 1, 2, 4, 8, 16, 32 or
+    64 samples (0: the default, 64 consecutive samples); anything else is
+    refused and leaves the launch settings as they were."""
     nm = _nm()
     m = nm.Mandelbrot(8, 8)
-    for run in (0, 4, 8, 16, 32, 64):
+    for run in (0, 1, 2, 4, 8, 16, 32, 64):
         m.set_launch(chunk_run=run)
         assert m.get_launch()["chunk_run"] == run
-    for bad in (-1, 1, 2, 3, 12, 128):
+    for bad in (-1, 3, 12, 128):
         with pytest.raises(nm.NewmanError):
             m.set_launch(chunk_run=bad)
     assert m.get_launch()["chunk_run"] == 64
