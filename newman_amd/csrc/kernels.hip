@@ -387,10 +387,11 @@ __device__ __forceinline__ int cardioid_box_filter(double x, double y) {
 }
 
 // Pre-iteration of the start kernel: a lane's state, its own flag and the
-// wave's scalar mask of lanes still iterating.  A lane that is not iterating
-// holds z = c = 0, which stays 0 and never passes the escape test, so the test
-// of a step needs no mask: a lane leaves at its first escape by recording it and
-// parking at 0.
+// wave's scalar mask of lanes still iterating.  A lane that does not iterate
+// (cardioid, invalid) holds z = c = 0, which stays 0; a lane that escaped
+// records its escape and keeps updating unrecorded (its values overflow to inf
+// or NaN), so the escape test is masked by the wave's mask of lanes still
+// iterating (a scalar AND, instead of parking the lane's six doubles).
 struct PreLane {
   double &x, &y, &r2, &i2, &cx, &cy;
   unsigned long long& live;
@@ -422,13 +423,15 @@ inline bool grouped_test_ok(double radius, double c_bound) {
 
 // G updates and their escape test: G = 1 tests every update as the reference
 // does; G = 2 or 4 tests only the group's last |z|^2 (grouped_test_ok).  When
-// some lane's test holds, the wave calls esc(bool e, int j, double mg) once: e
-// is the lane's test, j (0..G-1) its group's first escaping update and mg that
-// update's |z|^2, r2 + i2 as the reference's sqMag (mandelbrot.cpp:245), kept
-// in registers for the earlier updates (valid where e holds).
+// the test holds for some lane of the wave mask `live`, the wave calls
+// esc(bool e, int j, double mg) once: e is the lane's test, j (0..G-1) its
+// group's first escaping update and mg that update's |z|^2, r2 + i2 as the
+// reference's sqMag (mandelbrot.cpp:245), kept in registers for the earlier
+// updates (valid where e holds on a lane of `live`).
 template <int G, bool FMA2, class Esc>
 __device__ __forceinline__ void update_group(double& x, double& y, double& r2, double& i2, const double cx,
-                                             const double cy, const double b2, Esc&& esc) {
+                                             const double cy, const double b2, const unsigned long long live,
+                                             Esc&& esc) {
   static_assert(G == 1 || G == 2 || G == 4, "groups of 1, 2 or 4 updates");
   double r2s[G], i2s[G];
 #pragma unroll
@@ -439,7 +442,7 @@ __device__ __forceinline__ void update_group(double& x, double& y, double& r2, d
   }
   const double md = r2 + i2;
   const bool e = md > b2;
-  if (__builtin_expect(__ballot(e) != 0, 0)) {
+  if (__builtin_expect((__ballot(e) & live) != 0, 0)) {
     int j = G - 1;
     double mg = md;
 #pragma unroll
@@ -459,13 +462,13 @@ __device__ __forceinline__ void update_group(double& x, double& y, double& r2, d
 template <int J, int G, bool FMA2>
 __device__ __forceinline__ void pre_block(PreLane& L, const double b2, const int k0) {
   if constexpr (J < 4) {
-    update_group<G, FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy, b2, [&](bool e, int j, double mg) {
-      L.live &= ~__ballot(e);
-      if (e) {
+    update_group<G, FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy, b2, L.live, [&](bool e, int j, double mg) {
+      const bool ne = e && L.alive;
+      L.live &= ~__ballot(ne);
+      if (ne) {
         L.kesc = k0 + J + j;
         L.mgesc = mg;
         L.alive = false;
-        park(L.x, L.y, L.r2, L.i2, L.cx, L.cy);
       }
     });
     pre_block<J + G, G, FMA2>(L, b2, k0);
@@ -834,7 +837,7 @@ template <int J, int K, int G, bool FMA2>
 __device__ __forceinline__ void plain_block(PlainLane& L, const double b2) {
   static_assert(K % G == 0, "a block holds whole groups");
   if constexpr (J < K) {
-    update_group<G, FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy, b2, [&](bool e, int j, double mg) {
+    update_group<G, FMA2>(L.x, L.y, L.r2, L.i2, L.cx, L.cy, b2, ~0ull, [&](bool e, int j, double mg) {
       if (e) {
         L.jesc = J + j;
         L.mgesc = mg;
