@@ -498,9 +498,14 @@ __device__ __forceinline__ DevFrame uniform_frame(const DevFrame* frames, uint32
 #endif
 }
 
-// Per-workgroup counters of the start kernel.
+// Per-lane counters of the start kernel, in 32 bits: a sample adds at most
+// pre_iter + 3 <= 1027 updates or credited iterations (the kernel resolves only
+// samples whose count is decided within its lockstep updates), and a lane takes
+// at most 4 samples of each of its workgroup's tiles, at most 2^22 / 16 tiles
+// (the grid is at least 16 workgroups when there are 16 tiles), so a lane's sums
+// stay below 2^20 * 1027 < 2^31.
 struct StartCounts {
-  unsigned long long ncard = 0, nres = 0, executed = 0, credited = 0;
+  uint32_t ncard = 0, nres = 0, executed = 0, credited = 0;
 };
 
 // One sample of the start kernel (getIterationsHW, mandelbrot.cpp:231-254): the
@@ -552,17 +557,17 @@ __device__ __forceinline__ bool start_sample(const PlainArgs& a, const DevFrame&
     // the reference tests k >= N before each update: an escape counts
     // only below N; a lane still iterating after kend updates is handed on
     if (kesc >= 0 && kesc < N) {
-      n.executed += (unsigned long long)(kesc + 1);
+      n.executed += (uint32_t)(kesc + 1);
       c.out[gid] = EscapeValue{kesc, smooth_of(mgesc, c, ltab)};
       n.nres++;
-      n.credited += (unsigned long long)kesc;
+      n.credited += (uint32_t)kesc;
     } else if (kesc >= N || kend >= N) {
-      n.executed += (unsigned long long)N;
+      n.executed += (uint32_t)N;
       c.out[gid] = EscapeValue{N, 0.0f};
       n.nres++;
-      n.credited += (unsigned long long)N;
+      n.credited += (uint32_t)N;
     } else {
-      n.executed += (unsigned long long)kend;
+      n.executed += (uint32_t)kend;
       // kend == k_start for every lane handed on (the lockstep loop ran to
       // the first multiple of 4 at or past pre_iter)
       c.state[gid] = make_double4(x, y, cx, cy);
