@@ -772,7 +772,7 @@ struct ChunkClaim {
     done = 0;
   }
   // the next chunk, or kNoChunk when every range is empty
-  __device__ uint32_t next() {
+  __device__ __forceinline__ uint32_t next() {
     for (int t = 0; t < kQueues; t++) {
       const uint32_t h = (home + (uint32_t)t) & (kQueues - 1);
       if (done & (1u << h)) continue;
@@ -805,21 +805,21 @@ struct ChunkMap {
   uint32_t n;      // entries of the active list
   uint32_t nch;    // chunks: ceil(n / 64)
   uint32_t shift;  // log2 R
-  __device__ void init(uint32_t total, uint32_t run_shift = 0) {
+  __device__ __forceinline__ void init(uint32_t total, uint32_t run_shift = 0) {
     n = total;
     nch = chunk_count(total);
     shift = run_shift;
   }
   // below R (64 / R nch) <= n + 63 < 2^32 for a real chunk (prepare_common caps
   // the sample count); kNoChunk is tested first
-  __device__ uint32_t pos(uint32_t c) const { return chunk_pos_run((uint32_t)lane_id(), c, nch, shift); }
+  __device__ __forceinline__ uint32_t pos(uint32_t c) const { return chunk_pos_run((uint32_t)lane_id(), c, nch, shift); }
   // the chunk's entries are the lanes below the returned count (pos grows with
   // the lane)
-  __device__ uint32_t items(uint32_t c) const {
+  __device__ __forceinline__ uint32_t items(uint32_t c) const {
     const bool ok = c < nch && pos(c) < n;
     return (uint32_t)__popcll(__ballot(ok));
   }
-  __device__ uint32_t pix(const uint32_t* active, uint32_t c, uint32_t cnt) const {
+  __device__ __forceinline__ uint32_t pix(const uint32_t* active, uint32_t c, uint32_t cnt) const {
     return (uint32_t)lane_id() < cnt ? active[pos(c)] : 0u;
   }
 };
@@ -1576,7 +1576,9 @@ __device__ __forceinline__ void pert_block_fx(float& wr, float& wi, int& ex, flo
 // held to 64 VGPRs (8 waves per SIMD, four 512-thread workgroups per CU): only
 // the next chunk's sample ids are prefetched, its states are loaded when it
 // becomes current (a wave waits one load per 64 refills, while the other waves
-// of its SIMD run); one 8-byte value spills, outside the loop.
+// of its SIMD run).  The chunk map and claims are inlined (as calls they kept
+// their objects in scratch and spilled the LDS head pointer, reloaded in the
+// step loop's rebase branch): no scratch at all.
 template <int K, bool FX>
 __global__ __launch_bounds__(kSeriesIterThreads) __attribute__((amdgpu_waves_per_eu(FX || K < 16 ? 1 : 8, 8)))
 void k_series_iterate(SeriesArgs a) {
