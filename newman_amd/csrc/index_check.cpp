@@ -62,7 +62,7 @@ extern "C" {
 // own negative controls, 0 for none): 1 moves the last frame's column tables
 // one entry on, 2 gives the frames one sample more than the sample order has,
 // 3 interleaves the plain chunks with one chunk too few, 4 counts a lane's own
-// bit into its scatter position.
+// bit into its scatter position, 5 launches the tile scan with one group too few.
 long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int mask, int skip_T, int plant,
                          char* msg, int cap, unsigned long long* checked) {
   using namespace nm;
@@ -184,6 +184,83 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
         R.fail("active position %llu out of sample order", (unsigned long long)p);
         break;
       }
+    }
+  }
+
+  // ---- the tile scan (k_tile_sums, then k_scan_tiles) over the scatter's tile
+  // counts: every thread's loads and stores, the groups' carries and the total;
+  // plant 5 launches one group too few
+  {
+    std::vector<uint32_t> cnt(ntiles), seq(ntiles);
+    uint64_t acc = 0;
+    {  // the marked samples of the scatter check's generator, per tile
+      uint64_t x = 0x9E3779B97F4A7C15ull ^ total;
+      for (uint32_t gid = 0; gid < total; gid++) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        if ((x & 7) < 3) cnt[gid / kTile]++;
+      }
+    }
+    for (uint32_t t = 0; t < ntiles; t++) {
+      seq[t] = (uint32_t)acc;
+      acc += cnt[t];
+    }
+    const ScanSplit sp = scan_split(ntiles);
+    const uint32_t groups = plant == 5 ? sp.groups - 1 : sp.groups;
+    if (sp.groups < 1 || sp.groups > kScanGroups) R.fail("tile scan of %u counts: %u groups", ntiles, sp.groups);
+    std::vector<uint32_t> gsum(groups, 0), out(ntiles, 0);
+    std::vector<uint8_t> wrote(ntiles, 0);
+    for (int pass = 0; pass < 2; pass++) {  // 0: k_tile_sums, 1: k_scan_tiles
+      for (uint32_t g = 0; g < groups; g++) {
+        uint32_t lo, hi;
+        scan_group_range(ntiles, sp.per, g, lo, hi);
+        uint32_t carry = 0;
+        for (uint32_t q = 0; q < g; q++) carry += gsum[q];
+        for (uint32_t base = lo; base < hi; base += kScanRound) {
+          uint32_t before = 0;  // the counts of the round's earlier threads (the wave scans)
+          for (uint32_t tid = 0; tid < 1024; tid++) {
+            const uint32_t i0 = base + 16 * tid;
+            uint32_t mine = 0;
+            for (uint32_t k = 0; k < 16; k++) {
+              if (i0 + k >= hi) continue;
+              if (!R.in(i0 + k, ntiles, "tile counts (scan)", (uint64_t)g)) continue;
+              if (pass == 1) {
+                if (wrote[i0 + k]++) R.fail("tile offset %u written twice", i0 + k);
+                out[i0 + k] = carry + before + mine;
+              }
+              mine += cnt[i0 + k];
+            }
+            before += mine;
+          }
+          carry += before;
+          if (pass == 0) gsum[g] += before;
+        }
+        if (pass == 1 && g + 1 == groups && carry != (uint32_t)acc)
+          R.fail("tile scan total %u, want %llu", carry, (unsigned long long)acc);
+      }
+    }
+    for (uint32_t t = 0; t < ntiles; t++)
+      if (!wrote[t] || out[t] != seq[t]) {
+        R.fail("tile offset %u: %s %u, want %u", t, wrote[t] ? "wrote" : "not written", out[t], seq[t]);
+        break;
+      }
+    // the split at edge sizes: at most kScanGroups groups of consecutive,
+    // non-empty count ranges covering [0, ntiles)
+    const uint32_t edge[] = {0u, 1u, kScanRound - 1, kScanRound, kScanRound + 1, kScanGroups * kScanRound,
+                             kScanGroups * kScanRound + 1, 100 * kScanRound + 7, start_tiles(0xFFFFEFFFu), ntiles};
+    for (uint32_t n : edge) {
+      const ScanSplit e = scan_split(n);
+      if (e.groups < 1 || e.groups > kScanGroups) R.fail("tile scan of %u counts: %u groups", n, e.groups);
+      uint32_t next = 0;
+      for (uint32_t g = 0; g < e.groups; g++) {
+        uint32_t lo, hi;
+        scan_group_range(n, e.per, g, lo, hi);
+        if (lo != next || hi < lo || (n > 0 && hi == lo))
+          R.fail("tile scan of %u counts: group %u covers [%u, %u) after %u", n, g, lo, hi, next);
+        next = hi;
+      }
+      if (next != n) R.fail("tile scan of %u counts: the groups end at %u", n, next);
     }
   }
 

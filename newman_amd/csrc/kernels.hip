@@ -630,13 +630,18 @@ __global__ __launch_bounds__(kThreads) void k_plain_start(PlainArgs a) {
 }
 
 // ============================================================ compaction
-// exclusive scan of tile_count (in place) by one workgroup; total -> *nactive.
-// Rounds of 16384 consecutive counts: 16 per thread (four 16-byte loads), wave
-// scans, a carry between rounds; the next round's loads are issued before this
-// round's scan, so one memory latency is paid per round at most.
+// exclusive scan of tile_count (in place), total -> *nactive, in two launches
+// over the groups of scan_split: k_tile_sums writes each group's total of
+// counts, then k_scan_tiles starts each group at the sum of the earlier groups'
+// totals. A workgroup takes a round of 16384 consecutive counts at a time: 16
+// per thread (four 16-byte loads), wave scans, a carry between rounds; the next
+// round's loads are issued before this round's scan, so one memory latency is
+// paid per round at most. (One workgroup over all of a 512-frame 640x480 set's
+// 153,600 counts took 34 us, ten rounds back to back.)
 constexpr int kScanPer = 16;
-__device__ __forceinline__ void scan_load(const uint32_t* tc, uint32_t i0, uint32_t ntiles, uint32_t v[kScanPer]) {
-  if (i0 + kScanPer <= ntiles) {
+static_assert(kScanRound == 1024 * kScanPer, "a scan round is 16 counts for each of 1024 threads");
+__device__ __forceinline__ void scan_load(const uint32_t* tc, uint32_t i0, uint32_t hi, uint32_t v[kScanPer]) {
+  if (i0 + kScanPer <= hi) {
     const uint4* q = (const uint4*)(tc + i0);
 #pragma unroll
     for (int k = 0; k < kScanPer / 4; k++) {
@@ -648,18 +653,44 @@ __device__ __forceinline__ void scan_load(const uint32_t* tc, uint32_t i0, uint3
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < kScanPer; k++) v[k] = i0 + k < ntiles ? tc[i0 + k] : 0u;
+    for (int k = 0; k < kScanPer; k++) v[k] = i0 + k < hi ? tc[i0 + k] : 0u;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint32_t ntiles, uint32_t* nactive) {
+// group blockIdx.x's total of counts -> gsum[blockIdx.x]
+__global__ __launch_bounds__(1024) void k_tile_sums(const uint32_t* tile_count, uint32_t ntiles, uint32_t per,
+                                                     uint32_t* gsum) {
+  __shared__ uint32_t wsum[16];
+  uint32_t lo, hi;
+  scan_group_range(ntiles, per, blockIdx.x, lo, hi);  // the group's counts [lo, hi)
+  uint32_t s = 0;
+  for (uint32_t base = lo; base < hi; base += kScanRound) {
+    uint32_t v[kScanPer];
+    scan_load(tile_count, base + kScanPer * threadIdx.x, hi, v);
+#pragma unroll
+    for (int k = 0; k < kScanPer; k++) s += v[k];
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 16; w++) t += wsum[w];
+    gsum[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint32_t ntiles, uint32_t per,
+                                                      const uint32_t* gsum, uint32_t* nactive) {
   __shared__ uint32_t wsum[16];
   const int lane = lane_id(), wave = threadIdx.x >> 6;
-  constexpr uint32_t kRound = 1024 * kScanPer;
+  uint32_t lo, hi;
+  scan_group_range(ntiles, per, blockIdx.x, lo, hi);  // the group's counts [lo, hi)
   uint32_t carry = 0;
+  for (uint32_t g = 0; g < blockIdx.x; g++) carry += gsum[g];
   uint32_t nxt[kScanPer];
-  scan_load(tile_count, kScanPer * threadIdx.x, ntiles, nxt);
-  for (uint32_t base = 0; base < ntiles; base += kRound) {
+  scan_load(tile_count, lo + kScanPer * threadIdx.x, hi, nxt);
+  for (uint32_t base = lo; base < hi; base += kScanRound) {
     const uint32_t i0 = base + kScanPer * threadIdx.x;
     uint32_t v[kScanPer];
     uint32_t mine = 0;
@@ -668,7 +699,7 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint3
       v[k] = nxt[k];
       mine += v[k];
     }
-    if (base + kRound < ntiles) scan_load(tile_count, i0 + kRound, ntiles, nxt);
+    if (base + kScanRound < hi) scan_load(tile_count, i0 + kScanRound, hi, nxt);
     uint32_t x = mine;  // inclusive wave scan of the thread totals
     for (int off = 1; off < 64; off <<= 1) {
       uint32_t y = __shfl_up(x, off);
@@ -683,7 +714,7 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint3
       all += t;
     }
     uint32_t run = carry + before + x - mine;
-    if (i0 + kScanPer <= ntiles) {
+    if (i0 + kScanPer <= hi) {
       uint4* q = (uint4*)(tile_count + i0);
 #pragma unroll
       for (int k = 0; k < kScanPer / 4; k++) {
@@ -701,14 +732,14 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint3
     } else {
 #pragma unroll
       for (int k = 0; k < kScanPer; k++) {
-        if (i0 + k < ntiles) tile_count[i0 + k] = run;
+        if (i0 + k < hi) tile_count[i0 + k] = run;
         run += v[k];
       }
     }
     carry += all;
     __syncthreads();
   }
-  if (threadIdx.x == 0) *nactive = carry;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *nactive = carry;
 }
 
 // marked sample ids -> active[] in sample order: one wave per tile, lanes
@@ -761,7 +792,12 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* hand, cons
 // waves take more.  One atomic per 64 samples; counters sharing one cache line
 // had serialised the claims (~100 ns each).
 constexpr int kQueueStride = 64;  // u32 between counters: 256 bytes
-constexpr int kPartWords = kPartSlots * C_COUNT + kQueues * kQueueStride / 2;  // u64 words of d_part
+// d_part (u64 words): the partial counter slots, the claim counters, then the
+// tile scan's group totals (kScanGroups u32)
+constexpr int kPartWords = kPartSlots * C_COUNT + kQueues * kQueueStride / 2 + kScanGroups / 2;
+__host__ __device__ inline uint32_t* scan_group_sums(unsigned long long* part) {
+  return (uint32_t*)(part + (size_t)kPartSlots * C_COUNT + kQueues * kQueueStride / 2);
+}
 constexpr uint32_t kNoChunk = 0xffffffffu;
 struct ChunkClaim {
   uint32_t* ctr;  // kQueues claim counters, kQueueStride apart
@@ -1893,7 +1929,12 @@ static hipError_t launch_pass_on(hipStream_t stream, int num_cu, unsigned long l
   if (evs && (e = hipEventRecord(evs[0], stream)) != hipSuccess) return e;
   NM_LAUNCH_OK(hipLaunchKernelGGL(start, dim3(gtile), dim3(kThreads), 0, stream, args));
   if (evs && (e = hipEventRecord(evs[1], stream)) != hipSuccess) return e;
-  NM_LAUNCH_OK(hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, c.tile_count, ntiles, nactive));
+  const ScanSplit sp = scan_split(ntiles);
+  uint32_t* gsum = scan_group_sums(part);
+  NM_LAUNCH_OK(hipLaunchKernelGGL(k_tile_sums, dim3(sp.groups), dim3(1024), 0, stream, (const uint32_t*)c.tile_count,
+                                  ntiles, sp.per, gsum));
+  NM_LAUNCH_OK(hipLaunchKernelGGL(k_scan_tiles, dim3(sp.groups), dim3(1024), 0, stream, c.tile_count, ntiles, sp.per,
+                                  (const uint32_t*)gsum, nactive));
   NM_LAUNCH_OK(hipLaunchKernelGGL(k_scatter, dim3(gtile), dim3(kThreads), 0, stream, (const uint64_t*)c.hand,
                                   (const uint32_t*)c.tile_count, c.total, c.active));
   if (evs && (e = hipEventRecord(evs[2], stream)) != hipSuccess) return e;

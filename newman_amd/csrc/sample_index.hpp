@@ -92,6 +92,29 @@ __host__ __device__ inline uint32_t hand_position(uint32_t tile_off, uint32_t be
   return tile_off + before + (uint32_t)__builtin_popcountll(lower);
 }
 
+// ---- the exclusive scan of the per-tile counts: rounds of kScanRound counts
+// (one 1024-thread workgroup pass), `per` consecutive rounds to each of `groups`
+// workgroups (at most kScanGroups); group g covers counts [g per kScanRound,
+// min((g + 1) per kScanRound, ntiles)), and every group has at least one count
+// when ntiles > 0 (ntiles = 0: one group, no counts)
+constexpr uint32_t kScanRound = 16384;
+constexpr uint32_t kScanGroups = 64;
+struct ScanSplit {
+  uint32_t per, groups;
+};
+__host__ __device__ inline ScanSplit scan_split(uint32_t ntiles) {
+  const uint32_t rounds = ntiles ? (ntiles - 1) / kScanRound + 1 : 1u;
+  const uint32_t per = (rounds - 1) / kScanGroups + 1;
+  return {per, (rounds - 1) / per + 1};
+}
+__host__ __device__ inline void scan_group_range(uint32_t ntiles, uint32_t per, uint32_t g, uint32_t& lo,
+                                                 uint32_t& hi) {
+  const uint64_t span = (uint64_t)per * kScanRound, n = ntiles;
+  const uint64_t l = (uint64_t)g * span;
+  lo = (uint32_t)(l < n ? l : n);
+  hi = (uint32_t)(l + span < n ? l + span : n);
+}
+
 // ---- the active list (sample ids that iterate) as the iterate kernels read it
 constexpr int kChunk = 64;   // active samples per chunk (one per lane)
 constexpr int kQueues = 8;   // claim ranges of the chunks

@@ -315,6 +315,25 @@ def test_frame_batch_equals_single_renders(gpu):
         assert np.array_equal(batch[k]["smoothing"], one[0]["smoothing"])
 
 
+def test_plain_batch_over_two_tile_scan_groups(gpu):
+    """300 frames of 240 x 240 = 17,280,000 samples = 16,875 start tiles: the tile
+    scan splits its counts into two workgroups' ranges (16,384 and 491 counts),
+    the second starting at the first's total; every frame equals its single-frame
+    render (one range)."""
+    import newman_amd as nm
+    m = nm.Mandelbrot(240, 240)
+    ws = ["3.0", "0.75", "1.5"]
+    m.set_view(300, "-0.75", "0.1", "3.0")
+    batch, st = m.render_frames([ws[k % 3] for k in range(300)])
+    one = []
+    for w in ws:
+        m.set_view(300, "-0.75", "0.1", w)
+        m.render()
+        one.append(m.grid().copy())
+    for k in range(300):
+        assert np.array_equal(batch[k], one[k % 3]), k
+
+
 # 48 x 64 = 3 start tiles of 1024 samples per frame; 37 x 29 = 1073 samples, so
 # start tiles straddle frames (the per-sample frame lookup) and the widths differ
 @pytest.mark.parametrize("nr,nc", [(48, 64), (37, 29)])

@@ -1,7 +1,8 @@
 """Every index the render kernels form, walked on the host: newman_amd/csrc/
 index_check.cpp runs each thread of the start launches (plain and series), the
-emit kernel and the iterate kernels' active-list chunks through the kernels' own
-index code (csrc/sample_index.hpp) and checks each index against the length of
+tile scan's two launches, the emit kernel and the iterate kernels' active-list
+chunks through the kernels' own index code (csrc/sample_index.hpp) and checks
+each index against the length of
 the buffer it addresses, that the tiles' one-frame shortcut names each sample's
 frame, that every pixel of every frame is written exactly once and that every
 active-list entry is taken exactly once.  The shapes include the 2 x 2 frame of
@@ -58,11 +59,12 @@ def test_every_kernel_index_is_in_bounds(case):
     assert n > 0
 
 
-@pytest.mark.parametrize("plant", [1, 2, 3, 4])
+@pytest.mark.parametrize("plant", [1, 2, 3, 4, 5])
 def test_the_checker_finds_planted_mistakes(plant):
     """Negative controls of the checker itself: a column-table offset one entry
     on, frames one sample longer than their sample order, plain chunks
-    interleaved with one chunk too few, and scatter positions that count a
-    lane's own bit are each reported."""
+    interleaved with one chunk too few, scatter positions that count a lane's
+    own bit, and a tile scan launched with one group too few are each
+    reported."""
     v, msg, _ = _check(3, 7, 5, 16, 7, 1, 8, plant=plant)
     assert v > 0 and msg
