@@ -745,15 +745,32 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* tile_count, uint3
 // marked sample ids -> active[] in sample order: one wave per tile, lanes
 // 0..15 holding its 16 hand-on words (one load each) and their exclusive prefix
 // of set bits; then for each word (scalar) every lane writes its own sample's id
-// at hand_position (consecutive lanes, consecutive positions).
+// at hand_position (consecutive lanes, consecutive positions).  A wave's next
+// tile (words and offset) is loaded before it writes the current one, so the
+// loads' latency overlaps the stores.
 static_assert(kHandWordsPerTile == 16, "k_scatter: 16 words per tile, one per lane of lanes 0..15");
+__device__ __forceinline__ void scatter_load(const uint64_t* hand, const uint32_t* tile_off, uint32_t tile,
+                                             uint32_t ntiles, uint32_t lane, uint64_t& w, uint32_t& off) {
+  w = 0ull;
+  off = 0u;
+  if (tile < ntiles) {  // wave-uniform
+    if (lane < (uint32_t)kHandWordsPerTile) w = hand[tile * kHandWordsPerTile + lane];
+    off = tile_off[tile];
+  }
+}
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* hand, const uint32_t* tile_off, uint32_t total,
                                                       uint32_t* active) {
   const uint32_t ntiles = start_tiles(total);
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t waves = gridDim.x * (kThreads / 64);
-  for (uint32_t tile = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); tile < ntiles; tile += waves) {
-    const uint64_t w = lane < (uint32_t)kHandWordsPerTile ? hand[tile * kHandWordsPerTile + lane] : 0ull;
+  uint32_t tile = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  uint64_t wnext;
+  uint32_t offnext;
+  scatter_load(hand, tile_off, tile, ntiles, lane, wnext, offnext);
+  for (; tile < ntiles; tile += waves) {
+    const uint64_t w = wnext;
+    const uint32_t off = __builtin_amdgcn_readfirstlane(offnext);
+    scatter_load(hand, tile_off, tile + waves, ntiles, lane, wnext, offnext);
     const uint32_t n = (uint32_t)__popcll(w);
     uint32_t x = n;  // inclusive scan over lanes 0..15
     for (int o = 1; o < kHandWordsPerTile; o <<= 1) {
@@ -761,7 +778,6 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* hand, cons
       if (lane >= (uint32_t)o) x += y;
     }
     const uint32_t before = x - n;
-    const uint32_t off = __builtin_amdgcn_readfirstlane(tile_off[tile]);
 #pragma unroll 4
     for (int k = 0; k < kHandWordsPerTile; k++) {
       const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)w, k), hi = __builtin_amdgcn_readlane((uint32_t)(w >> 32), k);
