@@ -27,14 +27,15 @@
 //               (series path: stop index, series skip, escape inside the skip,
 //               first step); resolves what it can, marks the rest (one ballot
 //               word per wave), counts per tile
-//   scan        exclusive scan of the tile counts (one workgroup)
+//   scan        exclusive scan of the tile counts (range totals, then each
+//               range's scan from the earlier ranges' sum; up to 64 workgroups)
 //   scatter     compacts the marked sample ids into active[] in sample order
 //   *_iterate   persistent waves, one sample per lane; blocks of K steps; the
 //               first escape of a lane in a block records its step and |z|^2
 //               (FP32X: and poisons the lane with NaN); lanes are refilled (ballot +
-//               popcount ranks) from 64-sample chunks of active[], chunk c owned by
-//               wave c mod W, with the next chunk prefetched (plain: its states;
-//               series: its ids); the series kernel keeps the orbit's head and
+//               popcount ranks) from 64-sample chunks of active[], which waves
+//               claim from 8 claim ranges, with the next chunk prefetched (plain:
+//               its states; series: its ids); the series kernel keeps the orbit's head and
 //               tail windows in LDS
 //   finalize    smoothing of the iterated samples from r2
 //   emit        tiled sample orders only: results into the row-major grids
