@@ -62,7 +62,8 @@ extern "C" {
 // own negative controls, 0 for none): 1 moves the last frame's column tables
 // one entry on, 2 gives the frames one sample more than the sample order has,
 // 3 interleaves the plain chunks with one chunk too few, 4 counts a lane's own
-// bit into its scatter position, 5 launches the tile scan with one group too few.
+// bit into its scatter position, 5 launches the tile scan with one group too few,
+// 6 selects the next set bit in the sparse scatter.
 long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int mask, int skip_T, int plant,
                          char* msg, int cap, unsigned long long* checked) {
   using namespace nm;
@@ -140,7 +141,13 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
         }
     }
 
-  // ---- k_scatter over hand-on words with pseudo-random bits on valid ids
+  // ---- k_scatter over hand-on words with pseudo-random bits on valid ids: tiles
+  // in turn about 3 in 8, 1 in 16 and 1 in 64 marked, and none (the kernel's
+  // dense and sparse paths)
+  auto marked = [](uint32_t gid, uint64_t x) {
+    static const uint64_t dens[4] = {24, 4, 1, 0};
+    return (x & 63) < dens[(gid / kTile) % 4];
+  };
   {
     const uint32_t nwords = hand_words(total);
     std::vector<uint64_t> words(nwords, 0);
@@ -149,7 +156,7 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
       x ^= x << 13;
       x ^= x >> 7;
       x ^= x << 17;
-      if ((x & 7) < 3) words[hand_word(gid)] |= 1ull << (gid % 64);  // about 3 in 8 marked
+      if (marked(gid, x)) words[hand_word(gid)] |= 1ull << (gid % 64);
     }
     std::vector<uint32_t> off(ntiles, 0);
     uint64_t count = 0;
@@ -173,6 +180,47 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
           at[pos] = gid;
         }
         before += (uint32_t)__builtin_popcountll(w);
+      }
+    }
+    // the sparse-tile path (tiles with at most kScatterSparse marked samples):
+    // lane j of batch p0 takes entry p = p0 + j, the last word k whose prefix
+    // is <= p (the kernel's 4-step search) and bit select_bit(w_k, p - before_k)
+    for (uint32_t t = 0; t < ntiles; t++) {
+      uint32_t pre[kHandWordsPerTile], m = 0;
+      for (int k = 0; k < kHandWordsPerTile; k++) {
+        pre[k] = m;
+        m += (uint32_t)__builtin_popcountll(words[(uint64_t)t * kHandWordsPerTile + k]);
+      }
+      if (m > kScatterSparse) continue;
+      for (uint32_t p0 = 0; p0 < m; p0 += 64)
+        for (uint32_t j = 0; j < 64; j++) {
+          const uint32_t p = p0 + j;
+          int k = 0;
+          for (int step = kHandWordsPerTile / 2; step >= 1; step >>= 1)
+            if (pre[k + step] <= p) k += step;
+          if (p >= m) continue;
+          const uint64_t wk = words[(uint64_t)t * kHandWordsPerTile + k];
+          const uint32_t gid = t * kTile + (uint32_t)k * 64 + select_bit(wk, p - pre[k] + (plant == 6 ? 1u : 0u));
+          if (!R.in((uint64_t)off[t] + p, count, "active (sparse scatter)", gid)) continue;
+          if (at[off[t] + p] != (int64_t)gid)
+            R.fail("sparse scatter: tile %u entry %u takes sample %u, want %lld", t, p, gid, (long long)at[off[t] + p]);
+        }
+    }
+    // select_bit against a bit-by-bit walk
+    {
+      uint64_t y = 0x2545F4914F6CDD1Dull ^ total;
+      for (int trial = 0; trial < 2000; trial++) {
+        y ^= y << 13;
+        y ^= y >> 7;
+        y ^= y << 17;
+        const uint64_t w = trial < 3 ? (trial == 0 ? 1ull : trial == 1 ? ~0ull : 1ull << 63) : y & (y >> (trial % 7));
+        uint32_t r = 0;
+        for (uint32_t b = 0; b < 64; b++)
+          if ((w >> b) & 1ull) {
+            if (select_bit(w, r) != b) R.fail("select_bit(%llx, %u) = %u, want %u", (unsigned long long)w, r,
+                                              select_bit(w, r), b);
+            r++;
+          }
       }
     }
     for (uint64_t p = 0; p < count; p++) {
@@ -199,7 +247,7 @@ long long nm_index_check(int nr, int nc, int nframes, int tile, int num_cu, int 
         x ^= x << 13;
         x ^= x >> 7;
         x ^= x << 17;
-        if ((x & 7) < 3) cnt[gid / kTile]++;
+        if (marked(gid, x)) cnt[gid / kTile]++;
       }
     }
     for (uint32_t t = 0; t < ntiles; t++) {

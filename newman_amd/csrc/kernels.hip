@@ -779,6 +779,21 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t* hand, cons
       if (lane >= (uint32_t)o) x += y;
     }
     const uint32_t before = x - n;
+    const uint32_t m = __builtin_amdgcn_readlane(x, kHandWordsPerTile - 1);  // the tile's marked samples
+    if (m <= kScatterSparse) {  // wave-uniform: one store per 64 marked samples
+      const uint32_t wlo = (uint32_t)w, whi = (uint32_t)(w >> 32);
+      for (uint32_t p0 = 0; p0 < m; p0 += 64) {
+        const uint32_t p = p0 + lane;
+        int k = 0;  // the last word whose prefix is <= p (lanes 0..15 hold the prefixes)
+#pragma unroll
+        for (int step = kHandWordsPerTile / 2; step >= 1; step >>= 1)
+          if ((uint32_t)__shfl((int)before, k + step) <= p) k += step;
+        const uint32_t bk = (uint32_t)__shfl((int)before, k);
+        const uint64_t wk = ((uint64_t)(uint32_t)__shfl((int)whi, k) << 32) | (uint32_t)__shfl((int)wlo, k);
+        if (p < m) active[off + p] = tile * kTile + (uint32_t)k * 64 + select_bit(wk, p - bk);
+      }
+      continue;
+    }
 #pragma unroll 4
     for (int k = 0; k < kHandWordsPerTile; k++) {
       const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)w, k), hi = __builtin_amdgcn_readlane((uint32_t)(w >> 32), k);

@@ -92,6 +92,25 @@ __host__ __device__ inline uint32_t hand_position(uint32_t tile_off, uint32_t be
   return tile_off + before + (uint32_t)__builtin_popcountll(lower);
 }
 
+// k_scatter on a sparse tile (at most kScatterSparse marked samples): lane j
+// writes active-list entries off + j, off + j + 64, ...; entry p is bit
+// select_bit(w_k, p - before_k) of the last word k with before_k <= p (the
+// words' exclusive prefix of set bits), which is never an empty word
+constexpr uint32_t kScatterSparse = 256;
+// the position of set bit r (0-based) of w; r < popcount(w)
+__host__ __device__ inline uint32_t select_bit(uint64_t w, uint32_t r) {
+  uint32_t pos = 0;
+  for (uint32_t width = 32; width >= 1; width >>= 1) {
+    const uint32_t c = (uint32_t)__builtin_popcountll(w & ((1ull << width) - 1ull));
+    if (r >= c) {
+      r -= c;
+      w >>= width;
+      pos += width;
+    }
+  }
+  return pos;
+}
+
 // ---- the exclusive scan of the per-tile counts: rounds of kScanRound counts
 // (one 1024-thread workgroup pass), `per` consecutive rounds to each of `groups`
 // workgroups (at most kScanGroups); group g covers counts [g per kScanRound,

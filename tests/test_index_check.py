@@ -59,12 +59,12 @@ def test_every_kernel_index_is_in_bounds(case):
     assert n > 0
 
 
-@pytest.mark.parametrize("plant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("plant", [1, 2, 3, 4, 5, 6])
 def test_the_checker_finds_planted_mistakes(plant):
     """Negative controls of the checker itself: a column-table offset one entry
     on, frames one sample longer than their sample order, plain chunks
     interleaved with one chunk too few, scatter positions that count a lane's
-    own bit, and a tile scan launched with one group too few are each
-    reported."""
+    own bit, a tile scan launched with one group too few, and a sparse-tile
+    scatter that selects the next set bit are each reported."""
     v, msg, _ = _check(3, 7, 5, 16, 7, 1, 8, plant=plant)
     assert v > 0 and msg
