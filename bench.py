@@ -587,7 +587,8 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="sweep block size and waves per SIMD first (series: then the chunk interleaving)")
     ap.add_argument("--chunk-run", type=int, default=0,
-                    help="series path: active-list chunks of 64 consecutive samples (0) or runs of 4, 8, 16, 32")
+                    help="series path: active-list chunks of 64 consecutive samples (64) or interleaved runs of "
+                         "1, 2, 4, 8, 16, 32; 0 = the default (runs of 4 for a floatexp series, else 64)")
     ap.add_argument("--cpu-rows-every", type=int, default=36)
     ap.add_argument("--cpu-threads", type=int, default=16)
     args = ap.parse_args()
