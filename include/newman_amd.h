@@ -191,7 +191,7 @@ typedef struct nm_launch {
   int32_t chunk_run; /* series path: each wave takes 64 consecutive samples of the active list (64)
                         or 64 / R runs of R consecutive samples spread over the list (R = 1, 2, 4, 8,
                         16, 32): spreads deep views' long samples over the waves; 0 = the default,
-                        runs of 4 when FP64 deltas follow a floatexp series, else 64 */
+                        runs of 4 when the deltas follow a floatexp series, else 64 */
   double ms;
 } nm_launch;
 int nm_set_launch(nm_mandelbrot* h, const nm_launch* l);

@@ -114,7 +114,7 @@ struct RenderOpts {
   int pre_iter = 0;       // plain path: updates in the start kernel (0 = default)
   int chunk_run = 0;      // series path: active-list chunks of 64 consecutive samples (64)
                           // or interleaved runs of 1, 2, 4, 8, 16, 32 (chunk_run_shift);
-                          // 0: runs of 4 for FP64 deltas on a floatexp series, else 64
+                          // 0: runs of 4 for deltas on a floatexp series, else 64
   const int* tile_ua = nullptr;  // series path: per-tile first unstable index (null: per sample)
   int skip_T = 0;                // its tile side
   uint64_t set_serial = 0; // identifies the frame set: the engine keeps its tables
