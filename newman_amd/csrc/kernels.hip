@@ -1834,31 +1834,38 @@ void k_series_iterate(SeriesArgs a) {
 // thread per entry of active[]
 // tiled sample order: results (sample order) -> the row-major grids
 __global__ __launch_bounds__(kThreads) void k_emit(Common c, EscapeValue* grid) {
-  for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < c.total; gid += gridDim.x * blockDim.x) {
+  // 64-bit loop index: a total near 2^32 must not wrap the stride
+  for (uint64_t g = blockIdx.x * blockDim.x + threadIdx.x; g < c.total; g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t gid = (uint32_t)g;
     const DevFrame& f = c.frames[frame_of(gid, c.inv_frame)];
     const Loc lc = locate(f, gid - f.sample_base);
     if (lc.valid) grid[f.out_base + (uint64_t)lc.r * f.nc + lc.c] = c.out[gid];
   }
 }
 
+constexpr int kFinalizePer = 8;
 __global__ __launch_bounds__(kThreads) void k_finalize(Common c) {
   const uint32_t total = *c.nactive;
   const uint32_t stride = gridDim.x * blockDim.x;
   unsigned long long credited = 0;  // the iterated samples' counts (the start kernel adds the rest)
-  // 4 samples per thread per round, their loads issued together
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += 4 * stride) {
-    uint32_t pix[4];
-    int it[4];
-    double r2[4];
+  // kFinalizePer samples per thread per round, their loads issued together
+  // (64-bit entry indices: a total near 2^32 must not wrap the stride)
+  for (uint64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)kFinalizePer * stride) {
+    uint32_t pix[kFinalizePer];
+    int it[kFinalizePer];
+    double r2[kFinalizePer];
 #pragma unroll
-    for (int k = 0; k < 4; k++) pix[k] = i + k * stride < total ? c.active[i + k * stride] : c.total;
+    for (int k = 0; k < kFinalizePer; k++) {
+      const uint64_t e = i + (uint64_t)k * stride;
+      pix[k] = e < total ? c.active[e] : c.total;
+    }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < kFinalizePer; k++) {
       it[k] = pix[k] < c.total ? c.out[pix[k]].iterations : c.N;
       r2[k] = pix[k] < c.total ? c.r2[pix[k]] : 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < kFinalizePer; k++) {
       if (pix[k] >= c.total) continue;
       credited += (unsigned long long)it[k];
       if (it[k] < c.N) c.out[pix[k]].smoothing = smooth_of(r2[k], c);
