@@ -675,7 +675,7 @@ def measure(config, args, block_k=None, waves=None):
         "data": "synthetic (fixed view)" if config == "c1" else "synthetic (seeded zoom-walk view)",
         "config": dict(desc, parallelism="single GPU", block_k=s["block_k"], waves_per_simd=s["waves_per_simd"],
                        tile=s["tile"], graph=s["graph"], pre_iter=m.get_launch()["pre_iter"],
-                       chunk_run=m.get_launch()["chunk_run"]),
+                       chunk_run=s["chunk_run"]),
         "roofline": roofline(config, flop, s, fxs=desc.get("series") == "floatexp"),
         "detail": {"credited_px_it": credited, "executed_px_it": s["executed"],
                    "executed_in_start_kernel": s["executed_start"], "skipped": s["skipped"],

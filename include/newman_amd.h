@@ -68,6 +68,8 @@ typedef struct nm_render_stats {
   int32_t grouped;       /* plain path: 1 when the escape test ran once per group of four updates, the first
                             escaping update found from the group's earlier |z|^2 (bailout radius 2 to 2^30 and
                             every |c| <= 0.999 (R^2 - R): an escape at one update implies one at the next) */
+  int32_t chunk_run;     /* the iterate kernel's chunk order used: 64 = 64 consecutive samples of the active
+                            list per chunk, R < 64 = runs of R (nm_launch.chunk_run; the plain path: 1) */
 } nm_render_stats;
 
 typedef struct nm_mandelbrot nm_mandelbrot; /* class Mandelbrot (mandelbrot.h:7) */

@@ -50,7 +50,7 @@ class RenderStats(C.Structure):
                 ("ms_iterate", C.c_double), ("ms_total", C.c_double), ("block_k", C.c_int32),
                 ("waves_per_simd", C.c_int32), ("executed_start", C.c_uint64), ("ms_compact", C.c_double),
                 ("ms_finalize", C.c_double), ("graph", C.c_int32), ("tile", C.c_int32),
-                ("fma2", C.c_int32), ("grouped", C.c_int32)]
+                ("fma2", C.c_int32), ("grouped", C.c_int32), ("chunk_run", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -82,6 +82,7 @@ static void copy_stats(const nm::RenderStats& s, nm_render_stats* o) {
   o->tile = s.tile;
   o->fma2 = s.fma2;
   o->grouped = s.grouped;
+  o->chunk_run = s.chunk_run;
 }
 
 extern "C" {

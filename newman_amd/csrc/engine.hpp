@@ -140,6 +140,7 @@ struct RenderStats {
   int tile = 0;              // the sample order used
   int fma2 = 0;              // plain path: the folded update ran (plain_update)
   int grouped = 0;           // plain path: updates were tested in groups of four (grouped_test_ok)
+  int chunk_run = 0;         // the iterate kernel's chunk order used: 64 consecutive samples, or runs of R
   int block_k = 0, waves_per_simd = 0;
 };
 

@@ -2396,6 +2396,7 @@ class EngineImpl : public Engine {
     if (stats) {
       stats->block_k = K;
       stats->waves_per_simd = waves;
+      stats->chunk_run = 1;
     }
     // 256-thread workgroups = 4 waves = one per SIMD: num_cu * waves workgroups
     // the folded update (plain_update) needs every frame's coordinates clear of
@@ -2533,6 +2534,7 @@ class EngineImpl : public Engine {
     if (stats) {
       stats->block_k = K;
       stats->waves_per_simd = waves;
+      stats->chunk_run = 1 << a.run_shift;
     }
     auto go = [&](auto iter) {
       return a.fxs ? run(a, k_series_start<true>, iter, waves, opt, stats, nout, host_out, emit_to)

@@ -205,16 +205,18 @@ def test_series_block_size_keeps_results(gpu, name, nr, nc, width, K):
     assert _smooth_ok(g["smoothing"], ref["smoothing"]).all()
 
 
-@pytest.mark.parametrize("run", [1, 4, 16, 64])
+@pytest.mark.parametrize("run", [0, 1, 4, 16, 64])
 def test_series_chunk_interleaving_keeps_results(gpu, run):
     """The series kernel's chunk interleaving only changes which wave takes a
-    sample, so every run length gives the oracle's grid bit for bit."""
+    sample, so every run length gives the oracle's grid bit for bit; the render
+    reports the order it used (0: the default, runs of 4 or 64 samples)."""
     m, o = _deep(*DEEP_CASES[4])
     m.precompute()
     o.precompute()
     m.set_launch(chunk_run=run)
-    m.render()
+    st = m.render()
     assert m.get_launch()["chunk_run"] == run
+    assert st["chunk_run"] == run if run else st["chunk_run"] in (4, 64)
     nr, nc = DEEP_CASES[4][1], DEEP_CASES[4][2]
     g = m.grid()
     rc = np.array([(r, c) for r in range(nr) for c in range(nc)], dtype=np.int32)
